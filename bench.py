@@ -1,0 +1,183 @@
+"""Benchmark: the production day of per-column soil solves on one MI355X.
+
+Workload (one "step" = one 1800 s model step of every column): 1,048,576 uneven
+columns (0..12 snow layers, bedrock index 5..20, water table 0.5..10 m), each
+running the snow+soil heat step then the moisture-form Richards step with a
+water-table lower boundary, FP64, synthetic inputs already resident in HBM.
+Prints one JSON line: column-steps/s, the HBM roofline of the dominant kernel,
+and the CPU oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "clm.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "column-steps/s @1M cols FP64 soil solves; HBM GB/s frac; learned vs plain dt"
+
+
+def algorithmic_bytes(nb, snl, bc_watertable=True):
+    """Bytes per column-step, each input read once and each output written once
+    (FP64 / Int64): Richards 8*(12n+4) (+8*(n+2) for zwt and zi under
+    BC_WATERTABLE); heat 8*(9L + 3ns + 198), L = ns + 25."""
+    import numpy as np
+    n = nb.astype(np.float64)
+    ns = (-snl).astype(np.float64)
+    br = 8.0 * (12.0 * n + 4.0) + (8.0 * (n + 2.0) if bc_watertable else 0.0)
+    L = ns + 25.0
+    bh = 8.0 * (9.0 * L + 3.0 * ns + 198.0)
+    return float(br.sum()), float(bh.sum())
+
+
+def make_state(nc, seed):
+    import torch
+    from clmsoil import api, inputs
+    t0 = time.time()
+    d = inputs.production_day(nc=nc, seed=seed, fill=0.0)
+    outs = inputs.heat_outputs(d, diagnostics=False, fill=0.0)
+    d.update(outs)
+    host = d
+    dev = api.to_device(d)
+    torch.cuda.synchronize()
+    return host, dev, time.time() - t0
+
+
+def gpu_step(dev, nc, cfg, heat_keys, events=None):
+    from clmsoil import api
+    if events:
+        events[0].record()
+    api.soil_temperature({k: dev[k] for k in heat_keys} | {"nc": nc}, 1800.0)
+    if events:
+        events[1].record()
+    st = api.MfState(dev["h2osoi_liq"], dev["smp_l"], dev["hk_l"], dev["qcharge"], dev["nsubsteps"])
+    mi = api.MfIn(dev["z"], dev["zi"], dev["dz"], dev["qflx_rootsoi"], dev["watsat"], dev["hksat"],
+                  dev["bsw"], dev["sucsat"], dev["icefrac"], dev["zwt"], dev["qflx_infl"], dev["smpmin"])
+    api.soilwater_moisture_form(st, mi, None, dev["nbedrock"], 12, 20, 25, 1800.0, cfg)
+    if events:
+        events[2].record()
+
+
+HEAT_KEYS = ["snl", "nbedrock", "z", "dz", "zi", "t_soisno", "h2osoi_liq", "h2osoi_ice",
+             "excess_ice", "watsat", "tkmg", "tkdry", "csol", "bsw", "sucsat", "frac_sno_eff",
+             "frac_h2osfc", "hs_top_snow", "hs_soil", "hs_top", "hs_h2osfc", "dhsdT",
+             "sabg_lyr_col", "eflx_bot", "t_h2osfc", "h2osfc", "h2osno_no_layers", "snow_depth",
+             "int_snow", "t_grnd", "xmf", "xmf_h2osfc", "qflx_snomelt", "qflx_snofrz",
+             "qflx_snow_drain", "qflx_h2osfc_to_ice", "eflx_fgr12", "qflx_snomelt_lyr",
+             "qflx_snofrz_lyr"]
+
+
+def cpu_baseline(host, sample, steps):
+    """The oracle (a C restatement of CLM.jl's CPU path, OpenMP over columns) on
+    the first `sample` columns for `steps` steps, best of 3."""
+    import numpy as np
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle
+    from clmsoil import bind
+    sub = {}
+    for k, v in host.items():
+        if isinstance(v, np.ndarray):
+            sub[k] = np.ascontiguousarray(v[..., :sample])
+        else:
+            sub[k] = v
+    sub["nc"] = sample
+    cfg = bind.swm_config(lower_bc=3)
+    hcfg = bind.heat_config()
+    best = float("inf")
+    for _ in range(3):
+        s = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in sub.items()}
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            oracle.soil_temperature(s, hcfg)
+            oracle.soilwater_moisture_form(s, cfg)
+        best = min(best, time.perf_counter() - t0)
+    return sample * steps / best, oracle.num_threads(), best
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--nc", type=int, default=1 << 20)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the per-column solves need no exchange between "
+              f"GPUs, so --gpus {args.gpus} is not run (columns shard across GPUs unchanged).",
+              file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+    from clmsoil import api
+
+    torch.cuda.set_device(0)
+    nc = args.nc
+    host, dev, gen_s = make_state(nc, args.seed)
+    print(f"# state ready: {nc} columns in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_WATERTABLE)
+    for _ in range(args.warmup):
+        gpu_step(dev, nc, cfg, HEAT_KEYS)
+    torch.cuda.synchronize()
+    assert api.device_errors() == 0
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for k in range(args.steps):
+        gpu_step(dev, nc, cfg, HEAT_KEYS, ev[k])
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = start.elapsed_time(end)
+    heat_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps
+    rich_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps
+    assert api.device_errors() == 0
+    value = nc * args.steps / (total_ms / 1e3)
+    nsub = dev["nsubsteps"].float().mean().item()
+    b_r, b_h = algorithmic_bytes(host["nbedrock"], host["snl"])
+    kern = {"richards": (rich_ms, b_r), "heat": (heat_ms, b_h)}
+    dom = max(kern, key=lambda k: kern[k][0])
+    ms, by = kern[dom]
+    achieved = by / (ms / 1e3) / 1e9
+    out = {
+        "metric": METRIC, "value": value, "unit": "column-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "production day: 1,048,576 uneven columns (0-12 snow layers, "
+                               "nbedrock 5-20, zwt 0.5-10 m), heat step + moisture-form Richards "
+                               "(BC_WATERTABLE) per 1800 s step" if nc == 1 << 20 else
+                   f"production day, {nc} columns",
+                   "columns": nc, "layers": "12 snow + 25 ground (20 soil)",
+                   "step_seconds": 1800.0, "tuning": list(api.get_tuning())},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "kernels_ms": {"heat": heat_ms, "richards": rich_ms},
+        "fused_step_hbm_frac": (b_r + b_h) / (total_ms / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "mean_substep_iterations": nsub,
+        "wall_s": wall,
+    }
+    if not args.no_cpu_baseline:
+        v, cores, secs = cpu_baseline(host, min(args.cpu_sample, nc), args.cpu_steps)
+        out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"first {min(args.cpu_sample, nc)} columns x {args.cpu_steps} "
+                                         f"steps of the same workload, best of 3 ({secs:.2f} s)"}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,134 @@
+"""ctypes mirror of include/clmsoil.h and the loader of the HIP library.
+
+The structs here are byte-for-byte the C structs of the ABI; `load_library()`
+opens the in-tree `libclmsoil.so` (built from `clm.jl_amd/csrc`) and fails
+loudly when it is missing — there is no CPU fallback in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libclmsoil.so"
+
+# status codes
+OK, ERR_NULL, ERR_BC, ERR_METHOD, ERR_SIZE, ERR_HIP, ERR_WORKSPACE, ERR_ARG, ERR_NOT_TRAINED = range(9)
+
+# soil_water_movement.jl:23-34
+ZENGDECKER_2009, MOISTURE_FORM, MIXED_FORM, HEAD_FORM = 0, 1, 2, 3
+BC_HEAD, BC_FLUX, BC_ZERO_FLUX, BC_WATERTABLE, BC_AQUIFER = 0, 1, 2, 3, 4
+FLUX_INEXPENSIVE, FLUX_EXPENSIVE = 1, 42
+
+_p = C.c_void_p
+
+
+class Levels(C.Structure):
+    _fields_ = [("nlevsno", C.c_int32), ("nlevsoi", C.c_int32),
+                ("nlevgrnd", C.c_int32), ("reserved", C.c_int32)]
+
+
+class SwmConfig(C.Structure):
+    _fields_ = [("soilwater_movement_method", C.c_int32),
+                ("upper_boundary_condition", C.c_int32),
+                ("lower_boundary_condition", C.c_int32),
+                ("flux_calculation", C.c_int32),
+                ("dtmin", C.c_double), ("verySmall", C.c_double),
+                ("xTolerUpper", C.c_double), ("xTolerLower", C.c_double),
+                ("e_ice", C.c_double)]
+
+
+MF_FIELDS = ["mask", "nbedrock", "h2osoi_liq", "smp_l", "hk_l", "qcharge", "nsubsteps",
+             "z", "zi", "dz", "qflx_rootsoi", "watsat", "hksat", "bsw", "sucsat",
+             "icefrac", "zwt", "qflx_infl", "smpmin", "nrejected", "dtsub0"]
+
+
+class MfArgs(C.Structure):
+    _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in MF_FIELDS]
+
+
+HEAT_FIELDS = ["mask", "snl", "nbedrock", "z", "dz", "zi", "t_soisno", "h2osoi_liq",
+               "h2osoi_ice", "excess_ice", "watsat", "tkmg", "tkdry", "csol", "bsw",
+               "sucsat", "frac_sno_eff", "frac_h2osfc", "hs_top_snow", "hs_soil",
+               "hs_top", "hs_h2osfc", "dhsdT", "sabg_lyr_col", "eflx_bot", "t_h2osfc",
+               "h2osfc", "h2osno_no_layers", "snow_depth", "int_snow", "t_grnd", "xmf",
+               "xmf_h2osfc", "qflx_snomelt", "qflx_snofrz", "qflx_snow_drain",
+               "qflx_h2osfc_to_ice", "eflx_fgr12", "qflx_snomelt_lyr", "qflx_snofrz_lyr",
+               "fact", "c_h2osfc", "imelt", "thk", "bw", "eflx_fgr", "exice_subs",
+               "eflx_snomelt", "eflx_snomelt_r", "eflx_h2osfc_to_snow", "snomelt_accum"]
+
+
+class HeatArgs(C.Structure):
+    _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in HEAT_FIELDS]
+
+
+class HeatConfig(C.Structure):
+    _fields_ = [("snow_thermal_cond_method", C.c_int32), ("reserved", C.c_int32)]
+
+
+def declare_common(lib: C.CDLL, prefix: str) -> C.CDLL:
+    """Declare the argument types of the entry points shared by the product
+    library (prefix "clmsoil_") and the oracle (prefix "orc_")."""
+    P = C.c_void_p
+    i64, i32, dbl = C.c_int64, C.c_int32, C.c_double
+    getattr(lib, prefix + "tridiagonal_multi").argtypes = [P, P, P, P, P, P, P, i64, i64, i64] + \
+        ([P] if prefix == "clmsoil_" else [])
+    getattr(lib, prefix + "batched_band_solve").argtypes = [P, P, P, P, P, P, i32, i32, i32, i64, i64, i64] + \
+        ([P] if prefix == "clmsoil_" else [])
+    for f in ("tridiagonal_multi", "batched_band_solve", "soilwater_moisture_form",
+              "soil_temperature"):
+        getattr(lib, prefix + f).restype = C.c_int
+    lib_sw = getattr(lib, prefix + "soilwater_moisture_form")
+    lib_sw.argtypes = [C.POINTER(MfArgs), C.POINTER(Levels), dbl, C.POINTER(SwmConfig)] + \
+        ([P] if prefix == "clmsoil_" else [])
+    lib_st = getattr(lib, prefix + "soil_temperature")
+    lib_st.argtypes = [C.POINTER(HeatArgs), C.POINTER(Levels), dbl, C.POINTER(HeatConfig)] + \
+        ([P] if prefix == "clmsoil_" else [])
+    return lib
+
+
+_LIB = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
+    """Open libclmsoil.so (the HIP library). Raises LibraryMissing if absent."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise LibraryMissing(
+            f"{p} is not built; run `make -C clm.jl_amd/csrc` (or __graft_entry__.build()). "
+            "The clmsoil product path has no CPU fallback.")
+    lib = C.CDLL(str(p))
+    declare_common(lib, "clmsoil_")
+    lib.clmsoil_abi_version.restype = C.c_int
+    lib.clmsoil_strerror.restype = C.c_char_p
+    lib.clmsoil_strerror.argtypes = [C.c_int]
+    lib.clmsoil_device_count.restype = C.c_int
+    lib.clmsoil_sync.argtypes = [C.c_void_p]
+    lib.clmsoil_sync.restype = C.c_int
+    lib.clmsoil_soilwm_moisture_form_solve.argtypes = [C.POINTER(MfArgs), C.POINTER(Levels),
+                                                       C.c_double, C.POINTER(SwmConfig), C.c_void_p]
+    lib.clmsoil_soilwm_moisture_form_solve.restype = C.c_int
+    _declare_extended(lib)
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def _declare_extended(lib: C.CDLL) -> None:
+    """Entry points beyond the reference's interfaces (batched day, adjoint, MLP)."""
+    from . import _abi_ext
+    _abi_ext.declare(lib)
+
+
+def check(lib: C.CDLL, status: int, what: str) -> None:
+    if status != OK:
+        msg = lib.clmsoil_strerror(status).decode() if hasattr(lib, "clmsoil_strerror") else str(status)
+        raise RuntimeError(f"{what}: {msg} (status {status})")

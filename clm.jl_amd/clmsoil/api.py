@@ -1,0 +1,261 @@
+"""Host-side mirror of CLM.jl's interfaces for the soil solves.
+
+Function names, argument meaning and error behaviour follow the reference:
+
+  SoilWaterMovementConfig, init_soilwater_movement, use_aquifer_layer
+      src/biogeophys/soil_water_movement.jl:654-703
+  soilwm_moisture_form_solve      soil_water_movement.jl:1498-1517
+  soilwater_moisture_form         soil_water_movement.jl:1532-1601
+  soil_water                      soil_water_movement.jl:1780-1846
+  soil_temperature                src/biogeophys/soil_temperature.jl:358-593
+  tridiagonal_multi               src/infrastructure/tridiagonal.jl:129-142
+  batched_band_solve              src/infrastructure/band_diagonal.jl:179-189
+
+State arrays are torch tensors on the GPU in CLM.jl's memory layout: a Julia
+`Matrix{Float64}` of size (nc, nlev) is a contiguous tensor of shape (nlev, nc)
+(column index fastest); the 3-D `bmatrix[c, band, row]` is a (nlev, nband, nc)
+tensor. Every call runs on torch's current stream and returns without
+synchronizing. The compute is the HIP library; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+from . import _abi
+from .bind import byref, heat_args, heat_config, levels, mf_args, swm_config
+
+ZENGDECKER_2009, MOISTURE_FORM = _abi.ZENGDECKER_2009, _abi.MOISTURE_FORM
+BC_HEAD, BC_FLUX, BC_ZERO_FLUX, BC_WATERTABLE, BC_AQUIFER = (
+    _abi.BC_HEAD, _abi.BC_FLUX, _abi.BC_ZERO_FLUX, _abi.BC_WATERTABLE, _abi.BC_AQUIFER)
+
+
+class CLMSoilError(RuntimeError):
+    """Raised for invalid inputs (the reference raises `error(...)`)."""
+
+
+@dataclass
+class SoilWaterMovementConfig:
+    """SoilWaterMovementConfig (soil_water_movement.jl:654-671)."""
+    soilwater_movement_method: int = ZENGDECKER_2009
+    upper_boundary_condition: int = BC_FLUX
+    lower_boundary_condition: int = BC_AQUIFER
+    dtmin: float = 60.0
+    verySmall: float = 1.0e-8
+    xTolerUpper: float = 1.0e-1
+    xTolerLower: float = 1.0e-2
+    expensive: int = 42
+    inexpensive: int = 1
+    flux_calculation: int = 1
+    e_ice: float = 6.0
+
+    def to_abi(self):
+        return swm_config(lower_bc=self.lower_boundary_condition,
+                          upper_bc=self.upper_boundary_condition,
+                          method=self.soilwater_movement_method,
+                          flux_calculation=self.flux_calculation, dtmin=self.dtmin,
+                          verySmall=self.verySmall, xTolerUpper=self.xTolerUpper,
+                          xTolerLower=self.xTolerLower, e_ice=self.e_ice)
+
+
+def init_soilwater_movement(*, use_bedrock: bool = False, **kwargs) -> SoilWaterMovementConfig:
+    """init_soilwater_movement (soil_water_movement.jl:681-698)."""
+    cfg = SoilWaterMovementConfig(**kwargs)
+    if (cfg.soilwater_movement_method == ZENGDECKER_2009 and
+            cfg.lower_boundary_condition != BC_AQUIFER):
+        raise CLMSoilError("init_soilwater_movement: ZD09 must use bc_aquifer lower boundary condition")
+    if use_bedrock and cfg.lower_boundary_condition != BC_ZERO_FLUX:
+        raise CLMSoilError("init_soilwater_movement: use_bedrock requires bc_zero_flux lower boundary condition")
+    return cfg
+
+
+def use_aquifer_layer(cfg: SoilWaterMovementConfig) -> bool:
+    """use_aquifer_layer (soil_water_movement.jl:700-703)."""
+    return cfg.lower_boundary_condition in (BC_AQUIFER, BC_WATERTABLE)
+
+
+@dataclass
+class MfState:
+    """MfState (soil_water_movement.jl:1230-1233): read/write column state."""
+    h2osoi_liq: object
+    smp_l: object
+    hk_l: object
+    qcharge: object
+    nsubsteps: object
+    nrejected: object = None
+
+
+@dataclass
+class MfIn:
+    """MfIn (soil_water_movement.jl:1234-1238): read-only inputs."""
+    z: object
+    zi: object
+    dz: object
+    qflx_rootsoi: object
+    watsat: object
+    hksat: object
+    bsw: object
+    sucsat: object
+    icefrac: object
+    zwt: object
+    qflx_infl: object
+    smpmin: object = None
+    dtsub0: object = None
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _lib():
+    return _abi.load_library()
+
+
+def _mf_dict(st: MfState, mfin: MfIn, mask, nbedrock, nlevsno, nlevsoi, nlevgrnd):
+    nc = nbedrock.shape[0]
+    d = dict(nc=nc, nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd, mask=mask,
+             nbedrock=nbedrock)
+    d.update({k: getattr(st, k) for k in ("h2osoi_liq", "smp_l", "hk_l", "qcharge",
+                                          "nsubsteps", "nrejected")})
+    d.update({k: getattr(mfin, k) for k in ("z", "zi", "dz", "qflx_rootsoi", "watsat",
+                                            "hksat", "bsw", "sucsat", "icefrac", "zwt",
+                                            "qflx_infl", "smpmin", "dtsub0")})
+    return d
+
+
+def _check_moisture_form_bc(cfg: SoilWaterMovementConfig):
+    # soil_water_movement.jl:1500-1505
+    if cfg.upper_boundary_condition != BC_FLUX:
+        raise CLMSoilError("soilwm_moisture_form_solve!: upper boundary condition must be BC_FLUX")
+    if cfg.lower_boundary_condition not in (BC_WATERTABLE, BC_FLUX, BC_ZERO_FLUX):
+        raise CLMSoilError("soilwm_moisture_form_solve!: unsupported lower boundary condition")
+
+
+def soilwm_moisture_form_solve(st: MfState, mfin: MfIn, mask, nbedrock, joff: int, joff_zi: int,
+                               nlevsoi: int, dtime: float, cfg: SoilWaterMovementConfig,
+                               nlevgrnd: int | None = None) -> None:
+    """One adaptive-substep moisture-form Richards step for every masked column."""
+    _check_moisture_form_bc(cfg)
+    if joff_zi != joff + 1:
+        raise CLMSoilError("joff_zi must equal joff + 1 (zi has one more level)")
+    c = cfg.to_abi()
+    c.soilwater_movement_method = MOISTURE_FORM
+    d = _mf_dict(st, mfin, mask, nbedrock, joff, nlevsoi, nlevgrnd or nlevsoi + 5)
+    if d["nc"] == 0:
+        return
+    L = _lib()
+    a, lev = mf_args(d), levels(d)
+    _abi.check(L, L.clmsoil_soilwm_moisture_form_solve(byref(a), byref(lev), float(dtime), byref(c),
+                                                       _stream()), "soilwm_moisture_form_solve")
+
+
+def soilwater_moisture_form(st: MfState, mfin: MfIn, mask, nbedrock, nlevsno: int, nlevsoi: int,
+                            nlevgrnd: int, dtime: float, cfg: SoilWaterMovementConfig) -> None:
+    """The solve followed by the aquifer-recharge pass when use_aquifer_layer(cfg)."""
+    _check_moisture_form_bc(cfg)
+    c = cfg.to_abi()
+    c.soilwater_movement_method = MOISTURE_FORM
+    d = _mf_dict(st, mfin, mask, nbedrock, nlevsno, nlevsoi, nlevgrnd)
+    if d["nc"] == 0:
+        return
+    L = _lib()
+    a, lev = mf_args(d), levels(d)
+    _abi.check(L, L.clmsoil_soilwater_moisture_form(byref(a), byref(lev), float(dtime), byref(c),
+                                                    _stream()), "soilwater_moisture_form")
+
+
+def soil_water(st: MfState, mfin: MfIn, mask, nbedrock, nlevsno: int, nlevsoi: int, nlevgrnd: int,
+               cfg: SoilWaterMovementConfig, dtime: float) -> None:
+    """soil_water! dispatch (soil_water_movement.jl:1780-1810). Only the
+    moisture form runs on this backend; Zeng-Decker 2009 raises."""
+    if cfg.soilwater_movement_method == MOISTURE_FORM:
+        soilwater_moisture_form(st, mfin, mask, nbedrock, nlevsno, nlevsoi, nlevgrnd, dtime, cfg)
+    elif cfg.soilwater_movement_method == ZENGDECKER_2009:
+        raise CLMSoilError("soil_water!: the Zeng-Decker 2009 method is not provided by this backend")
+    else:
+        raise CLMSoilError("soil_water!: a SoilWater implementation must be specified!")
+
+
+def soil_temperature(state: dict, dtime: float, nlevsno: int = 12, nlevgrnd: int = 25,
+                     nlevsoi: int = 20, snow_thermal_cond_method: str = "Jordan1991") -> None:
+    """Non-urban soil_temperature! for columns with given ground heat flux terms.
+
+    `state` maps the field names of clmsoil_heat_args (include/clmsoil.h) to
+    tensors; outputs that are absent are not written."""
+    code = {"Jordan1991": 1, "Sturm1997": 2}.get(snow_thermal_cond_method)
+    if code is None:
+        raise CLMSoilError(f"Unknown snow_thermal_cond_method: {snow_thermal_cond_method}")
+    d = dict(state)
+    d.setdefault("nc", d["snl"].shape[0])
+    d.update(nlevsno=nlevsno, nlevgrnd=nlevgrnd, nlevsoi=nlevsoi)
+    if d["nc"] == 0:
+        return
+    L = _lib()
+    a, lev, hc = heat_args(d), levels(d), heat_config(code)
+    _abi.check(L, L.clmsoil_soil_temperature(byref(a), byref(lev), float(dtime), byref(hc), _stream()),
+               "soil_temperature")
+
+
+def tridiagonal_multi(u, a, b, c, r, jtop, mask, ncols: int, nlevs: int) -> None:
+    """Batched Thomas solve; tensors (nlevs, ncols), jtop int64 (ncols,)."""
+    L = _lib()
+    ld = u.shape[-1]
+    _abi.check(L, L.clmsoil_tridiagonal_multi(u.data_ptr(), a.data_ptr(), b.data_ptr(), c.data_ptr(),
+                                              r.data_ptr(), jtop.data_ptr(),
+                                              None if mask is None else mask.data_ptr(),
+                                              int(ncols), int(nlevs), int(ld), _stream()),
+               "tridiagonal_multi")
+
+
+def batched_band_solve(tvector, bmatrix, rvector, jtop, jbot, mask, kl: int, ku: int,
+                       nlevsno_off: int) -> None:
+    """Batched banded solve; tvector/rvector (nlev, nc), bmatrix (nlev, nband, nc)."""
+    L = _lib()
+    nlev, nc = tvector.shape
+    _abi.check(L, L.clmsoil_batched_band_solve(tvector.data_ptr(), bmatrix.data_ptr(),
+                                               rvector.data_ptr(), jtop.data_ptr(), jbot.data_ptr(),
+                                               None if mask is None else mask.data_ptr(),
+                                               int(kl), int(ku), int(nlevsno_off), int(nc), int(nlev),
+                                               int(nc), _stream()), "batched_band_solve")
+
+
+def device_errors() -> int:
+    """Synchronize the current stream and return/clear the device error bits."""
+    L = _lib()
+    flags = C.c_uint32(0)
+    _abi.check(L, L.clmsoil_device_errors(_stream(), C.byref(flags)), "device_errors")
+    return int(flags.value)
+
+
+def set_tuning(richards_g: int, richards_block: int, heat_g: int, heat_block: int) -> None:
+    L = _lib()
+    _abi.check(L, L.clmsoil_set_tuning(richards_g, richards_block, heat_g, heat_block), "set_tuning")
+
+
+def get_tuning() -> tuple:
+    L = _lib()
+    v = [C.c_int() for _ in range(4)]
+    _abi.check(L, L.clmsoil_get_tuning(*[C.byref(x) for x in v]), "get_tuning")
+    return tuple(int(x.value) for x in v)
+
+
+def to_device(d: dict, device="cuda"):
+    """Copy the numpy arrays of a state dict to GPU tensors (same keys)."""
+    import numpy as np
+    import torch
+    out = {}
+    for k, v in d.items():
+        out[k] = torch.from_numpy(np.ascontiguousarray(v)).to(device) if isinstance(v, np.ndarray) else v
+    return out
+
+
+def to_host(d: dict):
+    import numpy as np
+    out = {}
+    for k, v in d.items():
+        out[k] = v.detach().cpu().numpy() if hasattr(v, "detach") else v
+    return out
+
+
+__all__ = [n for n in dir() if not n.startswith("_")] + ["field"]

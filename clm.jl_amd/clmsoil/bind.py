@@ -1,0 +1,86 @@
+"""Build ABI argument structs from numpy arrays (host) or torch tensors (device).
+
+Arrays follow the CLM layout described in `clmsoil.inputs`: shape (nlev, nc),
+C-contiguous, so the address of element (level k, column c) is
+`base + 8*(c + k*nc)` — the Julia `[c, j]` column-major offset with ld = nc.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._abi import HEAT_FIELDS, MF_FIELDS, HeatArgs, HeatConfig, Levels, MfArgs, SwmConfig
+
+_INT_FIELDS = {"nbedrock", "snl", "imelt"}
+_U8_FIELDS = {"mask"}
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        if not x.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays must be C-contiguous (CLM layout)")
+        return x.ctypes.data
+    # torch tensor
+    if not x.is_contiguous():
+        raise ValueError("tensors must be contiguous (CLM layout)")
+    return x.data_ptr()
+
+
+def _check_dtype(name, x):
+    if x is None:
+        return
+    dt = str(x.dtype)
+    if name in _INT_FIELDS:
+        ok = dt in ("int64", "torch.int64")
+    elif name in _U8_FIELDS:
+        ok = dt in ("uint8", "bool", "torch.uint8", "torch.bool")
+    else:
+        ok = dt in ("float64", "torch.float64")
+    if not ok:
+        raise TypeError(f"{name}: unexpected dtype {dt}")
+
+
+def levels(d) -> Levels:
+    return Levels(int(d["nlevsno"]), int(d["nlevsoi"]), int(d["nlevgrnd"]), 0)
+
+
+def mf_args(d: dict) -> MfArgs:
+    nc = int(d["nc"])
+    a = MfArgs()
+    a.nc = nc
+    a.ld = int(d.get("ld", nc))
+    for f in MF_FIELDS:
+        x = d.get(f)
+        _check_dtype(f, x)
+        setattr(a, f, _ptr(x))
+    return a
+
+
+def heat_args(d: dict) -> HeatArgs:
+    nc = int(d["nc"])
+    a = HeatArgs()
+    a.nc = nc
+    a.ld = int(d.get("ld", nc))
+    for f in HEAT_FIELDS:
+        x = d.get(f)
+        _check_dtype(f, x)
+        setattr(a, f, _ptr(x))
+    return a
+
+
+def swm_config(lower_bc=2, upper_bc=1, method=1, flux_calculation=1, dtmin=60.0,
+               verySmall=1.0e-8, xTolerUpper=1.0e-1, xTolerLower=1.0e-2, e_ice=6.0) -> SwmConfig:
+    """SoilWaterMovementConfig defaults (soil_water_movement.jl:654-671), moisture form."""
+    return SwmConfig(method, upper_bc, lower_bc, flux_calculation, dtmin, verySmall,
+                     xTolerUpper, xTolerLower, e_ice)
+
+
+def heat_config(snow_thermal_cond_method=1) -> HeatConfig:
+    return HeatConfig(snow_thermal_cond_method, 0)
+
+
+def byref(x):
+    return C.byref(x)

@@ -1,0 +1,278 @@
+"""Seeded synthetic column inputs for the soil solves.
+
+Arrays use CLM.jl's memory layout: a Julia `Matrix{Float64}` of size
+`(nc, nlev)` is column-major, so it is stored here as a C-ordered numpy array of
+shape `(nlev, nc)` — row `k` holds level `k` of every column. Snow+soil arrays
+have `nlevsno + nlevgrnd` rows (soil level j at row `j + nlevsno - 1`), `zi`
+one more, soil-only arrays `nlevgrnd` rows.
+
+Grid: the CLM5 "20SL_8.5m" layer structure (src/constants/varcon.jl:241-283).
+Soil properties: the CLM5 pedotransfer functions of init_soil_properties!
+(src/infrastructure/cold_start.jl:276-376) with the default SoilStateParams
+(src/types/soil_state.jl:19-33).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+NLEVSNO, NLEVSOI, NLEVGRND = 12, 20, 25
+SB = 5.67e-8
+DENH2O, DENICE = 1000.0, 917.0
+
+
+def clm5_grid(nlevsoi: int = NLEVSOI, nlevgrnd: int = NLEVGRND):
+    """dzsoi, zsoi, zisoi of the 20SL_8.5m structure (varcon.jl:254-283)."""
+    dz = np.zeros(nlevgrnd)
+    for j in range(1, min(4, nlevsoi) + 1):
+        dz[j - 1] = j * 0.02
+    for j in range(5, min(13, nlevsoi) + 1):
+        dz[j - 1] = dz[3] + (j - 4) * 0.04
+    for j in range(14, nlevsoi + 1):
+        dz[j - 1] = dz[min(13, nlevsoi) - 1] + (j - 13) * 0.10
+    for j in range(nlevsoi + 1, nlevgrnd + 1):
+        dz[j - 1] = dz[nlevsoi - 1] + (((j - nlevsoi) * 25.0) ** 1.5) / 100.0
+    zi = np.zeros(nlevgrnd + 1)
+    for j in range(1, nlevgrnd + 1):
+        zi[j] = zi[j - 1] + dz[j - 1]
+    z = 0.5 * (zi[:-1] + zi[1:])
+    return dz, z, zi
+
+
+def pedotransfer(sand, clay, om, zsoi):
+    """Per-(column, layer) hydraulic and thermal soil properties.
+
+    cold_start.jl:276-376 with every *_sf scale factor = 1 and zsapric = 0.5.
+    sand/clay/om are per column (shape (nc,)); zsoi per layer (nlevgrnd,).
+    Returns dict of (nlevgrnd, nc) arrays.
+    """
+    sand = sand[None, :]
+    clay = clay[None, :]
+    om = om[None, :]
+    dr = (zsoi / 0.5)[:, None]
+    watsat_min = 0.489 - 0.00126 * sand
+    bsw_min = 2.91 + 0.159 * clay
+    sucsat_min = 10.0 * 10.0 ** (1.88 - 0.0131 * sand)
+    xksat_min = 0.0070556 * 10.0 ** (-0.884 + 0.0153 * sand)
+    watsat_org = np.maximum(0.93 - 0.1 * dr, 0.83)
+    bsw_org = np.minimum(2.7 + 9.3 * dr, 12.0)
+    sucsat_org = np.minimum(10.3 - 0.2 * dr, 10.1)
+    xksat_org = np.maximum(0.28 - 0.2799 * dr, xksat_min)
+    watsat = np.minimum((1.0 - om) * watsat_min + om * watsat_org, 0.93)
+    bsw = (1.0 - om) * bsw_min + om * bsw_org
+    sucsat = (1.0 - om) * sucsat_min + om * sucsat_org
+    pcalpha, pcbeta = 0.5, 0.139
+    perc_frac = np.where(om > pcalpha,
+                         (1.0 - pcalpha) ** (-pcbeta) * np.maximum(om - pcalpha, 0.0) ** pcbeta, 0.0)
+    uncon_frac = (1.0 - om) + (1.0 - perc_frac) * om
+    uncon_hksat = uncon_frac / ((1.0 - om) / xksat_min + ((1.0 - perc_frac) * om) / xksat_org)
+    hksat = uncon_frac * uncon_hksat + (perc_frac * om) * xksat_org
+    pd = 2700.0
+    bd = (1.0 - watsat_min) * pd
+    sc = sand + clay
+    tkm = (1.0 - om) * (8.8 * sand + 2.92 * clay) / sc + 0.25 * om
+    tkmg = tkm ** (1.0 - watsat)
+    tkdry = ((0.135 * bd + 64.7) / (pd - 0.947 * bd)) * (1.0 - om) + 0.05 * om
+    csol_min = (2.128e6 * sand + 2.385e6 * clay) / sc
+    csol = (1.0 - om) * csol_min + 2.5e6 * om
+    nl = zsoi.shape[0]
+    out = dict(watsat=watsat, bsw=bsw, sucsat=sucsat, hksat=hksat, tkmg=tkmg,
+               tkdry=np.broadcast_to(tkdry, (nl, sand.shape[1])).copy(),
+               csol=np.broadcast_to(csol, (nl, sand.shape[1])).copy())
+    return {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in out.items()}
+
+
+def _texture(rng, nc):
+    sand = rng.uniform(5.0, 90.0, nc)
+    clay = rng.uniform(5.0, 60.0, nc)
+    om = rng.uniform(0.0, 0.3, nc)
+    return sand, clay, om
+
+
+def _geometry(nc, snl, snow_dz, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrnd=NLEVGRND,
+              fill=np.nan):
+    """z, dz, zi in CLM layout. Soil levels from the CLM5 grid; snow layers
+    stacked above the surface (zi(j-1) = zi(j) - dz(j)); inactive snow slots
+    hold `fill`."""
+    dzs, zs, zis = clm5_grid(nlevsoi, nlevgrnd)
+    nlev = nlevsno + nlevgrnd
+    dz = np.full((nlev, nc), fill)
+    z = np.full((nlev, nc), fill)
+    zi = np.full((nlev + 1, nc), fill)
+    dz[nlevsno:, :] = dzs[:, None]
+    z[nlevsno:, :] = zs[:, None]
+    zi[nlevsno:, :] = zis[:, None]
+    # snow: level j (−nlevsno+1..0) at row j + nlevsno − 1; interface j at row j + nlevsno
+    for j in range(0, -nlevsno, -1):
+        active = snl <= j - 1          # j >= snl + 1
+        r = j + nlevsno - 1
+        dz[r, active] = snow_dz[r, active]
+        zi[r, active] = zi[r + 1, active] - dz[r, active]
+        z[r, active] = 0.5 * (zi[r, active] + zi[r + 1, active])
+    return z, dz, zi
+
+
+def richards_config0(nc=4096, seed=0, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrnd=NLEVGRND,
+                     nbedrock=None, lower_bc=2, zwt_range=(0.5, 10.0), fill=np.nan):
+    """Config 0: one 1800 s moisture-form step over `nc` columns x 20 soil layers.
+
+    Draw order (numpy PCG64(seed)): sand, clay, om (per column), then the
+    per-(layer, column) liquid fraction U(0.3, 0.95), then qflx_infl U(0, 2e-3),
+    then (if nbedrock == "random") nbedrock U{5..20}, then zwt.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sand, clay, om = _texture(rng, nc)
+    dzs, zs, zis = clm5_grid(nlevsoi, nlevgrnd)
+    props = pedotransfer(sand, clay, om, zs)
+    snl = np.zeros(nc, dtype=np.int64)
+    z, dz, zi = _geometry(nc, snl, np.zeros((nlevsno, nc)), nlevsno, nlevsoi, nlevgrnd, fill)
+    nlev = nlevsno + nlevgrnd
+    frac = rng.uniform(0.3, 0.95, (nlevgrnd, nc))
+    liq = np.full((nlev, nc), fill)
+    liq[nlevsno:, :] = frac * props["watsat"] * dzs[:, None] * DENH2O
+    qflx_infl = rng.uniform(0.0, 2.0e-3, nc)
+    if nbedrock is None:
+        nb = np.full(nc, nlevsoi, dtype=np.int64)
+    elif isinstance(nbedrock, str) and nbedrock == "random":
+        nb = rng.integers(5, nlevsoi + 1, nc).astype(np.int64)
+    else:
+        nb = np.full(nc, int(nbedrock), dtype=np.int64)
+    zwt = rng.uniform(zwt_range[0], zwt_range[1], nc)
+    return dict(
+        nc=nc, nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd, dtime=1800.0,
+        lower_bc=lower_bc, nbedrock=nb, h2osoi_liq=liq,
+        smp_l=np.full((nlevgrnd, nc), fill), hk_l=np.full((nlevgrnd, nc), fill),
+        qcharge=np.full(nc, fill), nsubsteps=np.full(nc, fill),
+        z=z, zi=zi, dz=dz, qflx_rootsoi=np.zeros((nlevsoi, nc)),
+        watsat=props["watsat"], hksat=props["hksat"], bsw=props["bsw"],
+        sucsat=props["sucsat"], icefrac=np.zeros((nlevgrnd, nc)), zwt=zwt,
+        qflx_infl=qflx_infl, smpmin=np.full(nc, -1.0e8),
+        sand=sand, clay=clay, om=om)
+
+
+def heat_config1(nc=65536, seed=1, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrnd=NLEVGRND,
+                 h2osfc_variant=False, nbedrock=None, fill=np.nan):
+    """Config 1: one 1800 s snow+soil heat step.
+
+    Per column: texture as config 0; ns ~ U{0..12} snow layers (snl = -ns) with
+    dz ~ U(0.01, 0.5) m, density ~ U(80, 500) kg/m3 split into ice/liquid by an
+    ice fraction ~ U(0, 1); ground water theta ~ U(0.05, 0.95) x watsat split by
+    an ice fraction ~ U(0, 1); t_soisno ~ U(250, 300) K; hs_top_snow = hs_soil =
+    hs_top = hs_h2osfc ~ N(50, 100) W/m2; dhsdT = -4*0.97*sigma*T_top^3;
+    frac_sno_eff = 1 when snow exists. The h2osfc variant adds frac_h2osfc ~
+    U(0, 0.3), h2osfc ~ U(0, 40) kg/m2, t_h2osfc ~ U(265, 285) K, and unresolved
+    snow h2osno_no_layers ~ U(0, 5) kg/m2 on half of the snow-free columns.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sand, clay, om = _texture(rng, nc)
+    dzs, zs, zis = clm5_grid(nlevsoi, nlevgrnd)
+    props = pedotransfer(sand, clay, om, zs)
+    ns = rng.integers(0, nlevsno + 1, nc)
+    snl = (-ns).astype(np.int64)
+    snow_dz = rng.uniform(0.01, 0.5, (nlevsno, nc))
+    rho = rng.uniform(80.0, 500.0, (nlevsno, nc))
+    fice_snow = rng.uniform(0.0, 1.0, (nlevsno, nc))
+    z, dz, zi = _geometry(nc, snl, snow_dz, nlevsno, nlevsoi, nlevgrnd, fill)
+    nlev = nlevsno + nlevgrnd
+    liq = np.full((nlev, nc), fill)
+    ice = np.full((nlev, nc), fill)
+    t = np.full((nlev, nc), fill)
+    rows = np.arange(nlevsno)[:, None]
+    active_snow = (rows - nlevsno + 1) >= (snl[None, :] + 1)
+    mass = rho * snow_dz
+    liq[:nlevsno] = np.where(active_snow, (1.0 - fice_snow) * mass, fill)
+    ice[:nlevsno] = np.where(active_snow, fice_snow * mass, fill)
+    theta = rng.uniform(0.05, 0.95, (nlevgrnd, nc)) * props["watsat"]
+    fice = rng.uniform(0.0, 1.0, (nlevgrnd, nc))
+    liq[nlevsno:] = (1.0 - fice) * theta * dzs[:, None] * DENH2O
+    ice[nlevsno:] = fice * theta * dzs[:, None] * DENICE
+    tt = rng.uniform(250.0, 300.0, (nlev, nc))
+    t[nlevsno:] = tt[nlevsno:]
+    t[:nlevsno] = np.where(active_snow, tt[:nlevsno], fill)
+    hs = rng.normal(50.0, 100.0, nc)
+    top = (snl + nlevsno).astype(np.int64)          # row of the top active layer
+    t_top = t[top, np.arange(nc)]
+    dhsdT = -4.0 * 0.97 * SB * t_top ** 3
+    frac_sno_eff = (ns > 0).astype(np.float64)
+    snow_depth = np.where(active_snow, dz[:nlevsno], 0.0).sum(axis=0)
+    if nbedrock is None:
+        nb = np.full(nc, nlevsoi, dtype=np.int64)
+    elif isinstance(nbedrock, str) and nbedrock == "random":
+        nb = rng.integers(5, nlevsoi + 1, nc).astype(np.int64)
+    else:
+        nb = np.full(nc, int(nbedrock), dtype=np.int64)
+    frac_h2osfc = np.zeros(nc)
+    h2osfc = np.zeros(nc)
+    t_h2osfc = t[nlevsno].copy()
+    h2osno_nl = np.zeros(nc)
+    if h2osfc_variant:
+        frac_h2osfc = rng.uniform(0.0, 0.3, nc)
+        h2osfc = rng.uniform(0.0, 40.0, nc)
+        t_h2osfc = rng.uniform(265.0, 285.0, nc)
+        h2osno_nl = np.where((ns == 0) & (rng.uniform(0, 1, nc) < 0.5),
+                             rng.uniform(0.0, 5.0, nc), 0.0)
+    int_snow = np.where(active_snow, liq[:nlevsno] + ice[:nlevsno], 0.0).sum(axis=0) + h2osno_nl
+    return dict(
+        nc=nc, nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd, dtime=1800.0,
+        snl=snl, nbedrock=nb, z=z, dz=dz, zi=zi, t_soisno=t, h2osoi_liq=liq,
+        h2osoi_ice=ice, excess_ice=np.zeros((nlevgrnd, nc)),
+        watsat=props["watsat"], tkmg=props["tkmg"], tkdry=props["tkdry"],
+        csol=props["csol"], bsw=props["bsw"], sucsat=props["sucsat"],
+        hksat=props["hksat"],
+        frac_sno_eff=frac_sno_eff, frac_h2osfc=frac_h2osfc,
+        hs_top_snow=hs.copy(), hs_soil=hs.copy(), hs_top=hs.copy(), hs_h2osfc=hs.copy(),
+        dhsdT=dhsdT, sabg_lyr_col=np.zeros((nlevsno + 1, nc)), eflx_bot=np.zeros(nc),
+        t_h2osfc=t_h2osfc, h2osfc=h2osfc, h2osno_no_layers=h2osno_nl,
+        snow_depth=snow_depth, int_snow=int_snow)
+
+
+HEAT_OUTPUT_SHAPES = {
+    # name: (rows or None for per-column)
+    "t_grnd": None, "xmf": None, "xmf_h2osfc": None, "qflx_snomelt": None,
+    "qflx_snofrz": None, "qflx_snow_drain": None, "qflx_h2osfc_to_ice": None,
+    "eflx_fgr12": None, "qflx_snomelt_lyr": "nlevsno", "qflx_snofrz_lyr": "nlevsno",
+}
+HEAT_DIAG_SHAPES = {
+    "fact": "nlev", "c_h2osfc": None, "thk": "nlev", "bw": "nlevsno",
+    "eflx_fgr": "nlevgrnd", "exice_subs": "nlevgrnd", "eflx_snomelt": None,
+    "eflx_snomelt_r": None, "eflx_h2osfc_to_snow": None, "snomelt_accum": None,
+}
+
+
+def heat_outputs(cfg: dict, diagnostics: bool = True, fill=np.nan) -> dict:
+    """Allocate the heat step's output arrays (host, NaN-filled)."""
+    nc, ns, ng = cfg["nc"], cfg["nlevsno"], cfg["nlevgrnd"]
+    rows = {"nlevsno": ns, "nlevgrnd": ng, "nlev": ns + ng}
+    out = {}
+    shapes = dict(HEAT_OUTPUT_SHAPES)
+    if diagnostics:
+        shapes.update(HEAT_DIAG_SHAPES)
+    for k, r in shapes.items():
+        out[k] = np.full(nc if r is None else (rows[r], nc), fill)
+    if diagnostics:
+        out["snomelt_accum"][:] = 0.0
+        out["imelt"] = np.full((ns + ng, nc), -1, dtype=np.int64)
+    return out
+
+
+def production_day(nc=1 << 20, seed=2, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrnd=NLEVGRND,
+                   fill=0.0):
+    """Config 2: uneven production columns for a 48-step day of Richards + heat.
+
+    Snow layers ns ~ U{0..12}, bedrock index nbedrock ~ U{5..20}, water-table
+    depth zwt ~ U(0.5, 10) m (the Richards lower boundary is BC_WATERTABLE so
+    zwt takes effect), otherwise drawn as the heat-step configuration. icefrac is
+    the initial ice fraction of the pore space, min(1, vol_ice / watsat).
+    """
+    h = heat_config1(nc=nc, seed=seed, nlevsno=nlevsno, nlevsoi=nlevsoi,
+                     nlevgrnd=nlevgrnd, nbedrock="random", fill=fill)
+    rng = np.random.Generator(np.random.PCG64(seed + 1000))
+    dzs, _, _ = clm5_grid(nlevsoi, nlevgrnd)
+    vol_ice = h["h2osoi_ice"][nlevsno:] / (dzs[:, None] * DENICE)
+    icefrac = np.minimum(1.0, vol_ice / h["watsat"])
+    h.update(
+        lower_bc=3, qflx_infl=rng.uniform(0.0, 2.0e-3, nc), zwt=rng.uniform(0.5, 10.0, nc),
+        icefrac=np.ascontiguousarray(icefrac), qflx_rootsoi=np.zeros((nlevsoi, nc)),
+        smpmin=np.full(nc, -1.0e8), smp_l=np.full((nlevgrnd, nc), fill),
+        hk_l=np.full((nlevgrnd, nc), fill), qcharge=np.full(nc, fill),
+        nsubsteps=np.full(nc, fill))
+    return h

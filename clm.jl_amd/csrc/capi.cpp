@@ -1,0 +1,321 @@
+// capi.cpp — C ABI entry points of libclmsoil (see include/clmsoil.h).
+//
+// Host-side validation mirrors the reference's host checks
+// (soilwm_moisture_form_solve!, src/biogeophys/soil_water_movement.jl:1500-1505);
+// everything else is asynchronous on the caller's stream. Conditions the
+// reference only meets inside its kernels (nbedrock or snl out of range) are
+// recorded in a per-device error word that clmsoil_device_errors() reads back.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <mutex>
+#include <vector>
+
+#include "../../include/clmsoil.h"
+#include "../../include/clmsoil_ext.h"
+#include "common.hpp"
+#include "heat.hpp"
+#include "richards.hpp"
+#include "solvers.hpp"
+#include "state.hpp"
+
+using namespace clmsoil;
+
+namespace clmsoil {
+
+Tuning g_tuning;   // set by clmsoil_set_tuning (the autotuner), read at launch
+
+namespace {
+std::mutex g_mu;
+std::vector<DeviceState> g_dev;
+}  // namespace
+
+int current_device_state(DeviceState **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return CLMSOIL_ERR_HIP;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
+    DeviceState &s = g_dev[dev];
+    if (!s.err) {
+        // one-time per-device allocation of the 256-byte status block
+        void *p = nullptr;
+        if (hipMalloc(&p, 256) != hipSuccess) return CLMSOIL_ERR_HIP;
+        if (hipMemset(p, 0, 256) != hipSuccess) return CLMSOIL_ERR_HIP;
+        s.err = static_cast<unsigned *>(p);
+        s.pivot_count = s.err + 1;
+    }
+    *out = &s;
+    return CLMSOIL_OK;
+}
+
+int ensure_pivot_list(DeviceState *s, int64_t nc) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (s->pivot_cap >= nc) return CLMSOIL_OK;
+    if (s->pivot_list) {
+        if (hipDeviceSynchronize() != hipSuccess) return CLMSOIL_ERR_HIP;
+        (void)hipFree(s->pivot_list);
+        s->pivot_list = nullptr;
+    }
+    void *p = nullptr;
+    const int64_t cap = nc < 1024 ? 1024 : nc;
+    if (hipMalloc(&p, (size_t)cap * sizeof(int64_t)) != hipSuccess) return CLMSOIL_ERR_HIP;
+    s->pivot_list = static_cast<int64_t *>(p);
+    s->pivot_cap = cap;
+    return CLMSOIL_OK;
+}
+
+int validate_levels(const clmsoil_levels *lev) {
+    if (!lev) return CLMSOIL_ERR_NULL;
+    if (lev->nlevsno < 1 || lev->nlevsno > MAX_NLEVSNO) return CLMSOIL_ERR_SIZE;
+    if (lev->nlevsoi < 2 || lev->nlevsoi > MAX_NLEVSOI) return CLMSOIL_ERR_SIZE;
+    if (lev->nlevgrnd < lev->nlevsoi || lev->nlevgrnd > MAX_NLEVGRND) return CLMSOIL_ERR_SIZE;
+    return CLMSOIL_OK;
+}
+
+int validate_mf(const clmsoil_mf_args *a, const clmsoil_levels *lev, double dtime,
+                const clmsoil_swm_config *cfg) {
+    if (!a || !cfg) return CLMSOIL_ERR_NULL;
+    int st = validate_levels(lev);
+    if (st) return st;
+    if (cfg->soilwater_movement_method != CLMSOIL_MOISTURE_FORM) return CLMSOIL_ERR_METHOD;
+    if (cfg->upper_boundary_condition != CLMSOIL_BC_FLUX) return CLMSOIL_ERR_BC;
+    const int bc = cfg->lower_boundary_condition;
+    if (bc != CLMSOIL_BC_WATERTABLE && bc != CLMSOIL_BC_FLUX && bc != CLMSOIL_BC_ZERO_FLUX)
+        return CLMSOIL_ERR_BC;
+    if (a->nc < 0 || a->ld < a->nc) return CLMSOIL_ERR_SIZE;
+    if (!(dtime > 0.0) || !std::isfinite(dtime)) return CLMSOIL_ERR_ARG;
+    if (!(cfg->dtmin > 0.0) || !(cfg->verySmall > 0.0) || !std::isfinite(cfg->dtmin))
+        return CLMSOIL_ERR_ARG;
+    if (a->nc == 0) return CLMSOIL_OK;
+    if (!a->nbedrock || !a->h2osoi_liq || !a->smp_l || !a->hk_l || !a->qcharge ||
+        !a->nsubsteps || !a->z || !a->dz || !a->qflx_rootsoi || !a->watsat || !a->hksat ||
+        !a->bsw || !a->sucsat || !a->icefrac || !a->qflx_infl)
+        return CLMSOIL_ERR_NULL;
+    if (bc == CLMSOIL_BC_WATERTABLE && (!a->zwt || !a->zi)) return CLMSOIL_ERR_NULL;
+    return CLMSOIL_OK;
+}
+
+RichardsParams richards_params(const clmsoil_mf_args *a, const clmsoil_levels *lev,
+                               double dtime, const clmsoil_swm_config *cfg, bool aquifer,
+                               unsigned *err) {
+    RichardsParams P{};
+    P.nc = a->nc; P.ld = a->ld; P.mask = a->mask; P.nbedrock = a->nbedrock;
+    P.h2osoi_liq = a->h2osoi_liq; P.smp_l = a->smp_l; P.hk_l = a->hk_l;
+    P.qcharge = a->qcharge; P.nsubsteps = a->nsubsteps; P.nrejected = a->nrejected;
+    P.z = a->z; P.zi = a->zi; P.dz = a->dz; P.qflx_rootsoi = a->qflx_rootsoi;
+    P.watsat = a->watsat; P.hksat = a->hksat; P.bsw = a->bsw; P.sucsat = a->sucsat;
+    P.icefrac = a->icefrac; P.zwt = a->zwt; P.qflx_infl = a->qflx_infl;
+    P.smpmin = a->smpmin; P.dtsub0 = a->dtsub0;
+    P.nlevsno = lev->nlevsno; P.nlevsoi = lev->nlevsoi;
+    P.lower_bc = cfg->lower_boundary_condition;
+    P.expensive = cfg->flux_calculation == CLMSOIL_FLUX_EXPENSIVE;
+    P.aquifer_pass = aquifer ? 1 : 0;
+    P.max_iter = 100000;
+    P.dtime = dtime; P.e_ice = cfg->e_ice; P.dtmin = cfg->dtmin;
+    P.xtu = cfg->xTolerUpper; P.xtl = cfg->xTolerLower; P.verysmall = cfg->verySmall;
+    P.err = err;
+    return P;
+}
+
+int validate_heat(const clmsoil_heat_args *a, const clmsoil_levels *lev) {
+    if (!a) return CLMSOIL_ERR_NULL;
+    int st = validate_levels(lev);
+    if (st) return st;
+    if (a->nc < 0 || a->ld < a->nc) return CLMSOIL_ERR_SIZE;
+    if (a->nc == 0) return CLMSOIL_OK;
+    if (!a->snl || !a->nbedrock || !a->z || !a->dz || !a->zi || !a->t_soisno ||
+        !a->h2osoi_liq || !a->h2osoi_ice || !a->excess_ice || !a->watsat || !a->tkmg ||
+        !a->tkdry || !a->csol || !a->bsw || !a->sucsat || !a->frac_sno_eff ||
+        !a->frac_h2osfc || !a->hs_top_snow || !a->hs_soil || !a->hs_top || !a->hs_h2osfc ||
+        !a->dhsdT || !a->t_h2osfc || !a->h2osfc || !a->h2osno_no_layers || !a->snow_depth ||
+        !a->t_grnd || !a->xmf || !a->xmf_h2osfc || !a->qflx_snomelt || !a->qflx_snofrz ||
+        !a->qflx_snow_drain || !a->qflx_h2osfc_to_ice || !a->eflx_fgr12 ||
+        !a->qflx_snomelt_lyr || !a->qflx_snofrz_lyr)
+        return CLMSOIL_ERR_NULL;
+    return CLMSOIL_OK;
+}
+
+__global__ void zero_words_kernel(unsigned *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+int run_heat(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime,
+             const clmsoil_heat_config *cfg, hipStream_t stream, unsigned *err,
+             unsigned *pivot_count, int64_t *pivot_list) {
+    HeatParams P{};
+    P.a = *a;
+    P.nlevsno = lev->nlevsno;
+    P.nlevgrnd = lev->nlevgrnd;
+    P.snow_method = cfg ? cfg->snow_thermal_cond_method : 1;
+    P.dtime = dtime;
+    P.err = err;
+    P.pivot_count = pivot_count;
+    P.pivot_list = pivot_list;
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, pivot_count, 1);
+    if (hipGetLastError() != hipSuccess) return CLMSOIL_ERR_HIP;
+    if (launch_heat(P, g_tuning.heat_g, g_tuning.heat_block, stream) != hipSuccess)
+        return CLMSOIL_ERR_HIP;
+    if (launch_heat_pivot(P, stream) != hipSuccess) return CLMSOIL_ERR_HIP;
+    return CLMSOIL_OK;
+}
+
+}  // namespace clmsoil
+
+extern "C" {
+
+int clmsoil_abi_version(void) { return CLMSOIL_ABI_VERSION; }
+
+const char *clmsoil_strerror(int status) {
+    switch (status) {
+        case CLMSOIL_OK: return "ok";
+        case CLMSOIL_ERR_NULL: return "a required pointer is NULL";
+        case CLMSOIL_ERR_BC: return "unsupported boundary condition";
+        case CLMSOIL_ERR_METHOD: return "unsupported soil water movement method";
+        case CLMSOIL_ERR_SIZE: return "column or level count out of the supported range";
+        case CLMSOIL_ERR_HIP: return "HIP runtime error";
+        case CLMSOIL_ERR_WORKSPACE: return "workspace missing or too small";
+        case CLMSOIL_ERR_ARG: return "invalid argument";
+        case CLMSOIL_ERR_NOT_TRAINED: return "learned controller has no weights";
+        default: return "unknown status";
+    }
+}
+
+int clmsoil_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int clmsoil_sync(void *stream) {
+    return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? CLMSOIL_OK : CLMSOIL_ERR_HIP;
+}
+
+int clmsoil_device_errors(void *stream, uint32_t *flags) {
+    if (!flags) return CLMSOIL_ERR_NULL;
+    DeviceState *s = nullptr;
+    int st = current_device_state(&s);
+    if (st) return st;
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return CLMSOIL_ERR_HIP;
+    unsigned h = 0;
+    if (hipMemcpy(&h, s->err, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
+        return CLMSOIL_ERR_HIP;
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, s->err, 1);
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return CLMSOIL_ERR_HIP;
+    *flags = h;
+    return CLMSOIL_OK;
+}
+
+int clmsoil_set_tuning(int richards_g, int richards_block, int heat_g, int heat_block) {
+    auto okg = [](int g) { return g == 1 || g == 2 || g == 4; };
+    auto okb = [](int b) { return b >= 64 && b <= 1024 && b % 64 == 0; };
+    if (!okg(richards_g) || !okg(heat_g) || !okb(richards_block) || !okb(heat_block))
+        return CLMSOIL_ERR_ARG;
+    g_tuning.richards_g = richards_g;
+    g_tuning.richards_block = richards_block;
+    g_tuning.heat_g = heat_g;
+    g_tuning.heat_block = heat_block;
+    return CLMSOIL_OK;
+}
+
+int clmsoil_get_tuning(int *richards_g, int *richards_block, int *heat_g, int *heat_block) {
+    if (!richards_g || !richards_block || !heat_g || !heat_block) return CLMSOIL_ERR_NULL;
+    *richards_g = g_tuning.richards_g;
+    *richards_block = g_tuning.richards_block;
+    *heat_g = g_tuning.heat_g;
+    *heat_block = g_tuning.heat_block;
+    return CLMSOIL_OK;
+}
+
+int clmsoil_soilwm_moisture_form_solve(const clmsoil_mf_args *a, const clmsoil_levels *lev,
+                                       double dtime, const clmsoil_swm_config *cfg,
+                                       void *stream) {
+    int st = validate_mf(a, lev, dtime, cfg);
+    if (st || a->nc == 0) return st;
+    DeviceState *s = nullptr;
+    if ((st = current_device_state(&s))) return st;
+    RichardsParams P = richards_params(a, lev, dtime, cfg, false, s->err);
+    if (launch_richards(P, g_tuning.richards_g, g_tuning.richards_block, (hipStream_t)stream) !=
+        hipSuccess)
+        return CLMSOIL_ERR_HIP;
+    return CLMSOIL_OK;
+}
+
+int clmsoil_soilwater_moisture_form(const clmsoil_mf_args *a, const clmsoil_levels *lev,
+                                    double dtime, const clmsoil_swm_config *cfg, void *stream) {
+    int st = validate_mf(a, lev, dtime, cfg);
+    if (st || a->nc == 0) return st;
+    DeviceState *s = nullptr;
+    if ((st = current_device_state(&s))) return st;
+    const bool aquifer = cfg->lower_boundary_condition == CLMSOIL_BC_WATERTABLE ||
+                         cfg->lower_boundary_condition == CLMSOIL_BC_AQUIFER;
+    RichardsParams P = richards_params(a, lev, dtime, cfg, aquifer, s->err);
+    if (launch_richards(P, g_tuning.richards_g, g_tuning.richards_block, (hipStream_t)stream) !=
+        hipSuccess)
+        return CLMSOIL_ERR_HIP;
+    return CLMSOIL_OK;
+}
+
+int clmsoil_soil_temperature(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime,
+                             const clmsoil_heat_config *cfg, void *stream) {
+    int st = validate_heat(a, lev);
+    if (st || a->nc == 0) return st;
+    if (!(dtime > 0.0) || !std::isfinite(dtime)) return CLMSOIL_ERR_ARG;
+    if (cfg && cfg->snow_thermal_cond_method != 1 && cfg->snow_thermal_cond_method != 2)
+        return CLMSOIL_ERR_ARG;
+    DeviceState *s = nullptr;
+    if ((st = current_device_state(&s))) return st;
+    if ((st = ensure_pivot_list(s, a->nc))) return st;
+    return run_heat(a, lev, dtime, cfg, (hipStream_t)stream, s->err, s->pivot_count,
+                    s->pivot_list);
+}
+
+size_t clmsoil_heat_workspace_size(int64_t nc) {
+    return 256 + (size_t)(nc > 0 ? nc : 0) * sizeof(int64_t);
+}
+
+int clmsoil_soil_temperature_ws(const clmsoil_heat_args *a, const clmsoil_levels *lev,
+                                double dtime, const clmsoil_heat_config *cfg, void *workspace,
+                                size_t workspace_bytes, void *stream) {
+    int st = validate_heat(a, lev);
+    if (st || a->nc == 0) return st;
+    if (!(dtime > 0.0) || !std::isfinite(dtime)) return CLMSOIL_ERR_ARG;
+    if (!workspace || workspace_bytes < clmsoil_heat_workspace_size(a->nc))
+        return CLMSOIL_ERR_WORKSPACE;
+    unsigned *words = static_cast<unsigned *>(workspace);
+    int64_t *list = reinterpret_cast<int64_t *>(static_cast<char *>(workspace) + 256);
+    DeviceState *s = nullptr;
+    if ((st = current_device_state(&s))) return st;
+    return run_heat(a, lev, dtime, cfg, (hipStream_t)stream, s->err, words + 1, list);
+}
+
+int clmsoil_tridiagonal_multi(double *u, const double *a, const double *b, const double *c,
+                              const double *r, const int64_t *jtop, const uint8_t *mask,
+                              int64_t ncols, int64_t nlevs, int64_t ld, void *stream) {
+    if (ncols < 0 || ld < ncols || nlevs < 1 || nlevs > 64) return CLMSOIL_ERR_SIZE;
+    if (ncols == 0) return CLMSOIL_OK;
+    if (!u || !a || !b || !c || !r || !jtop) return CLMSOIL_ERR_NULL;
+    DeviceState *s = nullptr;
+    int st = current_device_state(&s);
+    if (st) return st;
+    TridiagParams P{u, a, b, c, r, jtop, mask, ncols, nlevs, ld, s->err};
+    return launch_tridiag_multi(P, (hipStream_t)stream) == hipSuccess ? CLMSOIL_OK : CLMSOIL_ERR_HIP;
+}
+
+int clmsoil_batched_band_solve(double *tvector, const double *bmatrix, const double *rvector,
+                               const int64_t *jtop, const int64_t *jbot, const uint8_t *mask,
+                               int32_t kl, int32_t ku, int32_t nlevsno_off, int64_t nc,
+                               int64_t nlev, int64_t ld, void *stream) {
+    if (nc < 0 || ld < nc || nlev < 1) return CLMSOIL_ERR_SIZE;
+    if (kl < 1 || ku < 1 || kl > 2 || ku > 2) return CLMSOIL_ERR_ARG;
+    if (nc == 0) return CLMSOIL_OK;
+    if (!tvector || !bmatrix || !rvector || !jtop || !jbot) return CLMSOIL_ERR_NULL;
+    DeviceState *s = nullptr;
+    int st = current_device_state(&s);
+    if (st) return st;
+    BandParams P{tvector, bmatrix, rvector, jtop, jbot, mask, kl, ku, nlevsno_off,
+                 nc, nlev, ld, s->err};
+    return launch_band_solve(P, (hipStream_t)stream) == hipSuccess ? CLMSOIL_OK : CLMSOIL_ERR_HIP;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// common.hpp — shared device helpers for the clmsoil HIP kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/clmsoil.h"
+
+namespace clmsoil {
+
+// Physical constants, src/constants/varcon.jl:15-70.
+constexpr double GRAV = 9.80616, CPLIQ = 4188.0, CPICE = 2117.27, HFUS = 3.337e5,
+                 DENH2O = 1000.0, DENICE = 917.0, TFRZ = 273.15, TKAIR = 0.023,
+                 TKICE = 2.290, TKWAT = 0.57, CAPR = 0.34, CNFAC = 0.5,
+                 THK_BEDROCK = 3.0, CSOL_BEDROCK = 2.0e6;
+constexpr double M_TO_MM = 1.0e3;          // soil_water_movement.jl:37
+constexpr double THIN_SFCLAYER = 1.0e-6;   // soil_temperature.jl:26
+
+// Compile-time layout limits (CLM5 20SL_8.5m, varpar.jl:112-149).
+constexpr int MAX_NLEVSOI = 20;
+constexpr int MAX_NLEVGRND = 25;
+constexpr int MAX_NLEVSNO = 12;
+
+// Julia Base.max / Base.min on Float64: the result of a diff-based select, so a
+// NaN operand propagates, max(-0.0, 0.0) = 0.0 and min(0.0, -0.0) = -0.0.
+__host__ __device__ __forceinline__ double jl_max(double x, double y) {
+    double d = x - y;
+    double am = __builtin_signbit(d) ? y : x;
+    return (x != x || y != y) ? d : am;
+}
+__host__ __device__ __forceinline__ double jl_min(double x, double y) {
+    double d = x - y;
+    double am = __builtin_signbit(d) ? x : y;
+    return (x != x || y != y) ? d : am;
+}
+
+// Column-major [c, j] access with 1-based level j (the reference's index).
+struct Mat {
+    double *p;
+    int64_t ld;
+    __device__ __forceinline__ double &operator()(int64_t c, int j) const {
+        return p[c + (int64_t)(j - 1) * ld];
+    }
+};
+struct CMat {
+    const double *p;
+    int64_t ld;
+    __device__ __forceinline__ double operator()(int64_t c, int j) const {
+        return __builtin_nontemporal_load(&p[c + (int64_t)(j - 1) * ld]);
+    }
+};
+
+// Broadcast lane `k` of each group of G consecutive lanes (G in {2, 4}) to the
+// whole group with a DPP quad permutation; G == 1 is the identity.
+template <int G>
+__device__ __forceinline__ double group_bcast(double v, int k);
+
+template <>
+__device__ __forceinline__ double group_bcast<1>(double v, int) { return v; }
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// quad_perm encoding: sel0 | sel1<<2 | sel2<<4 | sel3<<6
+template <>
+__device__ __forceinline__ double group_bcast<2>(double v, int k) {
+    // pairs (0,1) and (2,3) of each quad
+    return k == 0 ? dpp_f64<(0 | 0 << 2 | 2 << 4 | 2 << 6)>(v)
+                  : dpp_f64<(1 | 1 << 2 | 3 << 4 | 3 << 6)>(v);
+}
+template <>
+__device__ __forceinline__ double group_bcast<4>(double v, int k) {
+    switch (k) {
+        case 0: return dpp_f64<0x00>(v);
+        case 1: return dpp_f64<0x55>(v);
+        case 2: return dpp_f64<0xAA>(v);
+        default: return dpp_f64<0xFF>(v);
+    }
+}
+
+// Value held by the next lane of the group ((k+1) mod G) / previous lane.
+template <int G>
+__device__ __forceinline__ double group_next(double v);
+template <int G>
+__device__ __forceinline__ double group_prev(double v);
+template <>
+__device__ __forceinline__ double group_next<1>(double v) { return v; }
+template <>
+__device__ __forceinline__ double group_prev<1>(double v) { return v; }
+template <>
+__device__ __forceinline__ double group_next<2>(double v) {
+    return dpp_f64<(1 | 0 << 2 | 3 << 4 | 2 << 6)>(v);
+}
+template <>
+__device__ __forceinline__ double group_prev<2>(double v) {
+    return dpp_f64<(1 | 0 << 2 | 3 << 4 | 2 << 6)>(v);
+}
+template <>
+__device__ __forceinline__ double group_next<4>(double v) {
+    return dpp_f64<(1 | 2 << 2 | 3 << 4 | 0 << 6)>(v);
+}
+template <>
+__device__ __forceinline__ double group_prev<4>(double v) {
+    return dpp_f64<(3 | 0 << 2 | 1 << 4 | 2 << 6)>(v);
+}
+
+// Max over the G lanes of a group.
+template <int G>
+__device__ __forceinline__ double group_max(double v) {
+    if constexpr (G >= 2) v = jl_max(v, dpp_f64<(1 | 0 << 2 | 3 << 4 | 2 << 6)>(v));
+    if constexpr (G >= 4) v = jl_max(v, dpp_f64<(2 | 3 << 2 | 0 << 4 | 1 << 6)>(v));
+    return v;
+}
+
+// Device error word: kernels OR a bit in when a column hits an input the
+// reference would reject (e.g. nbedrock out of range); the host reads it back.
+enum DeviceError : unsigned {
+    DEV_ERR_NBEDROCK = 1u,
+    DEV_ERR_SNL = 2u,
+    DEV_ERR_ITER = 4u,
+};
+
+}  // namespace clmsoil

@@ -1,0 +1,820 @@
+// heat.hip — fused snow+soil heat step for non-urban soil columns (gfx950).
+//
+// One launch restates the body of soil_temperature!
+// (src/biogeophys/soil_temperature.jl:358-593) for columns whose ground-heat-flux
+// terms (hs_*, dhsdT, sabg_lyr_col) are given:
+//   thermal properties  _soil_snow_tk_kernel!, _soil_tk_interface_kernel!,
+//                       _tk_h2osfc_kernel!, _soil_cv_kernel!, _unresolved_snow_cv_kernel!,
+//                       _snow_cv_kernel!                        (:599-872)
+//   diffusion factor    _heat_diff_kernel!                      (:1271-1310)
+//   h2osfc properties   _h2osfc_thermprop_kernel!               (:116-130)
+//   RHS / 5-band matrix set_rhs_vec!, set_matrix!               (:1365-1748)
+//   banded solve        batched_band_solve!                     (band_diagonal.jl:87-170)
+//   unpack, fn1         _tvector_extract_kernel!, _fn1_kernel!  (:149-206)
+//   phase change        phase_change_h2osfc!, phase_change_beta! (:1757-2289)
+//   diagnostics         _tgrnd_kernel!, _eflx_fgr_kernel!       (:224-269)
+//
+// Layout: a group of G lanes owns one column; snow+soil level jj (1-based,
+// jj = j + nlevsno) lives in lane (jj-1) mod G, register slot (jj-1) div G. The
+// standing-surface-water (h2osfc) row of the reference's 5-band system couples
+// only to soil layer 1, so it is carried as a column scalar.
+//
+// Solve: the reference densifies each column and runs Gaussian elimination with
+// partial pivoting. Its matrix is tridiagonal in the level order except for the
+// snow-0 -> soil-1 (band 1) and h2osfc <-> soil-1 (bands 2/4) couplings, so when
+// no row interchange occurs the dense elimination performs exactly the
+// operations of the banded sweep below (every other update subtracts f*0). The
+// sweep checks the reference's pivot test at every step; a column that would
+// interchange rows is finished by the pivoting pass (dense elimination as in the
+// reference) instead. Elimination and back-substitution carries move between the
+// lanes of a group by DPP broadcast.
+#include "common.hpp"
+#include "heat.hpp"
+
+namespace clmsoil {
+
+namespace {
+
+template <int G>
+__device__ __forceinline__ double hnxt(const double *x, int i, int k, int S) {
+    double a = group_next<G>(x[i]);
+    double b = (i + 1 < S) ? group_next<G>(x[i + 1]) : 0.0;
+    return (k == G - 1) ? b : a;
+}
+template <int G>
+__device__ __forceinline__ double hprv(const double *x, int i, int k) {
+    double a = group_prev<G>(x[i]);
+    double b = (i > 0) ? group_prev<G>(x[i - 1]) : 0.0;
+    return (k == 0) ? b : a;
+}
+// value of level jj (runtime) broadcast to the whole group
+template <int G, int S>
+__device__ __forceinline__ double level_value(const double *x, int jj, int k) {
+    double v = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+        if (i * G + k + 1 == jj) v = x[i];
+    return group_bcast<G>(v, (jj - 1) % G);
+}
+
+struct Dense {
+    // reference's per-column dense elimination (band_diagonal.jl:87-170), used by
+    // the pivoting pass; n <= 38
+    double A[38 * 38];
+    double x[38];
+};
+
+}  // namespace
+
+template <int G, bool DENSE>
+__global__ __launch_bounds__(256) void heat_kernel(HeatParams P) {
+    constexpr int NL = MAX_NLEVSNO + MAX_NLEVGRND;   // 37 snow+soil levels
+    constexpr int S = (NL + G - 1) / G;
+    const clmsoil_heat_args &a = P.a;
+    int64_t c;
+    int k;
+    if constexpr (DENSE) {
+        const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= (int64_t)*P.pivot_count) return;
+        c = P.pivot_list[t];
+        k = 0;
+    } else {
+        const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        c = gtid / G;
+        k = (int)(threadIdx.x % G);
+        if (c >= a.nc) return;
+        if (a.mask && !a.mask[c]) return;
+    }
+    const int64_t ld = a.ld;
+    const int nlevsno = P.nlevsno, nlevgrnd = P.nlevgrnd;
+    const int nlev = nlevsno + nlevgrnd;
+    const int js1 = nlevsno + 1;                       // level index of soil layer 1
+    const int64_t snl = a.snl[c];
+    if (snl < -nlevsno || snl > 0) {
+        if (k == 0) atomicOr(P.err, (unsigned)DEV_ERR_SNL);
+        return;
+    }
+    const int jtop = (int)snl + nlevsno + 1;           // top active level
+    const int nbedrock = (int)a.nbedrock[c];
+    const double dtime = P.dtime;
+    const double cnfac = CNFAC, omc = 1.0 - CNFAC;
+
+    // ---- column scalars ----
+    const double frac_sno = a.frac_sno_eff[c];
+    const double frac_h2osfc = a.frac_h2osfc[c];
+    const double dhsdT = a.dhsdT[c];
+    const double hs_top_snow = a.hs_top_snow[c], hs_soil = a.hs_soil[c], hs_h2osfc = a.hs_h2osfc[c];
+    const double eflx_bot = a.eflx_bot ? a.eflx_bot[c] : 0.0;
+    double t_h2osfc = a.t_h2osfc[c];
+    double h2osfc = a.h2osfc[c];
+    double h2osno_nl = a.h2osno_no_layers[c];
+    double snow_depth = a.snow_depth[c];
+    double int_snow = a.int_snow ? a.int_snow[c] : 0.0;
+    double snomelt_accum = a.snomelt_accum ? a.snomelt_accum[c] : 0.0;
+
+    // ---- per-level loads ----
+    double t[S], liq[S], ice[S], z[S], dz[S], zit[S];
+    double wsat[S], tkmg[S], tkdry[S], csol[S], bsw[S], sucsat[S], exc[S], sabg[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const int jj = i * G + k + 1;
+        const int j = jj - nlevsno;
+        const bool act = jj >= jtop && jj <= nlev;
+        t[i] = liq[i] = ice[i] = 0.0;
+        z[i] = zit[i] = 0.0;
+        dz[i] = 1.0;
+        wsat[i] = tkmg[i] = tkdry[i] = csol[i] = bsw[i] = sucsat[i] = exc[i] = sabg[i] = 0.0;
+        if (act) {
+            const int64_t o = c + (int64_t)(jj - 1) * ld;
+            t[i] = a.t_soisno[o];
+            liq[i] = a.h2osoi_liq[o];
+            ice[i] = a.h2osoi_ice[o];
+            z[i] = __builtin_nontemporal_load(&a.z[o]);
+            dz[i] = __builtin_nontemporal_load(&a.dz[o]);
+            zit[i] = __builtin_nontemporal_load(&a.zi[o]);          // zi[c, jj] (top interface)
+            if (j >= 1) {
+                const int64_t os = c + (int64_t)(j - 1) * ld;
+                wsat[i] = __builtin_nontemporal_load(&a.watsat[os]);
+                tkmg[i] = __builtin_nontemporal_load(&a.tkmg[os]);
+                tkdry[i] = __builtin_nontemporal_load(&a.tkdry[os]);
+                csol[i] = __builtin_nontemporal_load(&a.csol[os]);
+                bsw[i] = __builtin_nontemporal_load(&a.bsw[os]);
+                sucsat[i] = __builtin_nontemporal_load(&a.sucsat[os]);
+                exc[i] = a.excess_ice[os];
+            }
+            if (a.sabg_lyr_col && jj <= nlevsno + 1)
+                sabg[i] = a.sabg_lyr_col[c + (int64_t)(jj - 1) * ld];
+        }
+    }
+
+    // ---- thermal conductivity thk (:647-716) ----
+    double thk[S], bwv_s[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const int jj = i * G + k + 1;
+        const int j = jj - nlevsno;
+        const bool act = jj >= jtop && jj <= nlev;
+        double th = 0.0, bwv = 0.0;
+        if (act && j >= 1) {
+            double satw = (liq[i] / DENH2O + ice[i] / DENICE + exc[i] / DENICE) / (dz[i] * wsat[i]);
+            satw = jl_min(1.0, satw);
+            if (satw > 1.0e-7) {
+                double dke;
+                if (t[i] >= TFRZ) dke = jl_max(0.0, log10(satw) + 1.0);
+                else dke = satw;
+                const double fl_denom = liq[i] / (DENH2O * dz[i]) + ice[i] / (DENICE * dz[i]) +
+                                        exc[i] / (DENICE * dz[i]);
+                const double fl = (liq[i] / (DENH2O * dz[i])) / fl_denom;
+                const double dksat = tkmg[i] * pow(TKWAT, fl * wsat[i]) * pow(TKICE, (1.0 - fl) * wsat[i]);
+                th = dke * dksat + (1.0 - dke) * tkdry[i];
+            } else {
+                th = tkdry[i];
+            }
+            if (j > nbedrock) th = THK_BEDROCK;
+        }
+        if (act && snl + 1 < 1 && j <= 0) {
+            const double denom = jl_max(frac_sno, 1.0e-6) * jl_max(dz[i], 1.0e-6);
+            bwv = (ice[i] + liq[i]) / denom;
+            if (P.snow_method == 2) {
+                if (bwv <= 156.0) th = 0.023 + 0.234 * (bwv / 1000.0);
+                else th = 0.138 - 1.01 * (bwv / 1000.0) + 3.233 * ((bwv / 1000.0) * (bwv / 1000.0));
+            } else {
+                th = TKAIR + (7.75e-5 * bwv + 1.105e-6 * (bwv * bwv)) * (TKICE - TKAIR);
+            }
+        }
+        thk[i] = th;
+        bwv_s[i] = bwv;
+    }
+    // ---- heat capacity cv (:781-872) ----
+    double cv[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const int jj = i * G + k + 1;
+        const int j = jj - nlevsno;
+        double v = 1.0;
+        if (j >= 1) {
+            v = csol[i] * (1.0 - wsat[i]) * dz[i] + (ice[i] * CPICE + liq[i] * CPLIQ) + exc[i] * CPICE;
+            if (j > nbedrock) v = CSOL_BEDROCK * dz[i];
+            if (j == 1 && h2osno_nl > 0.0) v += CPICE * h2osno_nl;
+        } else if (jj >= jtop) {
+            if (frac_sno > 0.0) v = jl_max(THIN_SFCLAYER, (CPLIQ * liq[i] + CPICE * ice[i]) / frac_sno);
+            else v = THIN_SFCLAYER;
+        }
+        cv[i] = v;
+    }
+    // ---- interface conductivity tk, fact, fn (:605-629, :1271-1310) ----
+    double tk[S], fact[S], fn[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const int jj = i * G + k + 1;
+        const int j = jj - nlevsno;
+        const bool act = jj >= jtop && jj <= nlev;
+        const double thkn = hnxt<G>(thk, i, k, S);
+        const double zn = hnxt<G>(z, i, k, S);
+        const double zin = hnxt<G>(zit, i, k, S);
+        const double tn = hnxt<G>(t, i, k, S);
+        double tkv = 0.0, fa = 0.0, f = 0.0;
+        if (act) {
+            if (j <= nlevgrnd - 1)
+                tkv = thk[i] * thkn * (zn - z[i]) / (thk[i] * (zn - zin) + thkn * (zin - z[i]));
+            if (jj == jtop) {
+                fa = dtime / cv[i] * dz[i] / (0.5 * (z[i] - zit[i] + CAPR * (zn - zit[i])));
+                f = tkv * (tn - t[i]) / (zn - z[i]);
+            } else if (j <= nlevgrnd - 1) {
+                fa = dtime / cv[i];
+                f = tkv * (tn - t[i]) / (zn - z[i]);
+            } else {
+                fa = dtime / cv[i];
+                f = eflx_bot;
+            }
+        }
+        tk[i] = tkv;
+        fact[i] = fa;
+        fn[i] = f;
+    }
+    // ---- h2osfc row: conductance, capacity, RHS and band entries ----
+    const double thk1 = level_value<G, S>(thk, js1, k);
+    const double z1 = level_value<G, S>(z, js1, k);
+    const double t1 = level_value<G, S>(t, js1, k);
+    const double fact1 = level_value<G, S>(fact, js1, k);
+    double tk_h2osfc;
+    {
+        const double zh2osfc = 1.0e-3 * (0.5 * h2osfc);
+        const double denom = TKWAT * z1 + thk1 * zh2osfc;
+        tk_h2osfc = denom > 0.0 ? TKWAT * thk1 * (z1 + zh2osfc) / denom : thk1;
+    }
+    double c_h2osfc, dz_h2osfc;
+    if (h2osfc > THIN_SFCLAYER && frac_h2osfc > THIN_SFCLAYER) {
+        c_h2osfc = jl_max(THIN_SFCLAYER, CPLIQ * h2osfc / frac_h2osfc);
+        dz_h2osfc = jl_max(THIN_SFCLAYER, 1.0e-3 * h2osfc / frac_h2osfc);
+    } else {
+        c_h2osfc = THIN_SFCLAYER;
+        dz_h2osfc = THIN_SFCLAYER;
+    }
+    const double dzm_h = 0.5 * dz_h2osfc + z1;
+    const double fn_h2osfc = tk_h2osfc * (t1 - t_h2osfc) / dzm_h;
+    const double rv_h = t_h2osfc + (dtime / c_h2osfc) * (hs_h2osfc - dhsdT * t_h2osfc + cnfac * fn_h2osfc);
+    const double b3_h = 1.0 + omc * (dtime / c_h2osfc) * tk_h2osfc / dzm_h - (dtime / c_h2osfc) * dhsdT;
+    const double b2_h = -omc * (dtime / c_h2osfc) * tk_h2osfc / dzm_h;
+
+    // ---- level rows: RHS and bands (:1365-1748) ----
+    // sub = coupling to the level above (band 4, or band 5 snow-0 for soil 1),
+    // sup = coupling to the next level below (band 2, or band 1 for snow 0),
+    // b4h = soil-1 coupling to the h2osfc row (band 4).
+    double rv[S], b3[S], sub[S], sup[S];
+    double b4h = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const int jj = i * G + k + 1;
+        const int j = jj - nlevsno;
+        const bool act = jj >= jtop && jj <= nlev;
+        const double fnp = hprv<G>(fn, i, k);
+        const double tkp = hprv<G>(tk, i, k);
+        const double zp = hprv<G>(z, i, k);
+        const double zn = hnxt<G>(z, i, k, S);
+        double r = 0.0, d = 1.0, lo = 0.0, up = 0.0;
+        if (act) {
+            const double dzp = zn - z[i];
+            const double dzm = z[i] - zp;
+            if (j <= 0) {                                     // snow (_rhs_snow, _mat_snow)
+                if (jj == jtop) {
+                    r = t[i] + fact[i] * (hs_top_snow - dhsdT * t[i] + cnfac * fn[i]);
+                    d = 1.0 + omc * fact[i] * tk[i] / dzp - fact[i] * dhsdT;
+                } else {
+                    r = t[i] + cnfac * fact[i] * (fn[i] - fnp);
+                    r += fact[i] * sabg[i];
+                    lo = -omc * fact[i] * tkp / dzm;
+                    d = 1.0 + omc * fact[i] * (tk[i] / dzp + tkp / dzm);
+                }
+                up = -omc * fact[i] * tk[i] / dzp;            // band 2, or band 1 at j == 0
+            } else if (jj == jtop) {                          // soil 1, no snow
+                r = t[i] + fact[i] * (hs_top_snow - dhsdT * t[i] + cnfac * fn[i]);
+                d = 1.0 + omc * fact[i] * tk[i] / dzp - fact[i] * dhsdT;
+                up = -omc * fact[i] * tk[i] / dzp;
+            } else if (j == 1) {                              // soil 1 under snow
+                r = t[i] + fact[i] * ((1.0 - frac_sno) * (hs_soil - dhsdT * t[i]) +
+                                      cnfac * (fn[i] - frac_sno * fnp));
+                r += frac_sno * fact[i] * sabg[i];
+                up = -omc * fact[i] * tk[i] / dzp;
+                d = 1.0 + omc * fact[i] * (tk[i] / dzp + frac_sno * tkp / dzm) -
+                    (1.0 - frac_sno) * fact[i] * dhsdT;
+                lo = -frac_sno * omc * fact[i] * tkp / dzm;   // band 5 (snow 0)
+            } else if (j <= nlevgrnd - 1) {
+                r = t[i] + cnfac * fact[i] * (fn[i] - fnp);
+                up = -omc * fact[i] * tk[i] / dzp;
+                d = 1.0 + omc * fact[i] * (tk[i] / dzp + tkp / dzm);
+                lo = -omc * fact[i] * tkp / dzm;
+            } else {
+                r = t[i] - cnfac * fact[i] * fnp + fact[i] * fn[i];
+                d = 1.0 + omc * fact[i] * tkp / dzm;
+                lo = -omc * fact[i] * tkp / dzm;
+            }
+            if (j == 1 && frac_h2osfc != 0.0) {               // h2osfc corrections
+                r -= frac_h2osfc * fact[i] * ((hs_soil - dhsdT * t[i]) + cnfac * fn_h2osfc);
+                d += frac_h2osfc * (omc * fact[i] * tk_h2osfc / dzm_h + fact[i] * dhsdT);
+                b4h = -frac_h2osfc * omc * fact[i] * tk_h2osfc / dzm_h;
+            }
+        }
+        rv[i] = r;
+        b3[i] = d;
+        sub[i] = lo;
+        sup[i] = up;
+    }
+    // b4h stays in the soil-1 owner lane, which is the lane that eliminates that row
+
+    // ---- solve ----
+    double x[S];
+    double x_h = 0.0;
+    bool pivot = false;
+    if constexpr (!DENSE) {
+        double u0[S], y[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) { u0[i] = 1.0; y[i] = 0.0; x[i] = 0.0; }
+        double u0p = 0.0, yp = 0.0, supp = 0.0;             // previous level row
+        double u0_s0 = 0.0, y_s0 = 0.0, b1_s0 = 0.0;         // snow-0 row
+#pragma unroll
+        for (int jj = 1; jj <= NL; ++jj) {
+            const int i = (jj - 1) / G, kk = (jj - 1) % G;
+            if (jj < jtop || jj > nlev) continue;
+            double ou = 0.0, oy = 0.0, os = 0.0;
+            bool piv = false;
+            if (k == kk) {
+                double diag = b3[i], rhs = rv[i];
+                if (jj == js1) {
+                    if (jtop <= nlevsno) {                    // pivot column: snow 0
+                        piv = piv || (fabs(sub[i]) > fabs(u0_s0));
+                        const double f = sub[i] / u0_s0;
+                        diag = diag - f * b1_s0;
+                        rhs = rhs - f * y_s0;
+                    }
+                    piv = piv || (fabs(b4h) > fabs(b3_h));    // pivot column: h2osfc
+                    const double f2 = b4h / b3_h;
+                    diag = diag - f2 * b2_h;
+                    rhs = rhs - f2 * rv_h;
+                } else if (jj > jtop) {
+                    piv = piv || (fabs(sub[i]) > fabs(u0p));
+                    const double f = sub[i] / u0p;
+                    diag = diag - f * supp;
+                    rhs = rhs - f * yp;
+                }
+                u0[i] = diag;
+                y[i] = rhs;
+                ou = diag; oy = rhs; os = sup[i];
+            }
+            u0p = group_bcast<G>(ou, kk);
+            yp = group_bcast<G>(oy, kk);
+            supp = group_bcast<G>(os, kk);
+            pivot = pivot || (group_bcast<G>(piv ? 1.0 : 0.0, kk) != 0.0);
+            if (jj == nlevsno) { u0_s0 = u0p; y_s0 = yp; b1_s0 = supp; }
+        }
+        if (pivot) {
+            if (P.pivot_list && k == 0) {
+                const unsigned slot = atomicAdd(P.pivot_count, 1u);
+                P.pivot_list[slot] = c;
+            }
+            return;                                           // finished by the pivoting pass
+        }
+        double xn = 0.0;
+#pragma unroll
+        for (int jj = NL; jj >= 1; --jj) {
+            const int i = (jj - 1) / G, kk = (jj - 1) % G;
+            if (jj < jtop || jj > nlev) continue;
+            double ox = 0.0;
+            if (k == kk) {
+                double s = y[i];
+                if (jj < nlev) s = s - sup[i] * xn;
+                ox = s / u0[i];
+                x[i] = ox;
+            }
+            xn = group_bcast<G>(ox, kk);
+            if (jj == js1) x_h = (rv_h - b2_h * xn) / b3_h;
+        }
+    } else {
+        // dense elimination with partial pivoting, exactly as the reference
+        Dense D;
+        const int64_t jt = jtop;                             // first level in the system
+        const int n = nlev - jtop + 2;                       // levels + h2osfc row
+        // system row of level jj: jj - jtop + 1 (snow), jj - jtop + 2 (soil); h2osfc
+        // row: nlevsno - jtop + 2
+        const int rh = nlevsno - (int)jt + 2;
+        for (int q = 0; q < n * n; ++q) D.A[q] = 0.0;
+#define AD(r, cc) D.A[((r) - 1) + ((cc) - 1) * n]
+        for (int i = 0; i < S; ++i) {
+            const int jj = i + 1;
+            if (jj < jtop || jj > nlev) continue;
+            const int r = jj <= nlevsno ? jj - (int)jt + 1 : jj - (int)jt + 2;
+            AD(r, r) = b3[i];
+            D.x[r - 1] = rv[i];
+            if (jj <= nlevsno) {
+                if (jj > jtop) AD(r, r - 1) = sub[i];
+                if (jj < nlevsno) AD(r, r + 1) = sup[i];
+                else AD(r, r + 2) = sup[i];                  // band 1: snow 0 -> soil 1
+            } else if (jj == js1) {
+                if (jtop <= nlevsno) AD(r, r - 2) = sub[i];  // band 5
+                AD(r, r - 1) = b4h;                          // band 4
+                if (jj < nlev) AD(r, r + 1) = sup[i];
+            } else {
+                AD(r, r - 1) = sub[i];
+                if (jj < nlev) AD(r, r + 1) = sup[i];
+            }
+        }
+        AD(rh, rh) = b3_h;
+        AD(rh, rh + 1) = b2_h;
+        D.x[rh - 1] = rv_h;
+        for (int kq = 1; kq <= n - 1; ++kq) {
+            int pv = kq;
+            double maxv = fabs(AD(kq, kq));
+            for (int r = kq + 1; r <= n; ++r) {
+                const double v = fabs(AD(r, kq));
+                if (v > maxv) { maxv = v; pv = r; }
+            }
+            const double akk = AD(pv, kq);
+            if (pv != kq) {
+                for (int cc = kq; cc <= n; ++cc) {
+                    const double akj = AD(kq, cc), apj = AD(pv, cc);
+                    AD(kq, cc) = apj;
+                    AD(pv, cc) = akj;
+                }
+                const double xk = D.x[kq - 1], xp = D.x[pv - 1];
+                D.x[kq - 1] = xp;
+                D.x[pv - 1] = xk;
+            }
+            for (int r = kq + 1; r <= n; ++r) {
+                const double f = AD(r, kq) / akk;
+                for (int cc = kq; cc <= n; ++cc) AD(r, cc) = AD(r, cc) - f * AD(kq, cc);
+                D.x[r - 1] = D.x[r - 1] - f * D.x[kq - 1];
+            }
+        }
+        for (int r = n; r >= 1; --r) {
+            double s = D.x[r - 1];
+            for (int cc = r + 1; cc <= n; ++cc) s = s - AD(r, cc) * D.x[cc - 1];
+            D.x[r - 1] = s / AD(r, r);
+        }
+        for (int i = 0; i < S; ++i) {
+            const int jj = i + 1;
+            x[i] = 0.0;
+            if (jj < jtop || jj > nlev) continue;
+            const int r = jj <= nlevsno ? jj - (int)jt + 1 : jj - (int)jt + 2;
+            x[i] = D.x[r - 1];
+        }
+        x_h = D.x[rh - 1];
+#undef AD
+    }
+
+    // ---- unpack (:149-167) and fn1 (:170-206) ----
+    double tnew[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const int jj = i * G + k + 1;
+        tnew[i] = (jj >= jtop && jj <= nlev) ? x[i] : t[i];
+    }
+    const double x_s1 = level_value<G, S>(tnew, js1, k);
+    t_h2osfc = (frac_h2osfc == 0.0) ? x_s1 : x_h;
+    double fn1[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const int jj = i * G + k + 1;
+        const int j = jj - nlevsno;
+        const double tn = hnxt<G>(tnew, i, k, S);
+        const double zn = hnxt<G>(z, i, k, S);
+        double v = 0.0;
+        if (jj >= jtop && j <= nlevgrnd - 1) v = tk[i] * (tn - tnew[i]) / (zn - z[i]);
+        fn1[i] = v;
+    }
+
+    // ---- phase_change_h2osfc! (:1757-1826) ----
+    double xmf_h2osfc = 0.0, qflx_h2osfc_to_ice = 0.0, eflx_h2osfc_to_snow = 0.0;
+    bool wrote_t0 = false;
+    if (frac_h2osfc > 0.0 && t_h2osfc <= TFRZ) {
+        wrote_t0 = true;
+        const double fact0 = level_value<G, S>(fact, nlevsno, k);   // snow layer 0
+        double t0 = level_value<G, S>(tnew, nlevsno, k);
+        double ice0 = level_value<G, S>(ice, nlevsno, k);
+        const double tinc = TFRZ - t_h2osfc;
+        t_h2osfc = TFRZ;
+        const double hm = frac_h2osfc * (dhsdT * tinc - tinc * c_h2osfc / dtime);
+        const double xm = hm * dtime / HFUS;
+        const double temp1 = h2osfc + xm;
+        double h2osno_total = h2osno_nl;
+        if (snl < 0) {
+#pragma unroll
+            for (int jj = 1; jj <= MAX_NLEVSNO; ++jj) {
+                if (jj < jtop || jj > nlevsno) continue;
+                const int i = (jj - 1) / G, kk = (jj - 1) % G;
+                const double w = group_bcast<G>(k == kk ? ice[i] + liq[i] : 0.0, kk);
+                h2osno_total += w;
+            }
+        }
+        const double z_avg = frac_sno * snow_depth;
+        double rho_avg = z_avg > 0.0 ? jl_min(800.0, h2osno_total / z_avg) : 200.0;
+        if (temp1 >= 0.0) {
+            int_snow -= xm;
+            if (snl == 0) h2osno_nl -= xm;
+            else ice0 -= xm;
+            h2osno_total -= xm;
+            h2osfc += xm;
+            xmf_h2osfc = hm;
+            qflx_h2osfc_to_ice = -xm / dtime;
+            if (frac_sno > 0.0 && snl < 0) snow_depth = h2osno_total / (rho_avg * frac_sno);
+            else snow_depth = h2osno_total / DENICE;
+            if (snl == 0) {
+                t0 = t_h2osfc;
+                eflx_h2osfc_to_snow = 0.0;
+            } else {
+                double c1;
+                if (snl == -1) c1 = frac_sno * (dtime / fact0 - dhsdT * dtime);
+                else c1 = frac_sno / fact0 * dtime;
+                const double c2 = frac_h2osfc != 0.0 ? (-CPLIQ * xm - frac_h2osfc * dhsdT * dtime) : 0.0;
+                t0 = (c1 * t0 + c2 * t_h2osfc) / (c1 + c2);
+                eflx_h2osfc_to_snow = (t_h2osfc - t0) * c2 / dtime;
+            }
+        } else {
+            rho_avg = (h2osno_total * rho_avg + h2osfc * DENICE) / (h2osno_total + h2osfc);
+            int_snow += h2osfc;
+            if (snl == 0) h2osno_nl += h2osfc;
+            else ice0 += h2osfc;
+            h2osno_total += h2osfc;
+            qflx_h2osfc_to_ice = h2osfc / dtime;
+            t_h2osfc -= temp1 * HFUS / (dtime * dhsdT - c_h2osfc);
+            xmf_h2osfc = hm - frac_h2osfc * temp1 * HFUS / dtime;
+            if (snl == 0) {
+                t0 = t_h2osfc;
+            } else if (snl == -1) {
+                const double c1 = frac_sno * (dtime / fact0 - dhsdT * dtime);
+                const double c2 = frac_h2osfc != 0.0 ? frac_h2osfc * (c_h2osfc - dtime * dhsdT) : 0.0;
+                t0 = (c1 * t0 + c2 * t_h2osfc) / (c1 + c2);
+                t_h2osfc = t0;
+            } else {
+                const double c1 = frac_sno / fact0 * dtime;
+                const double c2 = frac_h2osfc != 0.0 ? frac_h2osfc * (c_h2osfc - dtime * dhsdT) : 0.0;
+                t0 = (c1 * t0 + c2 * t_h2osfc) / (c1 + c2);
+                t_h2osfc = t0;
+            }
+            h2osfc = 0.0;
+            if (frac_sno > 0.0 && snl < 0) snow_depth = h2osno_total / (rho_avg * frac_sno);
+            else snow_depth = h2osno_total / DENICE;
+        }
+        // write back the snow-layer-0 slot (level nlevsno); for snl == 0 this is the
+        // inactive slot the reference also writes (t_soisno[c, joff])
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+            if (i * G + k + 1 == nlevsno) { tnew[i] = t0; ice[i] = ice0; }
+    }
+
+    // ---- phase_change_beta! (:1929-2289), Float64 (hard) branch ----
+    const double tfrz = TFRZ, hfus = HFUS;
+    int imelt[S];
+    double xm_a[S], wice0[S], wexice0[S], wmass0[S], tinc[S], dxmf[S], qmelt_l[S], qfrz_l[S];
+    double h2osno_nl_1 = h2osno_nl, snow_depth_1 = snow_depth, qsm_set = 0.0;
+    bool unresolved = false;
+    bool tgt_written[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const int jj = i * G + k + 1;
+        const int j = jj - nlevsno;
+        const bool act = jj >= jtop && jj <= nlev;
+        imelt[i] = 0;
+        wice0[i] = ice[i];
+        wexice0[i] = j >= 1 ? exc[i] : 0.0;
+        wmass0[i] = ice[i] + liq[i] + wexice0[i];
+        tinc[i] = 0.0;
+        xm_a[i] = 0.0;
+        dxmf[i] = qmelt_l[i] = qfrz_l[i] = 0.0;
+        tgt_written[i] = false;
+        if (!act) continue;
+        double tt = tnew[i];
+        double supercool = 0.0;
+        if (j <= 0) {
+            if (ice[i] > 0.0 && tt > tfrz) { imelt[i] = 1; tinc[i] = tfrz - tt; tt = tfrz; }
+            if (liq[i] > 0.0 && tt < tfrz) { imelt[i] = 2; tinc[i] = tfrz - tt; tt = tfrz; }
+        } else {
+            if (ice[i] > 0.0 && tt > tfrz) { imelt[i] = 1; tinc[i] = tfrz - tt; tt = tfrz; }
+            if (exc[i] > 0.0 && tt > tfrz) { imelt[i] = 1; tinc[i] = tfrz - tt; tt = tfrz; }
+            if (tt < tfrz) {
+                const double smp = hfus * (tfrz - tt) / (GRAV * tt) * 1000.0;
+                supercool = wsat[i] * pow(smp / sucsat[i], -1.0 / bsw[i]);
+                supercool *= dz[i] * 1000.0;
+            }
+            if (liq[i] > supercool && tt < tfrz) { imelt[i] = 2; tinc[i] = tfrz - tt; tt = tfrz; }
+            if (h2osno_nl > 0.0 && j == 1) {
+                if (tt > tfrz) { imelt[i] = 1; tinc[i] = tfrz - tt; tt = tfrz; }
+            }
+        }
+        // energy surplus/loss and rate of phase change
+        double hm = 0.0;
+        const double fct = fact[i];
+        if (imelt[i] > 0) {
+            if (jj == jtop) {
+                if (j > 0) hm = dhsdT * tinc[i] - tinc[i] / fct;
+                else hm = frac_sno * (dhsdT * tinc[i] - tinc[i] / fct);
+                if (j == 1 && frac_h2osfc != 0.0) hm -= frac_h2osfc * dhsdT * tinc[i];
+            } else if (j == 1) {
+                hm = (1.0 - frac_sno - frac_h2osfc) * dhsdT * tinc[i] - tinc[i] / fct;
+            } else {
+                if (j < 1) hm = -frac_sno * tinc[i] / fct;
+                else hm = -tinc[i] / fct;
+            }
+        }
+        if (imelt[i] == 1 && hm < 0.0) { hm = 0.0; imelt[i] = 0; }
+        if (imelt[i] == 2 && hm > 0.0) { hm = 0.0; imelt[i] = 0; }
+        double icev = ice[i], liqv = liq[i], excv = exc[i];
+        if (imelt[i] > 0 && fabs(hm) > 0.0) {
+            double xm = hm * dtime / hfus;
+            if (j == 1 && h2osno_nl > 0.0 && xm > 0.0) {
+                const double temp1 = h2osno_nl;
+                h2osno_nl_1 = jl_max(0.0, temp1 - xm);
+                const double propor = h2osno_nl_1 / temp1;
+                snow_depth_1 = propor * snow_depth;
+                const double heatr = hm - hfus * (temp1 - h2osno_nl_1) / dtime;
+                if (heatr > 0.0) { xm = heatr * dtime / hfus; hm = heatr; }
+                else { xm = 0.0; hm = 0.0; }
+                qsm_set = jl_max(0.0, temp1 - h2osno_nl_1) / dtime;
+                unresolved = true;
+            }
+            double heatr = 0.0;
+            if (xm > 0.0) {
+                icev = jl_max(0.0, wice0[i] - xm);
+                heatr = hm - hfus * (wice0[i] - icev) / dtime;
+                const double xm2 = xm - icev;
+                if (icev == 0.0) {
+                    if (wexice0[i] >= 0.0 && xm2 > 0.0 && j >= 2) {
+                        excv = jl_max(0.0, wexice0[i] - xm2);
+                        heatr = hm - hfus * (wexice0[i] - excv + wice0[i] - icev) / dtime;
+                    }
+                }
+            } else if (xm < 0.0) {
+                if (j <= 0) {
+                    icev = jl_min(wmass0[i], wice0[i] - xm);
+                } else {
+                    if (wmass0[i] - wexice0[i] < supercool) icev = 0.0;
+                    else icev = jl_min(wmass0[i] - wexice0[i] - supercool, wice0[i] - xm);
+                }
+                heatr = hm - hfus * (wice0[i] - icev) / dtime;
+            }
+            const double ei_val = j >= 1 ? excv : 0.0;
+            liqv = jl_max(0.0, wmass0[i] - icev - ei_val);
+            if (fabs(heatr) > 0.0) {
+                if (jj == jtop) {
+                    if (j == 1) {
+                        tt += fct * heatr / (1.0 - (1.0 - frac_h2osfc) * fct * dhsdT);
+                    } else {
+                        if (frac_sno > 0.0) tt += (fct / frac_sno) * heatr / (1.0 - fct * dhsdT);
+                    }
+                } else if (j == 1) {
+                    tt += fct * heatr / (1.0 - (1.0 - frac_sno - frac_h2osfc) * fct * dhsdT);
+                } else {
+                    if (j > 0) tt += fct * heatr;
+                    else if (frac_sno > 0.0) tt += (fct / frac_sno) * heatr;
+                }
+                if (j <= 0)
+                    if (liqv * icev > 0.0) tt = tfrz;
+            }
+            if (j >= 1) {
+                dxmf[i] = hfus * (wice0[i] - icev) / dtime + hfus * (wexice0[i] - excv) / dtime;
+            } else {
+                dxmf[i] = hfus * (wice0[i] - icev) / dtime;
+            }
+            if (imelt[i] == 1 && j < 1) qmelt_l[i] = jl_max(0.0, wice0[i] - icev) / dtime;
+            if (imelt[i] == 2 && j < 1) qfrz_l[i] = jl_max(0.0, icev - wice0[i]) / dtime;
+            tgt_written[i] = true;
+        }
+        tnew[i] = tt;
+        ice[i] = icev;
+        liq[i] = liqv;
+        exc[i] = excv;
+    }
+    // column accumulators, summed over levels in the reference's order
+    double xmf = 0.0, qflx_snomelt = 0.0, qflx_snofrz = 0.0, qflx_snow_drain = 0.0;
+    {
+        const bool ur = group_bcast<G>(unresolved ? 1.0 : 0.0, (js1 - 1) % G) != 0.0;
+        const double qs = group_bcast<G>(qsm_set, (js1 - 1) % G);
+        if (ur) {
+            h2osno_nl = group_bcast<G>(h2osno_nl_1, (js1 - 1) % G);
+            snow_depth = group_bcast<G>(snow_depth_1, (js1 - 1) % G);
+        }
+#pragma unroll
+        for (int jj = 1; jj <= NL; ++jj) {
+            const int i = (jj - 1) / G, kk = (jj - 1) % G;
+            if (jj < jtop || jj > nlev) continue;
+            const bool own = (k == kk);
+            const double wr = group_bcast<G>(own && tgt_written[i] ? 1.0 : 0.0, kk);
+            if (wr == 0.0) continue;
+            if (jj == js1 && ur) {
+                qflx_snomelt = qs;
+                xmf = hfus * qflx_snomelt;
+                qflx_snow_drain = qflx_snomelt;
+            }
+            xmf += group_bcast<G>(own ? dxmf[i] : 0.0, kk);
+            if (jj <= nlevsno) {
+                const double qm = group_bcast<G>(own ? qmelt_l[i] : 0.0, kk);
+                const double qf = group_bcast<G>(own ? qfrz_l[i] : 0.0, kk);
+                const int im = (int)group_bcast<G>(own ? (double)imelt[i] : 0.0, kk);
+                if (im == 1) {
+                    qflx_snomelt += qm;
+                    snomelt_accum += qm * dtime * 1.0e-3;
+                }
+                if (im == 2) qflx_snofrz += qf;
+            }
+        }
+    }
+
+    // ---- ground temperature (:224-247) and soil heat flux (:250-269) ----
+    const double t_top = level_value<G, S>(tnew, jtop, k);
+    const double t_s1 = level_value<G, S>(tnew, js1, k);
+    double tg;
+    if (snl < 0) {
+        if (frac_h2osfc != 0.0)
+            tg = frac_sno * t_top + (1.0 - frac_sno - frac_h2osfc) * t_s1 + frac_h2osfc * t_h2osfc;
+        else
+            tg = frac_sno * t_top + (1.0 - frac_sno) * t_s1;
+    } else {
+        if (frac_h2osfc != 0.0) tg = (1.0 - frac_h2osfc) * t_s1 + frac_h2osfc * t_h2osfc;
+        else tg = t_s1;
+    }
+    const double fn_s1 = level_value<G, S>(fn, js1, k);
+    const double fn1_s1 = level_value<G, S>(fn1, js1, k);
+
+    // ---- write back ----
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const int jj = i * G + k + 1;
+        const int j = jj - nlevsno;
+        const bool act = jj >= jtop && jj <= nlev;
+        const int64_t o = c + (int64_t)(jj - 1) * ld;
+        if (act) {
+            a.t_soisno[o] = tnew[i];
+            a.h2osoi_liq[o] = liq[i];
+            a.h2osoi_ice[o] = ice[i];
+            if (j >= 1 && exc[i] != wexice0[i]) a.excess_ice[c + (int64_t)(j - 1) * ld] = exc[i];
+            if (j <= 0) {
+                a.qflx_snomelt_lyr[o] = qmelt_l[i];
+                a.qflx_snofrz_lyr[o] = qfrz_l[i];
+            }
+            if (a.fact) a.fact[o] = fact[i];
+            if (a.thk) a.thk[o] = thk[i];
+            if (a.imelt) a.imelt[o] = imelt[i];
+            if (a.bw && j <= 0) a.bw[o] = bwv_s[i];
+            if (j >= 1) {
+                const int64_t os = c + (int64_t)(j - 1) * ld;
+                if (a.eflx_fgr)
+                    a.eflx_fgr[os] = (j < nlevgrnd) ? -cnfac * fn[i] - (1.0 - cnfac) * fn1[i] : 0.0;
+                if (a.exice_subs)
+                    a.exice_subs[os] = tgt_written[i] ? jl_max(0.0, (wexice0[i] - exc[i]) / DENICE) : 0.0;
+            }
+        } else if (jj == nlevsno && wrote_t0) {
+            // phase_change_h2osfc! writes t_soisno[c, joff] also without snow layers
+            a.t_soisno[o] = tnew[i];
+        }
+        if (!act && j <= 0 && jj >= 1) {
+            a.qflx_snomelt_lyr[o] = 0.0;
+            a.qflx_snofrz_lyr[o] = 0.0;
+        }
+    }
+    if (k == 0) {
+        a.t_h2osfc[c] = t_h2osfc;
+        a.h2osfc[c] = h2osfc;
+        a.h2osno_no_layers[c] = h2osno_nl;
+        a.snow_depth[c] = snow_depth;
+        if (a.int_snow) a.int_snow[c] = int_snow;
+        a.t_grnd[c] = tg;
+        a.xmf[c] = xmf;
+        a.xmf_h2osfc[c] = xmf_h2osfc;
+        a.qflx_snomelt[c] = qflx_snomelt;
+        a.qflx_snofrz[c] = qflx_snofrz;
+        a.qflx_snow_drain[c] = qflx_snow_drain;
+        a.qflx_h2osfc_to_ice[c] = qflx_h2osfc_to_ice;
+        a.eflx_fgr12[c] = -cnfac * fn_s1 - (1.0 - cnfac) * fn1_s1;
+        if (a.c_h2osfc) a.c_h2osfc[c] = c_h2osfc;
+        if (a.eflx_snomelt) a.eflx_snomelt[c] = qflx_snomelt * hfus;
+        if (a.eflx_snomelt_r) a.eflx_snomelt_r[c] = qflx_snomelt * hfus;
+        if (a.eflx_h2osfc_to_snow) a.eflx_h2osfc_to_snow[c] = eflx_h2osfc_to_snow;
+        if (a.snomelt_accum) a.snomelt_accum[c] = snomelt_accum;
+    }
+}
+
+template <int G>
+static hipError_t launch_g(const HeatParams &P, hipStream_t stream, int block) {
+    const int64_t threads = P.a.nc * G;
+    const int64_t grid = (threads + block - 1) / block;
+    hipLaunchKernelGGL((heat_kernel<G, false>), dim3((unsigned)grid), dim3(block), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_heat(const HeatParams &P, int G, int block, hipStream_t stream) {
+    if (P.a.nc == 0) return hipSuccess;
+    switch (G) {
+        case 1: return launch_g<1>(P, stream, block);
+        case 2: return launch_g<2>(P, stream, block);
+        default: return launch_g<4>(P, stream, block);
+    }
+}
+
+hipError_t launch_heat_pivot(const HeatParams &P, hipStream_t stream) {
+    if (P.a.nc == 0) return hipSuccess;
+    const int block = 64;
+    const int64_t grid = (P.a.nc + block - 1) / block;
+    hipLaunchKernelGGL((heat_kernel<1, true>), dim3((unsigned)grid), dim3(block), 0, stream, P);
+    return hipGetLastError();
+}
+
+}  // namespace clmsoil

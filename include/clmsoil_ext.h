@@ -1,0 +1,44 @@
+/*
+ * clmsoil_ext.h — entry points of libclmsoil with no direct counterpart in
+ * CLM.jl's interfaces: device status, launch tuning, caller-provided workspace.
+ * Conventions as in clmsoil.h.
+ */
+#ifndef CLMSOIL_EXT_H
+#define CLMSOIL_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "clmsoil.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Device error bits recorded by the kernels. */
+#define CLMSOIL_DEVERR_NBEDROCK 1u   /* nbedrock (or jtop) outside 1..nlevsoi      */
+#define CLMSOIL_DEVERR_SNL      2u   /* snl outside -nlevsno..0 (or band range)    */
+#define CLMSOIL_DEVERR_ITER     4u   /* substep iteration cap reached              */
+
+/* Synchronize `stream`, return and clear the current device's error bits. The
+ * reference raises these conditions as Julia bounds errors inside its kernels. */
+int clmsoil_device_errors(void *stream, uint32_t *flags);
+
+/* Lanes per column (1, 2 or 4) and workgroup size (multiple of 64) of the fused
+ * Richards and heat kernels; process-wide, read at launch. */
+int clmsoil_set_tuning(int richards_g, int richards_block, int heat_g, int heat_block);
+int clmsoil_get_tuning(int *richards_g, int *richards_block, int *heat_g, int *heat_block);
+
+/* Heat step with a caller-provided device workspace of at least
+ * clmsoil_heat_workspace_size(nc) bytes (pivot list); allocates nothing.
+ * clmsoil_soil_temperature instead uses a library-owned per-device list that is
+ * allocated on first use and grown when nc grows. */
+size_t clmsoil_heat_workspace_size(int64_t nc);
+int clmsoil_soil_temperature_ws(const clmsoil_heat_args *a, const clmsoil_levels *lev,
+                                double dtime, const clmsoil_heat_config *cfg,
+                                void *workspace, size_t workspace_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

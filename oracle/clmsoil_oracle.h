@@ -1,0 +1,83 @@
+/*
+ * clmsoil_oracle.h — CPU restatement of CLM.jl's per-column soil solves.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the parity oracle and the CPU
+ * baseline. Only the tests, the smoke check and the benchmark's CPU-baseline leg
+ * may load it; the product library (libclmsoil) never calls or links it.
+ *
+ * Every function follows the statement order of the CLM.jl source it cites
+ * (paths relative to the CLM.jl tree) using the same argument structs as the
+ * product C ABI (include/clmsoil.h), with host pointers instead of device ones.
+ * It is compiled with -ffp-contract=off so that, like Julia, a*b+c rounds twice.
+ */
+#ifndef CLMSOIL_ORACLE_H
+#define CLMSOIL_ORACLE_H
+
+#include "../include/clmsoil.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Julia Base.max / Base.min on Float64 (NaN-propagating, +0 > -0). */
+double orc_jl_max(double x, double y);
+double orc_jl_min(double x, double y);
+
+/* Clapp-Hornberger SWRC, swrc_clapp_hornberg.jl:43-55, 76-88, 109-117. */
+void   orc_soil_hk(double hksat, double bsw, double s, double imped,
+                   double *hk, double *dhkds);
+void   orc_soil_suction(double bsw, double sucsat, double s,
+                        double *smp, double *dsmpds);
+double orc_soil_suction_inverse(double bsw, double sucsat, double smp_target);
+/* ice_impedance, soil_water_movement.jl:722-724 */
+double orc_ice_impedance(double icefrac, double e_ice);
+
+/* Scalar helpers, soil_water_movement.jl:736-773, 1080-1140 (1-D vectors, 0-based
+ * storage of the reference's 1-based levels). */
+void orc_tridiagonal_col(double *u, const double *a, const double *b,
+                         const double *c, const double *r, int jtop, int lbj, int ubj);
+void orc_compute_rhs_moisture_form(int nlayers, const double *vert_trans_sink,
+                                   const double *qin, const double *qout,
+                                   const double *dt_dz, double *rmx);
+void orc_compute_lhs_moisture_form(int nlayers, const double *dt_dz,
+                                   const double *dqidw0, const double *dqidw1,
+                                   const double *dqodw1, const double *dqodw2,
+                                   double *amx, double *bmx, double *cmx);
+
+/* tridiagonal_solve!, tridiagonal.jl:35-63 (single column, no pivot guard). */
+void orc_tridiagonal_solve(double *u, const double *a, const double *b,
+                           const double *c, const double *r, int jtop, int nlevs);
+
+/* Batched operators: same signatures as the C ABI (host memory). */
+int orc_tridiagonal_multi(double *u, const double *a, const double *b,
+                          const double *c, const double *r, const int64_t *jtop,
+                          const uint8_t *mask, int64_t ncols, int64_t nlevs,
+                          int64_t ld);
+int orc_batched_band_solve(double *tvector, const double *bmatrix,
+                           const double *rvector, const int64_t *jtop,
+                           const int64_t *jbot, const uint8_t *mask, int32_t kl,
+                           int32_t ku, int32_t nlevsno_off, int64_t nc,
+                           int64_t nlev, int64_t ld);
+
+/* Moisture-form Richards step, soil_water_movement.jl:1250-1517.
+ * dbg_qin/dbg_qout (optional, [ld, nlevsoi]) receive the last iteration's
+ * interface fluxes, for the telescoping identity qin[j] == qout[j-1]. */
+int orc_soilwm_moisture_form_solve(const clmsoil_mf_args *a,
+                                   const clmsoil_levels *lev, double dtime,
+                                   const clmsoil_swm_config *cfg,
+                                   double *dbg_qin, double *dbg_qout);
+/* soilwater_moisture_form!, soil_water_movement.jl:1532-1601. */
+int orc_soilwater_moisture_form(const clmsoil_mf_args *a, const clmsoil_levels *lev,
+                                double dtime, const clmsoil_swm_config *cfg);
+
+/* Non-urban soil_temperature!, soil_temperature.jl:358-593. */
+int orc_soil_temperature(const clmsoil_heat_args *a, const clmsoil_levels *lev,
+                         double dtime, const clmsoil_heat_config *cfg);
+
+/* Number of OpenMP threads the batched functions use. */
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

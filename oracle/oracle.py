@@ -1,0 +1,98 @@
+"""ctypes loader of the CPU oracle (liboracle_clmsoil.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, never by the clmsoil package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+ORACLE_DIR = Path(__file__).resolve().parent
+REPO = ORACLE_DIR.parent
+sys.path.insert(0, str(REPO / "clm.jl_amd"))
+
+from clmsoil import _abi  # noqa: E402
+from clmsoil.bind import byref, heat_args, levels, mf_args  # noqa: E402
+
+LIB_PATH = ORACLE_DIR / "liboracle_clmsoil.so"
+_lib = None
+
+
+def build(force: bool = False) -> None:
+    cmd = ["make", "-C", str(ORACLE_DIR)] + (["-B"] if force else [])
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        _abi.declare_common(L, "orc_")
+        L.orc_soilwm_moisture_form_solve.argtypes = [C.POINTER(_abi.MfArgs), C.POINTER(_abi.Levels),
+                                                     C.c_double, C.POINTER(_abi.SwmConfig),
+                                                     C.c_void_p, C.c_void_p]
+        L.orc_soilwm_moisture_form_solve.restype = C.c_int
+        L.orc_jl_max.argtypes = [C.c_double, C.c_double]
+        L.orc_jl_max.restype = C.c_double
+        L.orc_jl_min.argtypes = [C.c_double, C.c_double]
+        L.orc_jl_min.restype = C.c_double
+        D = C.c_double
+        PD = C.POINTER(C.c_double)
+        L.orc_soil_hk.argtypes = [D, D, D, D, PD, PD]
+        L.orc_soil_suction.argtypes = [D, D, D, PD, PD]
+        L.orc_soil_suction_inverse.argtypes = [D, D, D]
+        L.orc_soil_suction_inverse.restype = D
+        L.orc_ice_impedance.argtypes = [D, D]
+        L.orc_ice_impedance.restype = D
+        L.orc_tridiagonal_col.argtypes = [C.c_void_p] * 5 + [C.c_int] * 3
+        L.orc_compute_rhs_moisture_form.argtypes = [C.c_int] + [C.c_void_p] * 5
+        L.orc_compute_lhs_moisture_form.argtypes = [C.c_int] + [C.c_void_p] * 8
+        L.orc_tridiagonal_solve.argtypes = [C.c_void_p] * 5 + [C.c_int] * 2
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        raise RuntimeError(f"oracle {what} failed with status {status}")
+
+
+def soilwm_moisture_form_solve(d: dict, cfg, dbg=None) -> None:
+    """Run the moisture-form step on the host arrays of `d` (updated in place)."""
+    a = mf_args(d)
+    lev = levels(d)
+    qin = qout = None
+    if dbg is not None:
+        qin, qout = dbg["qin"], dbg["qout"]
+    st = lib().orc_soilwm_moisture_form_solve(byref(a), byref(lev), float(d["dtime"]), byref(cfg),
+                                              None if qin is None else qin.ctypes.data,
+                                              None if qout is None else qout.ctypes.data)
+    check(st, "soilwm_moisture_form_solve")
+
+
+def soilwater_moisture_form(d: dict, cfg) -> None:
+    a = mf_args(d)
+    lev = levels(d)
+    check(lib().orc_soilwater_moisture_form(byref(a), byref(lev), float(d["dtime"]), byref(cfg)),
+          "soilwater_moisture_form")
+
+
+def soil_temperature(d: dict, hcfg) -> None:
+    a = heat_args(d)
+    lev = levels(d)
+    check(lib().orc_soil_temperature(byref(a), byref(lev), float(d["dtime"]), byref(hcfg)),
+          "soil_temperature")
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+os.environ.setdefault("OMP_NUM_THREADS", str(os.cpu_count() or 1))

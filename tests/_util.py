@@ -1,0 +1,27 @@
+"""Shared helpers for the parity tests."""
+from __future__ import annotations
+
+import numpy as np
+
+
+def rel_err(a, b, floor=0.0):
+    """max |a-b| / max(|a|, |b|, floor) over all elements (0 where both are 0,
+    NaN positions must coincide)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return np.inf
+    m = ~na
+    a, b = a[m], b[m]
+    if a.size == 0:
+        return 0.0
+    den = np.maximum(np.maximum(np.abs(a), np.abs(b)), floor)
+    d = np.abs(a - b)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        r = np.where(d == 0, 0.0, d / den)
+    return float(r.max())
+
+
+def clone(d):
+    return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in d.items()}

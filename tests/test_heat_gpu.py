@@ -1,0 +1,109 @@
+"""HIP snow+soil heat step vs the oracle (same seeded inputs).
+
+Bar: <= 1e-11 relative (|a-b|/max(|a|,|b|)) on t_soisno, h2osoi_liq/ice, the
+qflx_* outputs and every other output; imelt identical."""
+import numpy as np
+import pytest
+
+import oracle
+from _util import clone, rel_err
+from clmsoil import api, bind, inputs
+
+TOL = 1e-11
+OUT = ["t_soisno", "h2osoi_liq", "h2osoi_ice", "excess_ice", "t_h2osfc", "h2osfc",
+       "h2osno_no_layers", "snow_depth", "int_snow", "t_grnd", "xmf", "xmf_h2osfc", "qflx_snomelt",
+       "qflx_snofrz", "qflx_snow_drain", "qflx_h2osfc_to_ice", "eflx_fgr12", "qflx_snomelt_lyr",
+       "qflx_snofrz_lyr", "fact", "c_h2osfc", "thk", "bw", "eflx_fgr", "exice_subs",
+       "eflx_snomelt", "eflx_snomelt_r", "eflx_h2osfc_to_snow", "snomelt_accum"]
+# sums over layers with both signs (melt and freeze) can cancel: compared
+# relative to the magnitude of their largest term (W/m2 scale)
+SUMS = {"xmf": 1.0e3, "eflx_fgr12": 1.0, "eflx_fgr": 1.0}
+
+
+def run_gpu(d, method="Jordan1991"):
+    import torch
+    t = api.to_device(d)
+    st = {k: v for k, v in t.items() if not isinstance(v, (int, float))}
+    st["nc"] = d["nc"]
+    api.soil_temperature(st, d["dtime"], d["nlevsno"], d["nlevgrnd"], d["nlevsoi"], method)
+    torch.cuda.synchronize()
+    assert api.device_errors() == 0
+    return api.to_host(t)
+
+
+def compare(got, ref, keys=OUT):
+    bad = {}
+    for k in keys:
+        if k not in ref:
+            continue
+        if k == "imelt":
+            continue
+        e = rel_err(got[k], ref[k], floor=SUMS.get(k, 0.0))
+        if e > TOL:
+            bad[k] = e
+    assert not bad, bad
+    assert np.array_equal(got["imelt"], ref["imelt"])
+
+
+def heat_case(nc, seed, variant=False, nbedrock=None):
+    h = inputs.heat_config1(nc=nc, seed=seed, h2osfc_variant=variant, nbedrock=nbedrock)
+    h.update(inputs.heat_outputs(h))
+    return h
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("G", [1, 2, 4])
+@pytest.mark.parametrize("variant", [False, True])
+def test_config1_vs_oracle(gpu, G, variant):
+    h = heat_case(8192, 30 + int(variant), variant, nbedrock="random" if variant else None)
+    ref = clone(h)
+    oracle.soil_temperature(ref, bind.heat_config())
+    api.set_tuning(4, 256, G, 256)
+    compare(run_gpu(h), ref)
+
+
+@pytest.mark.gpu
+def test_sturm_snow_conductivity(gpu):
+    h = heat_case(4096, 33)
+    ref = clone(h)
+    oracle.soil_temperature(ref, bind.heat_config(2))
+    api.set_tuning(4, 256, 4, 256)
+    compare(run_gpu(h, "Sturm1997"), ref)
+
+
+def pivot_columns(h, every=3):
+    """Make every `every`-th column need a row interchange: a soil layer with an
+    almost-zero heat capacity gives its row a sub-diagonal far larger than the
+    pivot above it (band_diagonal.jl:125-140 then swaps rows)."""
+    ns = h["nlevsno"]
+    cols = np.arange(0, h["nc"], every)
+    for j in (5, 9):
+        h["csol"][j - 1, cols] = 1.0e-3
+        h["h2osoi_liq"][ns + j - 1, cols] = 0.0
+        h["h2osoi_ice"][ns + j - 1, cols] = 0.0
+    return cols
+
+
+@pytest.mark.gpu
+def test_pivoting_columns(gpu):
+    h = heat_case(2048, 34, variant=True)
+    pivot_columns(h)
+    ref = clone(h)
+    oracle.soil_temperature(ref, bind.heat_config())
+    for G in (1, 4):
+        api.set_tuning(4, 256, G, 256)
+        compare(run_gpu(clone(h)), ref)
+
+
+@pytest.mark.gpu
+def test_small_levels_and_mask(gpu):
+    """nlevsno = 5, nlevgrnd = 10 (the reference tests' layout) with a mask."""
+    h = inputs.heat_config1(nc=1024, seed=35, nlevsno=5, nlevsoi=8, nlevgrnd=10, h2osfc_variant=True)
+    h.update(inputs.heat_outputs(h))
+    h["mask"] = (np.arange(h["nc"]) % 4 != 1).astype(np.uint8)
+    ref = clone(h)
+    oracle.soil_temperature(ref, bind.heat_config())
+    api.set_tuning(4, 256, 4, 256)
+    got = run_gpu(h)
+    compare(got, ref)
+    assert np.all(np.isnan(got["t_grnd"][1::4]))

@@ -1,0 +1,131 @@
+"""HIP moisture-form Richards step vs the oracle (same seeded inputs).
+
+Bar: <= 1e-11 relative on h2osoi_liq, smp_l, hk_l, qcharge (|a-b|/max(|a|,|b|));
+nsubsteps identical (a differing accept/reject decision would show here)."""
+import numpy as np
+import pytest
+
+import oracle
+from _util import clone, rel_err
+from clmsoil import api, bind, inputs
+
+TOL = 1e-11
+NS = 12
+
+
+def run_gpu(d, cfg: api.SoilWaterMovementConfig, aquifer=False):
+    import torch
+    t = api.to_device(d)
+    st = api.MfState(t["h2osoi_liq"], t["smp_l"], t["hk_l"], t["qcharge"], t["nsubsteps"],
+                     t.get("nrejected"))
+    mi = api.MfIn(t["z"], t["zi"], t["dz"], t["qflx_rootsoi"], t["watsat"], t["hksat"], t["bsw"],
+                  t["sucsat"], t["icefrac"], t["zwt"], t["qflx_infl"], t.get("smpmin"), t.get("dtsub0"))
+    if aquifer:
+        api.soilwater_moisture_form(st, mi, t.get("mask"), t["nbedrock"], d["nlevsno"], d["nlevsoi"],
+                                    d["nlevgrnd"], d["dtime"], cfg)
+    else:
+        api.soilwm_moisture_form_solve(st, mi, t.get("mask"), t["nbedrock"], d["nlevsno"],
+                                       d["nlevsno"] + 1, d["nlevsoi"], d["dtime"], cfg,
+                                       nlevgrnd=d["nlevgrnd"])
+    torch.cuda.synchronize()
+    assert api.device_errors() == 0
+    return api.to_host(t)
+
+
+def compare(got, ref, d):
+    ns, n = d["nlevsno"], d["nlevsoi"]
+    assert rel_err(got["h2osoi_liq"], ref["h2osoi_liq"]) <= TOL
+    assert rel_err(got["smp_l"], ref["smp_l"]) <= TOL
+    assert rel_err(got["hk_l"], ref["hk_l"]) <= TOL
+    assert rel_err(got["qcharge"], ref["qcharge"]) <= TOL
+    assert np.array_equal(got["nsubsteps"], ref["nsubsteps"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("G", [1, 2, 4])
+@pytest.mark.parametrize("bc", [api.BC_FLUX, api.BC_ZERO_FLUX, api.BC_WATERTABLE])
+def test_config0_vs_oracle(gpu, G, bc):
+    nb = "random" if bc == api.BC_WATERTABLE else None
+    d = inputs.richards_config0(nc=4096, seed=10 + bc, nbedrock=nb, lower_bc=bc)
+    d["nrejected"] = np.full(d["nc"], np.nan)
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=bc)
+    ref = clone(d)
+    oracle.soilwm_moisture_form_solve(ref, cfg.to_abi())
+    api.set_tuning(G, 256, 4, 256)
+    got = run_gpu(d, cfg)
+    compare(got, ref, d)
+    assert np.array_equal(got["nrejected"], ref["nrejected"])
+
+
+@pytest.mark.gpu
+def test_aquifer_pass_and_expensive(gpu):
+    d = inputs.richards_config0(nc=2048, seed=21, nbedrock="random", lower_bc=api.BC_WATERTABLE)
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_WATERTABLE)
+    ref = clone(d)
+    oracle.soilwater_moisture_form(ref, cfg.to_abi())
+    api.set_tuning(4, 256, 4, 256)
+    got = run_gpu(d, cfg, aquifer=True)
+    compare(got, ref, d)
+    cfg.flux_calculation = cfg.expensive
+    d2 = inputs.richards_config0(nc=2048, seed=22, lower_bc=api.BC_FLUX)
+    cfg.lower_boundary_condition = api.BC_FLUX
+    ref = clone(d2)
+    oracle.soilwm_moisture_form_solve(ref, cfg.to_abi())
+    compare(run_gpu(d2, cfg), ref, d2)
+
+
+@pytest.mark.gpu
+def test_conservation_kat_on_gpu(gpu):
+    """The reference's 12x10 zero-flux conservation case on the HIP path."""
+    from test_oracle_kat import conservation_case
+    d = conservation_case()
+    liq0 = d["h2osoi_liq"].copy()
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_ZERO_FLUX)
+    for G in (1, 4):
+        api.set_tuning(G, 256, 4, 256)
+        got = run_gpu(clone(d), cfg)
+        dl = (got["h2osoi_liq"][5:15] - liq0[5:15]).sum(0)
+        np.testing.assert_allclose(dl, d["qflx_infl"] * d["dtime"], rtol=1e-10)
+        assert np.abs(dl - d["qflx_infl"] * d["dtime"]).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_mask_and_bad_nbedrock(gpu):
+    d = inputs.richards_config0(nc=1000, seed=23)
+    mask = np.zeros(d["nc"], dtype=np.uint8)
+    mask[::3] = 1
+    d["mask"] = mask
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_ZERO_FLUX)
+    ref = clone(d)
+    oracle.soilwm_moisture_form_solve(ref, cfg.to_abi())
+    api.set_tuning(4, 256, 4, 256)
+    got = run_gpu(d, cfg)
+    compare(got, ref, d)
+    assert np.all(np.isnan(got["qcharge"][1::3]))          # unmasked columns untouched
+    d["nbedrock"][5] = 25                                  # > nlevsoi: the reference errors
+    import torch
+    t = api.to_device(d)
+    st = api.MfState(t["h2osoi_liq"], t["smp_l"], t["hk_l"], t["qcharge"], t["nsubsteps"])
+    mi = api.MfIn(t["z"], t["zi"], t["dz"], t["qflx_rootsoi"], t["watsat"], t["hksat"], t["bsw"],
+                  t["sucsat"], t["icefrac"], t["zwt"], t["qflx_infl"])
+    api.soilwm_moisture_form_solve(st, mi, None, t["nbedrock"], 12, 13, 20, 1800.0, cfg)
+    torch.cuda.synchronize()
+    assert api.device_errors() & 1
+
+
+@pytest.mark.gpu
+def test_learned_first_substep_input(gpu):
+    """dtsub0 replaces the initial dtsub = dtime; clamped as the oracle clamps it."""
+    d = inputs.richards_config0(nc=2048, seed=24)
+    rng = np.random.default_rng(0)
+    d["dtsub0"] = rng.choice([np.nan, 5.0, 60.0, 225.0, 450.0, 1000.0, 5000.0], d["nc"])
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_ZERO_FLUX)
+    ref = clone(d)
+    oracle.soilwm_moisture_form_solve(ref, cfg.to_abi())
+    api.set_tuning(4, 256, 4, 256)
+    compare(run_gpu(d, cfg), ref, d)
