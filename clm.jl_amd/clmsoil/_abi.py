@@ -100,7 +100,7 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
     global _LIB
     if _LIB is not None and path is None:
         return _LIB
-    p = Path(path) if path else LIB_PATH
+    p = Path(path) if path else Path(os.environ.get("CLMSOIL_LIB", LIB_PATH))
     if not p.exists():
         raise LibraryMissing(
             f"{p} is not built; run `make -C clm.jl_amd/csrc` (or __graft_entry__.build()). "

@@ -16,6 +16,14 @@ constexpr double GRAV = 9.80616, CPLIQ = 4188.0, CPICE = 2117.27, HFUS = 3.337e5
 constexpr double M_TO_MM = 1.0e3;          // soil_water_movement.jl:37
 constexpr double THIN_SFCLAYER = 1.0e-6;   // soil_temperature.jl:26
 
+// Minimum waves per SIMD the fused kernels are compiled for (register cap).
+#ifndef CLMSOIL_RICHARDS_WAVES
+#define CLMSOIL_RICHARDS_WAVES 1
+#endif
+#ifndef CLMSOIL_HEAT_WAVES
+#define CLMSOIL_HEAT_WAVES 1
+#endif
+
 // Compile-time layout limits (CLM5 20SL_8.5m, varpar.jl:112-149).
 constexpr int MAX_NLEVSOI = 20;
 constexpr int MAX_NLEVGRND = 25;

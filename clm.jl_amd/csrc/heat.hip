@@ -67,7 +67,7 @@ struct Dense {
 }  // namespace
 
 template <int G, bool DENSE>
-__global__ __launch_bounds__(256) void heat_kernel(HeatParams P) {
+__global__ __launch_bounds__(256, CLMSOIL_HEAT_WAVES) void heat_kernel(HeatParams P) {
     constexpr int NL = MAX_NLEVSNO + MAX_NLEVGRND;   // 37 snow+soil levels
     constexpr int S = (NL + G - 1) / G;
     const clmsoil_heat_args &a = P.a;

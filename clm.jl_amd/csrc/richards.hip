@@ -40,7 +40,7 @@ __device__ __forceinline__ double prv(const double *x, int i, int k) {
 }
 
 template <int G>
-__global__ __launch_bounds__(256) void richards_mf_kernel(RichardsParams P) {
+__global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kernel(RichardsParams P) {
     constexpr int NMAX = MAX_NLEVSOI;
     constexpr int S = (NMAX + G - 1) / G;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

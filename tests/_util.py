@@ -6,7 +6,7 @@ import numpy as np
 
 def rel_err(a, b, floor=0.0):
     """max |a-b| / max(|a|, |b|, floor) over all elements (0 where both are 0,
-    NaN positions must coincide)."""
+    NaN positions must coincide). `floor` is a scalar or an array of a's shape."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     na, nb = np.isnan(a), np.isnan(b)
@@ -16,7 +16,8 @@ def rel_err(a, b, floor=0.0):
     a, b = a[m], b[m]
     if a.size == 0:
         return 0.0
-    den = np.maximum(np.maximum(np.abs(a), np.abs(b)), floor)
+    fl = np.broadcast_to(np.asarray(floor, dtype=np.float64), m.shape)[m] if np.ndim(floor) else floor
+    den = np.maximum(np.maximum(np.abs(a), np.abs(b)), fl)
     d = np.abs(a - b)
     with np.errstate(invalid="ignore", divide="ignore"):
         r = np.where(d == 0, 0.0, d / den)
