@@ -233,6 +233,11 @@ def set_tuning(richards_g: int, richards_block: int, heat_g: int, heat_block: in
     _abi.check(L, L.clmsoil_set_tuning(richards_g, richards_block, heat_g, heat_block), "set_tuning")
 
 
+def set_richards_schedule(persistent: bool) -> None:
+    L = _lib()
+    _abi.check(L, L.clmsoil_set_richards_schedule(1 if persistent else 0), "set_richards_schedule")
+
+
 def get_tuning() -> tuple:
     L = _lib()
     v = [C.c_int() for _ in range(4)]

@@ -115,6 +115,7 @@ RichardsParams richards_params(const clmsoil_mf_args *a, const clmsoil_levels *l
     P.dtime = dtime; P.e_ice = cfg->e_ice; P.dtmin = cfg->dtmin;
     P.xtu = cfg->xTolerUpper; P.xtl = cfg->xTolerLower; P.verysmall = cfg->verySmall;
     P.err = err;
+    P.work_counters = err + 8;
     return P;
 }
 
@@ -138,6 +139,19 @@ int validate_heat(const clmsoil_heat_args *a, const clmsoil_levels *lev) {
 
 __global__ void zero_words_kernel(unsigned *p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+int run_richards(const RichardsParams &P, hipStream_t stream) {
+    const bool persistent = g_tuning.richards_persistent != 0;
+    if (persistent) {
+        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, P.work_counters,
+                           NUM_XCD);
+        if (hipGetLastError() != hipSuccess) return CLMSOIL_ERR_HIP;
+    }
+    return launch_richards(P, g_tuning.richards_g, g_tuning.richards_block, persistent, stream) ==
+                   hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
 }
 
 int run_heat(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime,
@@ -218,6 +232,12 @@ int clmsoil_set_tuning(int richards_g, int richards_block, int heat_g, int heat_
     return CLMSOIL_OK;
 }
 
+int clmsoil_set_richards_schedule(int persistent) {
+    if (persistent != 0 && persistent != 1) return CLMSOIL_ERR_ARG;
+    g_tuning.richards_persistent = persistent;
+    return CLMSOIL_OK;
+}
+
 int clmsoil_get_tuning(int *richards_g, int *richards_block, int *heat_g, int *heat_block) {
     if (!richards_g || !richards_block || !heat_g || !heat_block) return CLMSOIL_ERR_NULL;
     *richards_g = g_tuning.richards_g;
@@ -235,10 +255,7 @@ int clmsoil_soilwm_moisture_form_solve(const clmsoil_mf_args *a, const clmsoil_l
     DeviceState *s = nullptr;
     if ((st = current_device_state(&s))) return st;
     RichardsParams P = richards_params(a, lev, dtime, cfg, false, s->err);
-    if (launch_richards(P, g_tuning.richards_g, g_tuning.richards_block, (hipStream_t)stream) !=
-        hipSuccess)
-        return CLMSOIL_ERR_HIP;
-    return CLMSOIL_OK;
+    return run_richards(P, (hipStream_t)stream);
 }
 
 int clmsoil_soilwater_moisture_form(const clmsoil_mf_args *a, const clmsoil_levels *lev,
@@ -250,10 +267,7 @@ int clmsoil_soilwater_moisture_form(const clmsoil_mf_args *a, const clmsoil_leve
     const bool aquifer = cfg->lower_boundary_condition == CLMSOIL_BC_WATERTABLE ||
                          cfg->lower_boundary_condition == CLMSOIL_BC_AQUIFER;
     RichardsParams P = richards_params(a, lev, dtime, cfg, aquifer, s->err);
-    if (launch_richards(P, g_tuning.richards_g, g_tuning.richards_block, (hipStream_t)stream) !=
-        hipSuccess)
-        return CLMSOIL_ERR_HIP;
-    return CLMSOIL_OK;
+    return run_richards(P, (hipStream_t)stream);
 }
 
 int clmsoil_soil_temperature(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime,

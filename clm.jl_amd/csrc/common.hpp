@@ -24,6 +24,10 @@ constexpr double THIN_SFCLAYER = 1.0e-6;   // soil_temperature.jl:26
 #define CLMSOIL_HEAT_WAVES 1
 #endif
 
+// MI355X (gfx950) has 8 accelerator complex dies (XCDs); workgroups are
+// dispatched to them round-robin by workgroup id.
+constexpr int NUM_XCD = 8;
+
 // Compile-time layout limits (CLM5 20SL_8.5m, varpar.jl:112-149).
 constexpr int MAX_NLEVSOI = 20;
 constexpr int MAX_NLEVGRND = 25;

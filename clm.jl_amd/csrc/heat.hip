@@ -67,24 +67,10 @@ struct Dense {
 }  // namespace
 
 template <int G, bool DENSE>
-__global__ __launch_bounds__(256, CLMSOIL_HEAT_WAVES) void heat_kernel(HeatParams P) {
+__device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c, const int k) {
     constexpr int NL = MAX_NLEVSNO + MAX_NLEVGRND;   // 37 snow+soil levels
     constexpr int S = (NL + G - 1) / G;
     const clmsoil_heat_args &a = P.a;
-    int64_t c;
-    int k;
-    if constexpr (DENSE) {
-        const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (t >= (int64_t)*P.pivot_count) return;
-        c = P.pivot_list[t];
-        k = 0;
-    } else {
-        const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        c = gtid / G;
-        k = (int)(threadIdx.x % G);
-        if (c >= a.nc) return;
-        if (a.mask && !a.mask[c]) return;
-    }
     const int64_t ld = a.ld;
     const int nlevsno = P.nlevsno, nlevgrnd = P.nlevgrnd;
     const int nlev = nlevsno + nlevgrnd;
@@ -793,10 +779,32 @@ __global__ __launch_bounds__(256, CLMSOIL_HEAT_WAVES) void heat_kernel(HeatParam
 }
 
 template <int G>
+__global__ __launch_bounds__(256, CLMSOIL_HEAT_WAVES) void heat_kernel(HeatParams P) {
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = gtid / G;
+    const int k = (int)(threadIdx.x % G);
+    if (c >= P.a.nc) return;
+    if (P.a.mask && !P.a.mask[c]) return;
+    heat_column<G, false>(P, c, k);
+}
+
+// Pivoting pass: a small grid strides over the columns the fast pass recorded,
+// so the per-lane dense-elimination storage stays bounded.
+constexpr int PIVOT_BLOCK = 64;
+constexpr int PIVOT_GRID = 64;
+
+__global__ __launch_bounds__(PIVOT_BLOCK) void heat_pivot_kernel(HeatParams P) {
+    const int64_t cnt = (int64_t)*P.pivot_count;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+         t += (int64_t)gridDim.x * blockDim.x)
+        heat_column<1, true>(P, P.pivot_list[t], 0);
+}
+
+template <int G>
 static hipError_t launch_g(const HeatParams &P, hipStream_t stream, int block) {
     const int64_t threads = P.a.nc * G;
     const int64_t grid = (threads + block - 1) / block;
-    hipLaunchKernelGGL((heat_kernel<G, false>), dim3((unsigned)grid), dim3(block), 0, stream, P);
+    hipLaunchKernelGGL(heat_kernel<G>, dim3((unsigned)grid), dim3(block), 0, stream, P);
     return hipGetLastError();
 }
 
@@ -811,9 +819,7 @@ hipError_t launch_heat(const HeatParams &P, int G, int block, hipStream_t stream
 
 hipError_t launch_heat_pivot(const HeatParams &P, hipStream_t stream) {
     if (P.a.nc == 0) return hipSuccess;
-    const int block = 64;
-    const int64_t grid = (P.a.nc + block - 1) / block;
-    hipLaunchKernelGGL((heat_kernel<1, true>), dim3((unsigned)grid), dim3(block), 0, stream, P);
+    hipLaunchKernelGGL(heat_pivot_kernel, dim3(PIVOT_GRID), dim3(PIVOT_BLOCK), 0, stream, P);
     return hipGetLastError();
 }
 

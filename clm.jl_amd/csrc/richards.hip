@@ -1,13 +1,13 @@
 // richards.hip — fused adaptive-substep moisture-form Richards step (gfx950).
 //
 // Restates the algorithm of _soilwm_moisture_form_kernel!
-// (src/biogeophys/soil_water_movement.jl:1250-1489) as one launch in which a
-// group of G lanes (G = 1, 2 or 4) owns one column. Layer j (1-based) of the
-// column lives in lane (j-1) mod G, register slot (j-1) div G, so a wavefront
-// load of one slot reads 64/G consecutive columns at G levels — whole 128-byte
-// lines of the column-major [c, j] arrays. Everything per layer stays in VGPRs
-// for the whole step: there is no per-column scratch in memory (the reference
-// keeps 22 nc x nlevsoi scratch matrices, :1555-1561).
+// (src/biogeophys/soil_water_movement.jl:1250-1489) with a group of G lanes
+// (G = 1, 2 or 4) owning one column. Layer j (1-based) of the column lives in lane
+// (j-1) mod G, register slot (j-1) div G, so a wavefront load of one slot reads
+// 64/G consecutive columns at G levels — whole 128-byte lines of the
+// column-major [c, j] arrays. Everything per layer stays in VGPRs for the whole
+// step: there is no per-column scratch in memory (the reference keeps 22
+// nc x nlevsoi scratch matrices, :1555-1561).
 //
 // Per substep iteration:
 //   * hydraulic properties (Clapp-Hornberger hk/smp and derivatives, :1281-1310)
@@ -18,8 +18,16 @@
 //     as a serial sweep over the group: the owner lane of row j computes it and a
 //     DPP quad broadcast hands the carry (bet, dwat, cmx) to the next owner;
 //   * error estimate, accept/reject/double (:1441-1483) are group-uniform.
-// Arithmetic follows the reference's operation order (no reassociation) and
-// Julia's max/min semantics; divisions stay divisions.
+// Arithmetic follows the reference's operation order (no reassociation, no
+// contraction) and Julia's max/min semantics; divisions stay divisions.
+//
+// Scheduling. Columns need from 1 to ~40 substep iterations. A plain launch
+// (one column per group for its lifetime) runs every wavefront for as many
+// iterations as its slowest column. The persistent launch instead keeps a fixed
+// grid resident and lets every group fetch its next column from a work counter
+// as soon as its column is done (wavefront-aggregated atomics, one counter per
+// XCD over a contiguous column range so neighbouring columns share an L2; idle
+// groups move on to other XCDs' ranges at the end).
 #include "common.hpp"
 #include "richards.hpp"
 
@@ -39,91 +47,95 @@ __device__ __forceinline__ double prv(const double *x, int i, int k) {
     return (k == 0) ? b : a;
 }
 
+// One column's state while a group works on it (all in registers).
 template <int G>
-__global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kernel(RichardsParams P) {
-    constexpr int NMAX = MAX_NLEVSOI;
-    constexpr int S = (NMAX + G - 1) / G;
-    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t c = gtid / G;
-    const int k = (int)(threadIdx.x % G);
-    if (c >= P.nc) return;
-    if (P.mask && !P.mask[c]) return;
-    const int n = (int)P.nbedrock[c];
-    if (n < 1 || n > P.nlevsoi) {
-        if (k == 0) atomicOr(P.err, (unsigned)DEV_ERR_NBEDROCK);
-        return;
-    }
-    const int64_t ld = P.ld;
-    const int joff = P.nlevsno, joff_zi = P.nlevsno + 1;
-    const int lower_bc = P.lower_bc;
-    const double dtime = P.dtime;
-
-    // ---- water-table index (BC_WATERTABLE, :1346-1350) ----
-    int jwt = P.nlevsoi;
-    double zwt = 0.0;
-    if (lower_bc == CLMSOIL_BC_WATERTABLE || P.aquifer_pass) {
-        zwt = P.zwt[c];
-        for (int jj = 1; jj <= P.nlevsoi; ++jj)
-            if (zwt <= P.zi[c + (int64_t)(joff_zi + jj - 1) * ld]) { jwt = jj - 1; break; }
-    }
-
-    // ---- per-layer constants and state (loaded once) ----
+struct RColumn {
+    static constexpr int NMAX = MAX_NLEVSOI;
+    static constexpr int S = (NMAX + G - 1) / G;
+    int64_t c;
+    int n, jwt, nsubstep, nrej, iter;
+    bool props_valid;
+    double zwt, qflx_infl, dtsub, dtdone, qc;
     double liq[S], dz[S], watsat[S], khat[S], bsw[S], sucsat[S], root[S], den[S];
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-        const int j = i * G + k + 1;
-        if (j <= n) {
-            const int64_t o = c + (int64_t)(j - 1) * ld;           // soil-array offset
-            const int64_t os = c + (int64_t)(joff + j - 1) * ld;   // snow+soil offset
-            liq[i] = P.h2osoi_liq[os];
-            dz[i] = __builtin_nontemporal_load(&P.dz[os]);
-            watsat[i] = __builtin_nontemporal_load(&P.watsat[o]);
-            bsw[i] = __builtin_nontemporal_load(&P.bsw[o]);
-            sucsat[i] = __builtin_nontemporal_load(&P.sucsat[o]);
-            root[i] = __builtin_nontemporal_load(&P.qflx_rootsoi[o]);
-            const double hks = __builtin_nontemporal_load(&P.hksat[o]);
-            const double icf = __builtin_nontemporal_load(&P.icefrac[o]);
-            const double zj = __builtin_nontemporal_load(&P.z[os]);
-            double imp;
-            if (j == n) {
-                imp = pow(10.0, -P.e_ice * icf);
-                den[i] = M_TO_MM * (zwt - zj);                     // :1357 (WATERTABLE)
-            } else {
-                const double icf1 = __builtin_nontemporal_load(&P.icefrac[o + ld]);
-                imp = pow(10.0, -P.e_ice * (0.5 * (icf + icf1)));
-                den[i] = M_TO_MM * (__builtin_nontemporal_load(&P.z[os + ld]) - zj);   // :1338
-            }
-            khat[i] = imp * hks;   // (imp * hksat) * s1^(2b+3), the reference's grouping
-        } else {
-            liq[i] = 0.0; dz[i] = 1.0; watsat[i] = 1.0; khat[i] = 0.0; bsw[i] = 1.0;
-            sucsat[i] = 0.0; root[i] = 0.0; den[i] = 1.0;
-        }
-    }
-    const double qflx_infl = P.qflx_infl[c];
-
-    // ---- retained hydraulic state (recomputed after accepted substeps) ----
     double hk[S], smp[S], dhkdw[S], qin[S], qout[S], dqidw0[S], dqidw1[S], dqodw1[S], dqodw2[S];
     double gam[S], dwat[S];
+
+    // Load column `col`; returns false (and flags the error word) for a bad nbedrock.
+    __device__ __forceinline__ bool load(const RichardsParams &P, int64_t col, int k) {
+        c = col;
+        n = (int)P.nbedrock[c];
+        if (n < 1 || n > P.nlevsoi) {
+            if (k == 0) atomicOr(P.err, (unsigned)DEV_ERR_NBEDROCK);
+            return false;
+        }
+        const int64_t ld = P.ld;
+        const int joff = P.nlevsno, joff_zi = P.nlevsno + 1;
+        // water-table index (BC_WATERTABLE, :1346-1350)
+        jwt = P.nlevsoi;
+        zwt = 0.0;
+        if (P.lower_bc == CLMSOIL_BC_WATERTABLE || P.aquifer_pass) {
+            zwt = P.zwt[c];
+            for (int jj = 1; jj <= P.nlevsoi; ++jj)
+                if (zwt <= P.zi[c + (int64_t)(joff_zi + jj - 1) * ld]) { jwt = jj - 1; break; }
+        }
 #pragma unroll
-    for (int i = 0; i < S; ++i) {
-        hk[i] = smp[i] = dhkdw[i] = qin[i] = qout[i] = 0.0;
-        dqidw0[i] = dqidw1[i] = dqodw1[i] = dqodw2[i] = gam[i] = dwat[i] = 0.0;
+        for (int i = 0; i < S; ++i) {
+            const int j = i * G + k + 1;
+            if (j <= n) {
+                const int64_t o = c + (int64_t)(j - 1) * ld;           // soil-array offset
+                const int64_t os = c + (int64_t)(joff + j - 1) * ld;   // snow+soil offset
+                liq[i] = P.h2osoi_liq[os];
+                dz[i] = __builtin_nontemporal_load(&P.dz[os]);
+                watsat[i] = __builtin_nontemporal_load(&P.watsat[o]);
+                bsw[i] = __builtin_nontemporal_load(&P.bsw[o]);
+                sucsat[i] = __builtin_nontemporal_load(&P.sucsat[o]);
+                root[i] = __builtin_nontemporal_load(&P.qflx_rootsoi[o]);
+                const double hks = __builtin_nontemporal_load(&P.hksat[o]);
+                const double icf = __builtin_nontemporal_load(&P.icefrac[o]);
+                const double zj = __builtin_nontemporal_load(&P.z[os]);
+                double imp;
+                if (j == n) {
+                    imp = pow(10.0, -P.e_ice * icf);
+                    den[i] = M_TO_MM * (zwt - zj);                     // :1357 (WATERTABLE)
+                } else {
+                    const double icf1 = __builtin_nontemporal_load(&P.icefrac[o + ld]);
+                    imp = pow(10.0, -P.e_ice * (0.5 * (icf + icf1)));
+                    den[i] = M_TO_MM * (__builtin_nontemporal_load(&P.z[os + ld]) - zj);   // :1338
+                }
+                khat[i] = imp * hks;   // (imp * hksat) * s1^(2b+3), the reference's grouping
+            } else {
+                liq[i] = 0.0; dz[i] = 1.0; watsat[i] = 1.0; khat[i] = 0.0; bsw[i] = 1.0;
+                sucsat[i] = 0.0; root[i] = 0.0; den[i] = 1.0;
+            }
+            hk[i] = smp[i] = dhkdw[i] = qin[i] = qout[i] = 0.0;
+            dqidw0[i] = dqidw1[i] = dqodw1[i] = dqodw2[i] = gam[i] = dwat[i] = 0.0;
+        }
+        qflx_infl = P.qflx_infl[c];
+        nsubstep = 0;
+        nrej = 0;
+        iter = 0;
+        const double dtime = P.dtime;
+        dtsub = dtime;
+        if (P.dtsub0) {
+            const double d0 = P.dtsub0[c], lo = jl_min(P.dtmin, dtime);
+            dtsub = (d0 != d0) ? dtime : jl_min(jl_max(d0, lo), dtime);
+        }
+        dtdone = 0.0;
+        qc = 0.0;
+        props_valid = false;
+        return true;
     }
 
-    int nsubstep = 0, nrej = 0;
-    double dtsub = dtime;
-    if (P.dtsub0) {
-        const double d0 = P.dtsub0[c], lo = jl_min(P.dtmin, dtime);
-        dtsub = (d0 != d0) ? dtime : jl_min(jl_max(d0, lo), dtime);
-    }
-    double dtdone = 0.0, qc = 0.0;
-    bool props_valid = false;
-
-    for (int iter = 0;; ++iter) {
+    // One pass of the reference's substep loop; returns true when the column's
+    // step is complete.
+    __device__ __forceinline__ bool iterate(const RichardsParams &P, int k) {
+        const int lower_bc = P.lower_bc;
+        const double dtime = P.dtime;
         if (iter >= P.max_iter) {
             if (k == 0) atomicOr(P.err, (unsigned)DEV_ERR_ITER);
-            break;
+            return true;
         }
+        ++iter;
         nsubstep += 1;
         if (!props_valid) {
             // node saturation s2 (:1273-1286)
@@ -279,7 +291,7 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kerne
         if (errorMax > P.xtu && dtsub > P.dtmin) {                 // :1460-1463
             dtsub = jl_max(dtsub / 2.0, P.dtmin);
             nrej += 1;
-            continue;
+            return false;
         }
         // accept (:1466-1478)
 #pragma unroll
@@ -287,70 +299,150 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kerne
         {
             const int in = (n - 1) / G, kn = (n - 1) % G;
             double qct = 0.0;
-            if (k == kn) {
-                if (lower_bc == CLMSOIL_BC_FLUX) qct = hk[in] + dhkdw[in] * dwat[in];
+#pragma unroll
+            for (int i = 0; i < S; ++i) {
+                if (i != in || k != kn) continue;
+                if (lower_bc == CLMSOIL_BC_FLUX) qct = hk[i] + dhkdw[i] * dwat[i];
                 else if (lower_bc == CLMSOIL_BC_ZERO_FLUX) qct = 0.0;
-                else qct = qout[in] + dqodw1[in] * dwat[in];
+                else qct = qout[i] + dqodw1[i] * dwat[i];
             }
             qct = group_bcast<G>(qct, kn);
             qc += qct * (dtsub / dtime);
         }
         props_valid = false;
         dtdone += dtsub;                                            // :1480-1483
-        if (fabs(dtime - dtdone) < P.verysmall) break;
+        if (fabs(dtime - dtdone) < P.verysmall) return true;
         if (errorMax < P.xtl) dtsub = dtsub * 2.0;
         dtsub = jl_min(dtsub, dtime - dtdone);
+        return false;
     }
 
-    // ---- write back ----
+    __device__ __forceinline__ void store(const RichardsParams &P, int k) {
+        const int64_t ld = P.ld;
+        const int joff = P.nlevsno;
+        const double dtime = P.dtime;
 #pragma unroll
-    for (int i = 0; i < S; ++i) {
-        const int j = i * G + k + 1;
-        if (j <= n) {
-            P.h2osoi_liq[c + (int64_t)(joff + j - 1) * ld] = liq[i];
-            P.smp_l[c + (int64_t)(j - 1) * ld] = smp[i];
-            P.hk_l[c + (int64_t)(j - 1) * ld] = hk[i];
+        for (int i = 0; i < S; ++i) {
+            const int j = i * G + k + 1;
+            if (j <= n) {
+                P.h2osoi_liq[c + (int64_t)(joff + j - 1) * ld] = liq[i];
+                P.smp_l[c + (int64_t)(j - 1) * ld] = smp[i];
+                P.hk_l[c + (int64_t)(j - 1) * ld] = hk[i];
+            }
         }
-    }
-    // liquid and thickness of layer jwt+1, read from its owner lane's registers
-    double liq_w = 0.0, dz_w = 1.0;
-    if (P.aquifer_pass && jwt < P.nlevsoi && jwt + 1 <= n) {
-        const int jw1 = jwt + 1;
-#pragma unroll
-        for (int i = 0; i < S; ++i)
-            if (i * G + k + 1 == jw1) { liq_w = liq[i]; dz_w = dz[i]; }
-        liq_w = group_bcast<G>(liq_w, (jw1 - 1) % G);
-        dz_w = group_bcast<G>(dz_w, (jw1 - 1) % G);
-    }
-    if (k == 0) {
-        double qcharge = qc;
-        if (P.aquifer_pass && jwt < P.nlevsoi) {
-            // compute_qcharge! with zero smp/imped (:1153-1217 via :1575-1598)
+        // liquid and thickness of layer jwt+1, read from its owner lane's registers
+        double liq_w = 0.0, dz_w = 1.0;
+        if (P.aquifer_pass && jwt < P.nlevsoi && jwt + 1 <= n) {
             const int jw1 = jwt + 1;
-            const int64_t o = c + (int64_t)(jw1 - 1) * ld;
-            double vwc = 0.0;
-            if (jw1 <= n) vwc = jl_max(liq_w, 1.0e-6) / (dz_w * DENH2O);
-            const double wh_zwt = -P.sucsat[o] - zwt * M_TO_MM;
-            double s1 = jl_max(vwc / P.watsat[o], 0.01);
-            s1 = jl_min(1.0, s1);
-            const double ka = 0.0 * P.hksat[o] * pow(s1, 2.0 * P.bsw[o] + 3.0);
-            const double smpmin = P.smpmin ? P.smpmin[c] : -1.0e8;
-            const double smp1 = jl_max(smpmin, 0.0);
-            const int jm = jwt > 1 ? jwt : 1;
-            const double wh = smp1 - P.z[c + (int64_t)(joff + jm - 1) * ld] * M_TO_MM;
-            double q;
-            if (jwt == 0)
-                q = -ka * (wh_zwt - wh) / ((zwt + 1.0e-3) * M_TO_MM);
-            else
-                q = -ka * (wh_zwt - wh) /
-                    ((zwt - P.z[c + (int64_t)(joff + jwt - 1) * ld]) * M_TO_MM * 2.0);
-            q = jl_max(-10.0 / dtime, q);
-            q = jl_min(10.0 / dtime, q);
-            qcharge = q;
+#pragma unroll
+            for (int i = 0; i < S; ++i)
+                if (i * G + k + 1 == jw1) { liq_w = liq[i]; dz_w = dz[i]; }
+            liq_w = group_bcast<G>(liq_w, (jw1 - 1) % G);
+            dz_w = group_bcast<G>(dz_w, (jw1 - 1) % G);
         }
-        P.qcharge[c] = qcharge;
-        P.nsubsteps[c] = (double)nsubstep;
-        if (P.nrejected) P.nrejected[c] = (double)nrej;
+        if (k == 0) {
+            double qcharge = qc;
+            if (P.aquifer_pass && jwt < P.nlevsoi) {
+                // compute_qcharge! with zero smp/imped (:1153-1217 via :1575-1598)
+                const int jw1 = jwt + 1;
+                const int64_t o = c + (int64_t)(jw1 - 1) * ld;
+                double vwc = 0.0;
+                if (jw1 <= n) vwc = jl_max(liq_w, 1.0e-6) / (dz_w * DENH2O);
+                const double wh_zwt = -P.sucsat[o] - zwt * M_TO_MM;
+                double s1 = jl_max(vwc / P.watsat[o], 0.01);
+                s1 = jl_min(1.0, s1);
+                const double ka = 0.0 * P.hksat[o] * pow(s1, 2.0 * P.bsw[o] + 3.0);
+                const double smpmin = P.smpmin ? P.smpmin[c] : -1.0e8;
+                const double smp1 = jl_max(smpmin, 0.0);
+                const int jm = jwt > 1 ? jwt : 1;
+                const double wh = smp1 - P.z[c + (int64_t)(joff + jm - 1) * ld] * M_TO_MM;
+                double q;
+                if (jwt == 0)
+                    q = -ka * (wh_zwt - wh) / ((zwt + 1.0e-3) * M_TO_MM);
+                else
+                    q = -ka * (wh_zwt - wh) /
+                        ((zwt - P.z[c + (int64_t)(joff + jwt - 1) * ld]) * M_TO_MM * 2.0);
+                q = jl_max(-10.0 / dtime, q);
+                q = jl_min(10.0 / dtime, q);
+                qcharge = q;
+            }
+            P.qcharge[c] = qcharge;
+            P.nsubsteps[c] = (double)nsubstep;
+            if (P.nrejected) P.nrejected[c] = (double)nrej;
+        }
+    }
+};
+
+// One column per group for the group's lifetime.
+template <int G>
+__global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kernel(RichardsParams P) {
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = gtid / G;
+    const int k = (int)(threadIdx.x % G);
+    if (c >= P.nc) return;
+    if (P.mask && !P.mask[c]) return;
+    RColumn<G> col;
+    if (!col.load(P, c, k)) return;
+    while (!col.iterate(P, k)) {
+    }
+    col.store(P, k);
+}
+
+// Persistent: a resident grid whose groups refill from per-XCD column ranges.
+// Every wavefront leaves the loop once all eight ranges are exhausted.
+template <int G>
+__global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persistent(RichardsParams P) {
+    const int k = (int)(threadIdx.x % G);
+    const int lane = (int)__lane_id();
+    const int xcd = (int)(blockIdx.x % NUM_XCD);
+    const int64_t chunk = (P.nc + NUM_XCD - 1) / NUM_XCD;
+    unsigned *ctrs = P.work_counters;
+    RColumn<G> col;
+    bool active = false;   // group holds a column
+    bool done = false;     // no work left for this group
+    int steal = 0;         // XCD ranges exhausted so far (wavefront-uniform)
+    for (;;) {
+        // ---- refill groups that have no column ----
+        const bool want = !active && !done;
+        const unsigned long long need = __ballot(want && k == 0);
+        if (need) {
+            const int leader = __ffsll((long long)need) - 1;
+            const int cnt = __popcll(need);
+            const int x = (xcd + steal) % NUM_XCD;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(&ctrs[x], (unsigned)cnt);
+            base = (unsigned)__builtin_amdgcn_readlane((int)base, leader);
+            const int64_t lo = (int64_t)x * chunk;
+            const int64_t hi = lo + chunk < P.nc ? lo + chunk : P.nc;
+            int64_t cidx = -1;
+            if (want) {
+                // rank of this group among the refilling groups of the wavefront
+                const int my_k0 = lane - k;
+                const unsigned long long below = need & ((1ull << my_k0) - 1ull);
+                const int64_t idx = lo + (int64_t)base + __popcll(below);
+                if (idx < hi) cidx = idx;
+            }
+            if (cidx >= 0) {
+                const bool on = !P.mask || P.mask[cidx];
+                active = on ? col.load(P, cidx, k) : false;
+            }
+            // a wavefront that got fewer columns than it asked for moves on to the
+            // next XCD range; after the last range its idle groups are done
+            if (__ballot(want && k == 0 && cidx < 0)) {
+                ++steal;
+                if (steal >= NUM_XCD && want && cidx < 0) done = true;
+            }
+        }
+        if (!__ballot(active)) {
+            if (!__ballot(!done)) break;
+            continue;
+        }
+        if (active) {
+            if (col.iterate(P, k)) {
+                col.store(P, k);
+                active = false;
+            }
+        }
     }
 }
 
@@ -362,8 +454,34 @@ static hipError_t launch_g(const RichardsParams &P, hipStream_t stream, int bloc
     return hipGetLastError();
 }
 
-hipError_t launch_richards(const RichardsParams &P, int G, int block, hipStream_t stream) {
+template <int G>
+static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int block) {
+    int dev = 0, cus = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return hipErrorUnknown;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return hipErrorUnknown;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G>, block, 0) !=
+        hipSuccess)
+        return hipErrorUnknown;
+    if (nb < 1) nb = 1;
+    int64_t grid = (int64_t)nb * cus;
+    grid = (grid + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    const int64_t need = (P.nc * G + block - 1) / block;
+    if (grid > need) grid = (need + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    hipLaunchKernelGGL(richards_mf_persistent<G>, dim3((unsigned)grid), dim3(block), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_richards(const RichardsParams &P, int G, int block, bool persistent,
+                           hipStream_t stream) {
     if (P.nc == 0) return hipSuccess;
+    if (persistent) {
+        switch (G) {
+            case 1: return launch_p<1>(P, stream, block);
+            case 2: return launch_p<2>(P, stream, block);
+            default: return launch_p<4>(P, stream, block);
+        }
+    }
     switch (G) {
         case 1: return launch_g<1>(P, stream, block);
         case 2: return launch_g<2>(P, stream, block);

@@ -19,9 +19,12 @@ struct RichardsParams {
     int max_iter;           // substep-iteration cap (hang guard)
     double dtime, e_ice, dtmin, xtu, xtl, verysmall;
     unsigned *err;          // device error word
+    unsigned *work_counters;  // NUM_XCD zeroed counters (persistent launch)
 };
 
-// G = lanes per column (1, 2 or 4); block = threads per workgroup.
-hipError_t launch_richards(const RichardsParams &P, int G, int block, hipStream_t stream);
+// G = lanes per column (1, 2 or 4); block = threads per workgroup; persistent
+// selects the resident-grid launch whose groups refill from work counters.
+hipError_t launch_richards(const RichardsParams &P, int G, int block, bool persistent,
+                           hipStream_t stream);
 
 }  // namespace clmsoil

@@ -11,11 +11,13 @@ struct Tuning {
     int richards_block = 256;
     int heat_g = 4;
     int heat_block = 256;
+    int richards_persistent = 1;   // resident grid with per-group column refill
 };
 extern Tuning g_tuning;
 
-// Allocated once per device on first use: a 256-byte status block (device error
-// word, pivot counter) and the heat step's pivot list.
+// Allocated once per device on first use: a 256-byte status block (word 0 device
+// error bits, word 1 pivot counter, words 8..15 per-XCD work counters) and the
+// heat step's pivot list.
 struct DeviceState {
     unsigned *err = nullptr;
     unsigned *pivot_count = nullptr;

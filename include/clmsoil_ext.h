@@ -28,6 +28,10 @@ int clmsoil_device_errors(void *stream, uint32_t *flags);
  * Richards and heat kernels; process-wide, read at launch. */
 int clmsoil_set_tuning(int richards_g, int richards_block, int heat_g, int heat_block);
 int clmsoil_get_tuning(int *richards_g, int *richards_block, int *heat_g, int *heat_block);
+/* 1 (default): the Richards step runs as a resident grid whose lane groups each
+ * take a new column from a work counter as soon as their column's substeps are
+ * done; 0: one column per lane group for the launch. Results are identical. */
+int clmsoil_set_richards_schedule(int persistent);
 
 /* Heat step with a caller-provided device workspace of at least
  * clmsoil_heat_workspace_size(nc) bytes (pivot list); allocates nothing.

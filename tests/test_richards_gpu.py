@@ -42,9 +42,10 @@ def compare(got, ref, d):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("persistent", [True, False])
 @pytest.mark.parametrize("G", [1, 2, 4])
 @pytest.mark.parametrize("bc", [api.BC_FLUX, api.BC_ZERO_FLUX, api.BC_WATERTABLE])
-def test_config0_vs_oracle(gpu, G, bc):
+def test_config0_vs_oracle(gpu, G, bc, persistent):
     nb = "random" if bc == api.BC_WATERTABLE else None
     d = inputs.richards_config0(nc=4096, seed=10 + bc, nbedrock=nb, lower_bc=bc)
     d["nrejected"] = np.full(d["nc"], np.nan)
@@ -53,7 +54,9 @@ def test_config0_vs_oracle(gpu, G, bc):
     ref = clone(d)
     oracle.soilwm_moisture_form_solve(ref, cfg.to_abi())
     api.set_tuning(G, 256, 4, 256)
+    api.set_richards_schedule(persistent)
     got = run_gpu(d, cfg)
+    api.set_richards_schedule(True)
     compare(got, ref, d)
     assert np.array_equal(got["nrejected"], ref["nrejected"], equal_nan=True)
 
