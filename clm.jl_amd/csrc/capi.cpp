@@ -62,6 +62,11 @@ int ensure_pivot_list(DeviceState *s, int64_t nc) {
     if (hipMalloc(&p, (size_t)cap * sizeof(int64_t)) != hipSuccess) return CLMSOIL_ERR_HIP;
     s->pivot_list = static_cast<int64_t *>(p);
     s->pivot_cap = cap;
+    if (!s->dense_ws) {
+        void *q = nullptr;
+        if (hipMalloc(&q, DENSE_WS_BYTES) != hipSuccess) return CLMSOIL_ERR_HIP;
+        s->dense_ws = static_cast<double *>(q);
+    }
     return CLMSOIL_OK;
 }
 
@@ -156,7 +161,7 @@ int run_richards(const RichardsParams &P, hipStream_t stream) {
 
 int run_heat(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime,
              const clmsoil_heat_config *cfg, hipStream_t stream, unsigned *err,
-             unsigned *pivot_count, int64_t *pivot_list) {
+             unsigned *pivot_count, int64_t *pivot_list, double *dense_ws) {
     HeatParams P{};
     P.a = *a;
     P.nlevsno = lev->nlevsno;
@@ -166,6 +171,7 @@ int run_heat(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime
     P.err = err;
     P.pivot_count = pivot_count;
     P.pivot_list = pivot_list;
+    P.dense_ws = dense_ws;
     hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, pivot_count, 1);
     if (hipGetLastError() != hipSuccess) return CLMSOIL_ERR_HIP;
     if (launch_heat(P, g_tuning.heat_g, g_tuning.heat_block, stream) != hipSuccess)
@@ -281,11 +287,12 @@ int clmsoil_soil_temperature(const clmsoil_heat_args *a, const clmsoil_levels *l
     if ((st = current_device_state(&s))) return st;
     if ((st = ensure_pivot_list(s, a->nc))) return st;
     return run_heat(a, lev, dtime, cfg, (hipStream_t)stream, s->err, s->pivot_count,
-                    s->pivot_list);
+                    s->pivot_list, s->dense_ws);
 }
 
 size_t clmsoil_heat_workspace_size(int64_t nc) {
-    return 256 + (size_t)(nc > 0 ? nc : 0) * sizeof(int64_t);
+    const size_t list = ((size_t)(nc > 0 ? nc : 0) * sizeof(int64_t) + 255) / 256 * 256;
+    return 256 + list + DENSE_WS_BYTES;
 }
 
 int clmsoil_soil_temperature_ws(const clmsoil_heat_args *a, const clmsoil_levels *lev,
@@ -298,9 +305,11 @@ int clmsoil_soil_temperature_ws(const clmsoil_heat_args *a, const clmsoil_levels
         return CLMSOIL_ERR_WORKSPACE;
     unsigned *words = static_cast<unsigned *>(workspace);
     int64_t *list = reinterpret_cast<int64_t *>(static_cast<char *>(workspace) + 256);
+    const size_t list_bytes = ((size_t)a->nc * sizeof(int64_t) + 255) / 256 * 256;
+    double *dense = reinterpret_cast<double *>(static_cast<char *>(workspace) + 256 + list_bytes);
     DeviceState *s = nullptr;
     if ((st = current_device_state(&s))) return st;
-    return run_heat(a, lev, dtime, cfg, (hipStream_t)stream, s->err, words + 1, list);
+    return run_heat(a, lev, dtime, cfg, (hipStream_t)stream, s->err, words + 1, list, dense);
 }
 
 int clmsoil_tridiagonal_multi(double *u, const double *a, const double *b, const double *c,

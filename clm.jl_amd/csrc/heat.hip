@@ -57,12 +57,8 @@ __device__ __forceinline__ double level_value(const double *x, int jj, int k) {
     return group_bcast<G>(v, (jj - 1) % G);
 }
 
-struct Dense {
-    // reference's per-column dense elimination (band_diagonal.jl:87-170), used by
-    // the pivoting pass; n <= 38
-    double A[38 * 38];
-    double x[38];
-};
+constexpr int DENSE_N = MAX_NLEVSNO + MAX_NLEVGRND + 1;   // 38 rows at most
+
 
 }  // namespace
 
@@ -376,21 +372,26 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             if (jj == js1) x_h = (rv_h - b2_h * xn) / b3_h;
         }
     } else {
-        // dense elimination with partial pivoting, exactly as the reference
-        Dense D;
-        const int64_t jt = jtop;                             // first level in the system
-        const int n = nlev - jtop + 2;                       // levels + h2osfc row
-        // system row of level jj: jj - jtop + 1 (snow), jj - jtop + 2 (soil); h2osfc
-        // row: nlevsno - jtop + 2
-        const int rh = nlevsno - (int)jt + 2;
-        for (int q = 0; q < n * n; ++q) D.A[q] = 0.0;
-#define AD(r, cc) D.A[((r) - 1) + ((cc) - 1) * n]
+        // Dense elimination with partial pivoting, as the reference does it, on a
+        // per-group n x n matrix in the library's global workspace: every lane
+        // writes its rows, lane 0 of the group eliminates, every lane reads back.
+        const int64_t grp = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+        double *Am = P.dense_ws + grp * (int64_t)(DENSE_N * DENSE_N + DENSE_N);
+        double *xd = Am + DENSE_N * DENSE_N;
+        const int jt = jtop;
+        const int n = nlev - jtop + 2;                 // levels + h2osfc row
+        const int rh = nlevsno - jt + 2;               // system row of the h2osfc row
+#define AD(r, cc) Am[((r) - 1) + ((cc) - 1) * n]
+        for (int q = k; q < n * n; q += G) Am[q] = 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
         for (int i = 0; i < S; ++i) {
-            const int jj = i + 1;
+            const int jj = i * G + k + 1;
             if (jj < jtop || jj > nlev) continue;
-            const int r = jj <= nlevsno ? jj - (int)jt + 1 : jj - (int)jt + 2;
+            const int r = jj <= nlevsno ? jj - jt + 1 : jj - jt + 2;
             AD(r, r) = b3[i];
-            D.x[r - 1] = rv[i];
+            xd[r - 1] = rv[i];
             if (jj <= nlevsno) {
                 if (jj > jtop) AD(r, r - 1) = sub[i];
                 if (jj < nlevsno) AD(r, r + 1) = sup[i];
@@ -404,46 +405,55 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
                 if (jj < nlev) AD(r, r + 1) = sup[i];
             }
         }
-        AD(rh, rh) = b3_h;
-        AD(rh, rh + 1) = b2_h;
-        D.x[rh - 1] = rv_h;
-        for (int kq = 1; kq <= n - 1; ++kq) {
-            int pv = kq;
-            double maxv = fabs(AD(kq, kq));
-            for (int r = kq + 1; r <= n; ++r) {
-                const double v = fabs(AD(r, kq));
-                if (v > maxv) { maxv = v; pv = r; }
-            }
-            const double akk = AD(pv, kq);
-            if (pv != kq) {
-                for (int cc = kq; cc <= n; ++cc) {
-                    const double akj = AD(kq, cc), apj = AD(pv, cc);
-                    AD(kq, cc) = apj;
-                    AD(pv, cc) = akj;
+        if (k == 0) {
+            AD(rh, rh) = b3_h;
+            AD(rh, rh + 1) = b2_h;
+            xd[rh - 1] = rv_h;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        if (k == 0) {
+            for (int kq = 1; kq <= n - 1; ++kq) {
+                int pv = kq;
+                double maxv = fabs(AD(kq, kq));
+                for (int r = kq + 1; r <= n; ++r) {
+                    const double v = fabs(AD(r, kq));
+                    if (v > maxv) { maxv = v; pv = r; }
                 }
-                const double xk = D.x[kq - 1], xp = D.x[pv - 1];
-                D.x[kq - 1] = xp;
-                D.x[pv - 1] = xk;
+                const double akk = AD(pv, kq);
+                if (pv != kq) {
+                    for (int cc = kq; cc <= n; ++cc) {
+                        const double akj = AD(kq, cc), apj = AD(pv, cc);
+                        AD(kq, cc) = apj;
+                        AD(pv, cc) = akj;
+                    }
+                    const double xk = xd[kq - 1], xp = xd[pv - 1];
+                    xd[kq - 1] = xp;
+                    xd[pv - 1] = xk;
+                }
+                for (int r = kq + 1; r <= n; ++r) {
+                    const double f = AD(r, kq) / akk;
+                    for (int cc = kq; cc <= n; ++cc) AD(r, cc) = AD(r, cc) - f * AD(kq, cc);
+                    xd[r - 1] = xd[r - 1] - f * xd[kq - 1];
+                }
             }
-            for (int r = kq + 1; r <= n; ++r) {
-                const double f = AD(r, kq) / akk;
-                for (int cc = kq; cc <= n; ++cc) AD(r, cc) = AD(r, cc) - f * AD(kq, cc);
-                D.x[r - 1] = D.x[r - 1] - f * D.x[kq - 1];
+            for (int r = n; r >= 1; --r) {
+                double sacc = xd[r - 1];
+                for (int cc = r + 1; cc <= n; ++cc) sacc = sacc - AD(r, cc) * xd[cc - 1];
+                xd[r - 1] = sacc / AD(r, r);
             }
         }
-        for (int r = n; r >= 1; --r) {
-            double s = D.x[r - 1];
-            for (int cc = r + 1; cc <= n; ++cc) s = s - AD(r, cc) * D.x[cc - 1];
-            D.x[r - 1] = s / AD(r, r);
-        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
         for (int i = 0; i < S; ++i) {
-            const int jj = i + 1;
+            const int jj = i * G + k + 1;
             x[i] = 0.0;
             if (jj < jtop || jj > nlev) continue;
-            const int r = jj <= nlevsno ? jj - (int)jt + 1 : jj - (int)jt + 2;
-            x[i] = D.x[r - 1];
+            const int r = jj <= nlevsno ? jj - jt + 1 : jj - jt + 2;
+            x[i] = xd[r - 1];
         }
-        x_h = D.x[rh - 1];
+        x_h = xd[rh - 1];
 #undef AD
     }
 
@@ -788,16 +798,16 @@ __global__ __launch_bounds__(256, CLMSOIL_HEAT_WAVES) void heat_kernel(HeatParam
     heat_column<G, false>(P, c, k);
 }
 
-// Pivoting pass: a small grid strides over the columns the fast pass recorded,
-// so the per-lane dense-elimination storage stays bounded.
-constexpr int PIVOT_BLOCK = 64;
-constexpr int PIVOT_GRID = 64;
+// Pivoting pass: a small grid strides over the columns the fast pass recorded;
+// each lane group owns one dense system in the workspace (PIVOT_GROUPS of them).
 
 __global__ __launch_bounds__(PIVOT_BLOCK) void heat_pivot_kernel(HeatParams P) {
+    constexpr int G = 4;
     const int64_t cnt = (int64_t)*P.pivot_count;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
-         t += (int64_t)gridDim.x * blockDim.x)
-        heat_column<1, true>(P, P.pivot_list[t], 0);
+    const int k = (int)(threadIdx.x % G);
+    const int64_t ngroups = (int64_t)gridDim.x * blockDim.x / G;
+    for (int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; t < cnt; t += ngroups)
+        heat_column<G, true>(P, P.pivot_list[t], k);
 }
 
 template <int G>

@@ -23,6 +23,7 @@ struct DeviceState {
     unsigned *pivot_count = nullptr;
     int64_t *pivot_list = nullptr;
     int64_t pivot_cap = 0;
+    double *dense_ws = nullptr;    // dense systems of the heat pivoting pass
 };
 
 int current_device_state(DeviceState **out);
