@@ -17,8 +17,3 @@ def declare(lib) -> None:
     lib.clmsoil_get_tuning.restype = I
     lib.clmsoil_set_richards_schedule.argtypes = [I]
     lib.clmsoil_set_richards_schedule.restype = I
-    lib.clmsoil_heat_workspace_size.argtypes = [C.c_int64]
-    lib.clmsoil_heat_workspace_size.restype = C.c_size_t
-    lib.clmsoil_soil_temperature_ws.argtypes = [C.POINTER(HeatArgs), C.POINTER(Levels), C.c_double,
-                                                C.POINTER(HeatConfig), P, C.c_size_t, P]
-    lib.clmsoil_soil_temperature_ws.restype = I

@@ -43,30 +43,8 @@ int current_device_state(DeviceState **out) {
         if (hipMalloc(&p, 256) != hipSuccess) return CLMSOIL_ERR_HIP;
         if (hipMemset(p, 0, 256) != hipSuccess) return CLMSOIL_ERR_HIP;
         s.err = static_cast<unsigned *>(p);
-        s.pivot_count = s.err + 1;
     }
     *out = &s;
-    return CLMSOIL_OK;
-}
-
-int ensure_pivot_list(DeviceState *s, int64_t nc) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (s->pivot_cap >= nc) return CLMSOIL_OK;
-    if (s->pivot_list) {
-        if (hipDeviceSynchronize() != hipSuccess) return CLMSOIL_ERR_HIP;
-        (void)hipFree(s->pivot_list);
-        s->pivot_list = nullptr;
-    }
-    void *p = nullptr;
-    const int64_t cap = nc < 1024 ? 1024 : nc;
-    if (hipMalloc(&p, (size_t)cap * sizeof(int64_t)) != hipSuccess) return CLMSOIL_ERR_HIP;
-    s->pivot_list = static_cast<int64_t *>(p);
-    s->pivot_cap = cap;
-    if (!s->dense_ws) {
-        void *q = nullptr;
-        if (hipMalloc(&q, DENSE_WS_BYTES) != hipSuccess) return CLMSOIL_ERR_HIP;
-        s->dense_ws = static_cast<double *>(q);
-    }
     return CLMSOIL_OK;
 }
 
@@ -160,8 +138,7 @@ int run_richards(const RichardsParams &P, hipStream_t stream) {
 }
 
 int run_heat(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime,
-             const clmsoil_heat_config *cfg, hipStream_t stream, unsigned *err,
-             unsigned *pivot_count, int64_t *pivot_list, double *dense_ws) {
+             const clmsoil_heat_config *cfg, hipStream_t stream, unsigned *err) {
     HeatParams P{};
     P.a = *a;
     P.nlevsno = lev->nlevsno;
@@ -169,15 +146,9 @@ int run_heat(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime
     P.snow_method = cfg ? cfg->snow_thermal_cond_method : 1;
     P.dtime = dtime;
     P.err = err;
-    P.pivot_count = pivot_count;
-    P.pivot_list = pivot_list;
-    P.dense_ws = dense_ws;
-    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, pivot_count, 1);
-    if (hipGetLastError() != hipSuccess) return CLMSOIL_ERR_HIP;
-    if (launch_heat(P, g_tuning.heat_g, g_tuning.heat_block, stream) != hipSuccess)
-        return CLMSOIL_ERR_HIP;
-    if (launch_heat_pivot(P, stream) != hipSuccess) return CLMSOIL_ERR_HIP;
-    return CLMSOIL_OK;
+    return launch_heat(P, g_tuning.heat_g, g_tuning.heat_block, stream) == hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
 }
 
 }  // namespace clmsoil
@@ -228,7 +199,7 @@ int clmsoil_device_errors(void *stream, uint32_t *flags) {
 
 int clmsoil_set_tuning(int richards_g, int richards_block, int heat_g, int heat_block) {
     auto okg = [](int g) { return g == 1 || g == 2 || g == 4; };
-    auto okb = [](int b) { return b >= 64 && b <= 1024 && b % 64 == 0; };
+    auto okb = [](int b) { return b == 64 || b == 128 || b == 256; };
     if (!okg(richards_g) || !okg(heat_g) || !okb(richards_block) || !okb(heat_block))
         return CLMSOIL_ERR_ARG;
     g_tuning.richards_g = richards_g;
@@ -285,31 +256,7 @@ int clmsoil_soil_temperature(const clmsoil_heat_args *a, const clmsoil_levels *l
         return CLMSOIL_ERR_ARG;
     DeviceState *s = nullptr;
     if ((st = current_device_state(&s))) return st;
-    if ((st = ensure_pivot_list(s, a->nc))) return st;
-    return run_heat(a, lev, dtime, cfg, (hipStream_t)stream, s->err, s->pivot_count,
-                    s->pivot_list, s->dense_ws);
-}
-
-size_t clmsoil_heat_workspace_size(int64_t nc) {
-    const size_t list = ((size_t)(nc > 0 ? nc : 0) * sizeof(int64_t) + 255) / 256 * 256;
-    return 256 + list + DENSE_WS_BYTES;
-}
-
-int clmsoil_soil_temperature_ws(const clmsoil_heat_args *a, const clmsoil_levels *lev,
-                                double dtime, const clmsoil_heat_config *cfg, void *workspace,
-                                size_t workspace_bytes, void *stream) {
-    int st = validate_heat(a, lev);
-    if (st || a->nc == 0) return st;
-    if (!(dtime > 0.0) || !std::isfinite(dtime)) return CLMSOIL_ERR_ARG;
-    if (!workspace || workspace_bytes < clmsoil_heat_workspace_size(a->nc))
-        return CLMSOIL_ERR_WORKSPACE;
-    unsigned *words = static_cast<unsigned *>(workspace);
-    int64_t *list = reinterpret_cast<int64_t *>(static_cast<char *>(workspace) + 256);
-    const size_t list_bytes = ((size_t)a->nc * sizeof(int64_t) + 255) / 256 * 256;
-    double *dense = reinterpret_cast<double *>(static_cast<char *>(workspace) + 256 + list_bytes);
-    DeviceState *s = nullptr;
-    if ((st = current_device_state(&s))) return st;
-    return run_heat(a, lev, dtime, cfg, (hipStream_t)stream, s->err, words + 1, list, dense);
+    return run_heat(a, lev, dtime, cfg, (hipStream_t)stream, s->err);
 }
 
 int clmsoil_tridiagonal_multi(double *u, const double *a, const double *b, const double *c,

@@ -57,13 +57,18 @@ __device__ __forceinline__ double level_value(const double *x, int jj, int k) {
     return group_bcast<G>(v, (jj - 1) % G);
 }
 
-constexpr int DENSE_N = MAX_NLEVSNO + MAX_NLEVGRND + 1;   // 38 rows at most
+// LDS tile of one wavefront for the pivoting elimination: 38 rows x 7 band
+// entries (columns r-2 .. r+4) plus the right-hand side.
+constexpr int BAND_ROWS = MAX_NLEVSNO + MAX_NLEVGRND + 1;
+constexpr int BAND_W = 7;
+constexpr int TILE_DOUBLES = BAND_ROWS * BAND_W + BAND_ROWS;
 
 
 }  // namespace
 
-template <int G, bool DENSE>
-__device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c, const int k) {
+template <int G>
+__device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c, const int k,
+                                            double *band_tile) {
     constexpr int NL = MAX_NLEVSNO + MAX_NLEVGRND;   // 37 snow+soil levels
     constexpr int S = (NL + G - 1) / G;
     const clmsoil_heat_args &a = P.a;
@@ -308,7 +313,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     double x[S];
     double x_h = 0.0;
     bool pivot = false;
-    if constexpr (!DENSE) {
+    {
         double u0[S], y[S];
 #pragma unroll
         for (int i = 0; i < S; ++i) { u0[i] = 1.0; y[i] = 0.0; x[i] = 0.0; }
@@ -349,13 +354,6 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             pivot = pivot || (group_bcast<G>(piv ? 1.0 : 0.0, kk) != 0.0);
             if (jj == nlevsno) { u0_s0 = u0p; y_s0 = yp; b1_s0 = supp; }
         }
-        if (pivot) {
-            if (P.pivot_list && k == 0) {
-                const unsigned slot = atomicAdd(P.pivot_count, 1u);
-                P.pivot_list[slot] = c;
-            }
-            return;                                           // finished by the pivoting pass
-        }
         double xn = 0.0;
 #pragma unroll
         for (int jj = NL; jj >= 1; --jj) {
@@ -371,90 +369,104 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             xn = group_bcast<G>(ox, kk);
             if (jj == js1) x_h = (rv_h - b2_h * xn) / b3_h;
         }
-    } else {
-        // Dense elimination with partial pivoting, as the reference does it, on a
-        // per-group n x n matrix in the library's global workspace: every lane
-        // writes its rows, lane 0 of the group eliminates, every lane reads back.
-        const int64_t grp = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-        double *Am = P.dense_ws + grp * (int64_t)(DENSE_N * DENSE_N + DENSE_N);
-        double *xd = Am + DENSE_N * DENSE_N;
-        const int jt = jtop;
-        const int n = nlev - jtop + 2;                 // levels + h2osfc row
-        const int rh = nlevsno - jt + 2;               // system row of the h2osfc row
-#define AD(r, cc) Am[((r) - 1) + ((cc) - 1) * n]
-        for (int q = k; q < n * n; q += G) Am[q] = 0.0;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int i = 0; i < S; ++i) {
-            const int jj = i * G + k + 1;
-            if (jj < jtop || jj > nlev) continue;
-            const int r = jj <= nlevsno ? jj - jt + 1 : jj - jt + 2;
-            AD(r, r) = b3[i];
-            xd[r - 1] = rv[i];
-            if (jj <= nlevsno) {
-                if (jj > jtop) AD(r, r - 1) = sub[i];
-                if (jj < nlevsno) AD(r, r + 1) = sup[i];
-                else AD(r, r + 2) = sup[i];                  // band 1: snow 0 -> soil 1
-            } else if (jj == js1) {
-                if (jtop <= nlevsno) AD(r, r - 2) = sub[i];  // band 5
-                AD(r, r - 1) = b4h;                          // band 4
-                if (jj < nlev) AD(r, r + 1) = sup[i];
-            } else {
-                AD(r, r - 1) = sub[i];
-                if (jj < nlev) AD(r, r + 1) = sup[i];
+        // Columns whose elimination interchanges rows (the reference's partial
+        // pivoting, band_diagonal.jl:125-140) are re-solved one at a time per
+        // wavefront with the banded form of that elimination in an LDS tile.
+        unsigned long long pv = __ballot(pivot && k == 0);
+        while (pv) {
+            const int leader = __ffsll((long long)pv) - 1;
+            const bool mine = ((int)__lane_id() - k) == leader;
+            double *tile = band_tile;                   // this wavefront's tile
+            double *xt = tile + BAND_ROWS * BAND_W;
+            const int jt = jtop;
+            const int n = nlev - jtop + 2;
+            const int rh = nlevsno - jt + 2;
+            if (mine) {
+                for (int q = k; q < n * BAND_W; q += G) tile[q] = 0.0;
             }
-        }
-        if (k == 0) {
-            AD(rh, rh) = b3_h;
-            AD(rh, rh + 1) = b2_h;
-            xd[rh - 1] = rv_h;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        if (k == 0) {
-            for (int kq = 1; kq <= n - 1; ++kq) {
-                int pv = kq;
-                double maxv = fabs(AD(kq, kq));
-                for (int r = kq + 1; r <= n; ++r) {
-                    const double v = fabs(AD(r, kq));
-                    if (v > maxv) { maxv = v; pv = r; }
-                }
-                const double akk = AD(pv, kq);
-                if (pv != kq) {
-                    for (int cc = kq; cc <= n; ++cc) {
-                        const double akj = AD(kq, cc), apj = AD(pv, cc);
-                        AD(kq, cc) = apj;
-                        AD(pv, cc) = akj;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+#define BT(r, cc) tile[((r) - 1) * BAND_W + ((cc) - (r) + 2)]
+            if (mine) {
+#pragma unroll
+                for (int i = 0; i < S; ++i) {
+                    const int jj = i * G + k + 1;
+                    if (jj < jtop || jj > nlev) continue;
+                    const int r = jj <= nlevsno ? jj - jt + 1 : jj - jt + 2;
+                    BT(r, r) = b3[i];
+                    xt[r - 1] = rv[i];
+                    if (jj <= nlevsno) {
+                        if (jj > jtop) BT(r, r - 1) = sub[i];
+                        if (jj < nlevsno) BT(r, r + 1) = sup[i];
+                        else BT(r, r + 2) = sup[i];
+                    } else if (jj == js1) {
+                        if (jtop <= nlevsno) BT(r, r - 2) = sub[i];
+                        BT(r, r - 1) = b4h;
+                        if (jj < nlev) BT(r, r + 1) = sup[i];
+                    } else {
+                        BT(r, r - 1) = sub[i];
+                        if (jj < nlev) BT(r, r + 1) = sup[i];
                     }
-                    const double xk = xd[kq - 1], xp = xd[pv - 1];
-                    xd[kq - 1] = xp;
-                    xd[pv - 1] = xk;
                 }
-                for (int r = kq + 1; r <= n; ++r) {
-                    const double f = AD(r, kq) / akk;
-                    for (int cc = kq; cc <= n; ++cc) AD(r, cc) = AD(r, cc) - f * AD(kq, cc);
-                    xd[r - 1] = xd[r - 1] - f * xd[kq - 1];
+                if (k == 0) {
+                    BT(rh, rh) = b3_h;
+                    BT(rh, rh + 1) = b2_h;
+                    xt[rh - 1] = rv_h;
                 }
             }
-            for (int r = n; r >= 1; --r) {
-                double sacc = xd[r - 1];
-                for (int cc = r + 1; cc <= n; ++cc) sacc = sacc - AD(r, cc) * xd[cc - 1];
-                xd[r - 1] = sacc / AD(r, r);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            if ((int)__lane_id() == leader) {
+                for (int kq = 1; kq <= n - 1; ++kq) {
+                    const int rlast = kq + 2 < n ? kq + 2 : n;
+                    const int clast = kq + 4 < n ? kq + 4 : n;
+                    int pr = kq;
+                    double maxv = fabs(BT(kq, kq));
+                    for (int r = kq + 1; r <= rlast; ++r) {
+                        const double v = fabs(BT(r, kq));
+                        if (v > maxv) { maxv = v; pr = r; }
+                    }
+                    const double akk = BT(pr, kq);
+                    if (pr != kq) {
+                        for (int cc = kq; cc <= clast; ++cc) {
+                            const double akj = BT(kq, cc), apj = BT(pr, cc);
+                            BT(kq, cc) = apj;
+                            BT(pr, cc) = akj;
+                        }
+                        const double xk = xt[kq - 1], xp = xt[pr - 1];
+                        xt[kq - 1] = xp;
+                        xt[pr - 1] = xk;
+                    }
+                    for (int r = kq + 1; r <= rlast; ++r) {
+                        const double f = BT(r, kq) / akk;
+                        for (int cc = kq; cc <= clast; ++cc) BT(r, cc) = BT(r, cc) - f * BT(kq, cc);
+                        xt[r - 1] = xt[r - 1] - f * xt[kq - 1];
+                    }
+                }
+                for (int r = n; r >= 1; --r) {
+                    double sacc = xt[r - 1];
+                    const int clast = r + 4 < n ? r + 4 : n;
+                    for (int cc = r + 1; cc <= clast; ++cc) sacc = sacc - BT(r, cc) * xt[cc - 1];
+                    xt[r - 1] = sacc / BT(r, r);
+                }
             }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
+#undef BT
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            if (mine) {
 #pragma unroll
-        for (int i = 0; i < S; ++i) {
-            const int jj = i * G + k + 1;
-            x[i] = 0.0;
-            if (jj < jtop || jj > nlev) continue;
-            const int r = jj <= nlevsno ? jj - jt + 1 : jj - jt + 2;
-            x[i] = xd[r - 1];
+                for (int i = 0; i < S; ++i) {
+                    const int jj = i * G + k + 1;
+                    if (jj < jtop || jj > nlev) continue;
+                    const int r = jj <= nlevsno ? jj - jt + 1 : jj - jt + 2;
+                    x[i] = xt[r - 1];
+                }
+                x_h = xt[rh - 1];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            pv &= pv - 1;
         }
-        x_h = xd[rh - 1];
-#undef AD
     }
 
     // ---- unpack (:149-167) and fn1 (:170-206) ----
@@ -790,24 +802,13 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
 
 template <int G>
 __global__ __launch_bounds__(256, CLMSOIL_HEAT_WAVES) void heat_kernel(HeatParams P) {
+    __shared__ double tiles[256 / 64][TILE_DOUBLES];
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t c = gtid / G;
     const int k = (int)(threadIdx.x % G);
     if (c >= P.a.nc) return;
     if (P.a.mask && !P.a.mask[c]) return;
-    heat_column<G, false>(P, c, k);
-}
-
-// Pivoting pass: a small grid strides over the columns the fast pass recorded;
-// each lane group owns one dense system in the workspace (PIVOT_GROUPS of them).
-
-__global__ __launch_bounds__(PIVOT_BLOCK) void heat_pivot_kernel(HeatParams P) {
-    constexpr int G = 4;
-    const int64_t cnt = (int64_t)*P.pivot_count;
-    const int k = (int)(threadIdx.x % G);
-    const int64_t ngroups = (int64_t)gridDim.x * blockDim.x / G;
-    for (int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; t < cnt; t += ngroups)
-        heat_column<G, true>(P, P.pivot_list[t], k);
+    heat_column<G>(P, c, k, tiles[threadIdx.x / 64]);
 }
 
 template <int G>
@@ -825,12 +826,6 @@ hipError_t launch_heat(const HeatParams &P, int G, int block, hipStream_t stream
         case 2: return launch_g<2>(P, stream, block);
         default: return launch_g<4>(P, stream, block);
     }
-}
-
-hipError_t launch_heat_pivot(const HeatParams &P, hipStream_t stream) {
-    if (P.a.nc == 0) return hipSuccess;
-    hipLaunchKernelGGL(heat_pivot_kernel, dim3(PIVOT_GRID), dim3(PIVOT_BLOCK), 0, stream, P);
-    return hipGetLastError();
 }
 
 }  // namespace clmsoil

@@ -16,14 +16,9 @@ struct Tuning {
 extern Tuning g_tuning;
 
 // Allocated once per device on first use: a 256-byte status block (word 0 device
-// error bits, word 1 pivot counter, words 8..15 per-XCD work counters) and the
-// heat step's pivot list.
+// error bits, words 8..15 per-XCD work counters of the persistent Richards launch).
 struct DeviceState {
     unsigned *err = nullptr;
-    unsigned *pivot_count = nullptr;
-    int64_t *pivot_list = nullptr;
-    int64_t pivot_cap = 0;
-    double *dense_ws = nullptr;    // dense systems of the heat pivoting pass
 };
 
 int current_device_state(DeviceState **out);

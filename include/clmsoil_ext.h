@@ -1,6 +1,6 @@
 /*
  * clmsoil_ext.h — entry points of libclmsoil with no direct counterpart in
- * CLM.jl's interfaces: device status, launch tuning, caller-provided workspace.
+ * CLM.jl's interfaces: device status and launch tuning.
  * Conventions as in clmsoil.h.
  */
 #ifndef CLMSOIL_EXT_H
@@ -24,7 +24,7 @@ extern "C" {
  * reference raises these conditions as Julia bounds errors inside its kernels. */
 int clmsoil_device_errors(void *stream, uint32_t *flags);
 
-/* Lanes per column (1, 2 or 4) and workgroup size (multiple of 64) of the fused
+/* Lanes per column (1, 2 or 4) and workgroup size (64, 128 or 256) of the fused
  * Richards and heat kernels; process-wide, read at launch. */
 int clmsoil_set_tuning(int richards_g, int richards_block, int heat_g, int heat_block);
 int clmsoil_get_tuning(int *richards_g, int *richards_block, int *heat_g, int *heat_block);
@@ -32,15 +32,6 @@ int clmsoil_get_tuning(int *richards_g, int *richards_block, int *heat_g, int *h
  * take a new column from a work counter as soon as their column's substeps are
  * done; 0: one column per lane group for the launch. Results are identical. */
 int clmsoil_set_richards_schedule(int persistent);
-
-/* Heat step with a caller-provided device workspace of at least
- * clmsoil_heat_workspace_size(nc) bytes (pivot list); allocates nothing.
- * clmsoil_soil_temperature instead uses a library-owned per-device list that is
- * allocated on first use and grown when nc grows. */
-size_t clmsoil_heat_workspace_size(int64_t nc);
-int clmsoil_soil_temperature_ws(const clmsoil_heat_args *a, const clmsoil_levels *lev,
-                                double dtime, const clmsoil_heat_config *cfg,
-                                void *workspace, size_t workspace_bytes, void *stream);
 
 #ifdef __cplusplus
 }
