@@ -20,6 +20,11 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "clm.jl_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch at 1,048,576 columns (seed 2), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE counters, each scaled by what the counter reads for a plain 2 GiB
+# device copy (fetch x2.0, write x1.0); recorded in
+# profiles/hbm_traffic_production_day_1M.md.
+MEASURED_TRAFFIC_1M = {"heat": 4.734e9, "richards": 3.798e9}
 METRIC = "column-steps/s @1M cols FP64 soil solves; HBM GB/s frac; learned vs plain dt"
 
 
@@ -75,7 +80,7 @@ HEAT_KEYS = ["snl", "nbedrock", "z", "dz", "zi", "t_soisno", "h2osoi_liq", "h2os
 
 def cpu_baseline(host, sample, steps):
     """The oracle (a C restatement of CLM.jl's CPU path, OpenMP over columns) on
-    the first `sample` columns for `steps` steps, best of 3."""
+    the first `sample` columns for `steps` steps, best of 2."""
     import numpy as np
     sys.path.insert(0, str(REPO / "oracle"))
     import oracle
@@ -90,7 +95,7 @@ def cpu_baseline(host, sample, steps):
     cfg = bind.swm_config(lower_bc=3)
     hcfg = bind.heat_config()
     best = float("inf")
-    for _ in range(3):
+    for _ in range(2):
         s = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in sub.items()}
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -107,9 +112,12 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--nc", type=int, default=1 << 20)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=65536)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 20)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 4 times (the known-size kernel "
+                         "that scales FETCH_SIZE / WRITE_SIZE counters), then exit")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the per-column solves need no exchange between "
@@ -121,6 +129,15 @@ def main(argv=None):
     from clmsoil import api
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 8, "copies": 4}), flush=True)
+        return 0
     nc = args.nc
     host, dev, gen_s = make_state(nc, args.seed)
     print(f"# state ready: {nc} columns in {gen_s:.1f} s", file=sys.stderr, flush=True)
@@ -163,7 +180,9 @@ def main(argv=None):
                    "columns": nc, "layers": "12 snow + 25 ground (20 soil)",
                    "step_seconds": 1800.0, "tuning": list(api.get_tuning())},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC_1M[dom] if (nc == 1 << 20 and args.seed == 2)
+                     else None},
         "kernels_ms": {"heat": heat_ms, "richards": rich_ms},
         "fused_step_hbm_frac": (b_r + b_h) / (total_ms / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS,
         "mean_substep_iterations": nsub,
@@ -174,7 +193,7 @@ def main(argv=None):
         out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": cores,
                                "kind": "port",
                                "sample": f"first {min(args.cpu_sample, nc)} columns x {args.cpu_steps} "
-                                         f"steps of the same workload, best of 3 ({secs:.2f} s)"}
+                                         f"steps of the same workload, best of 2 ({secs:.2f} s)"}
     print(json.dumps(out), flush=True)
     return 0
 
