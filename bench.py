@@ -105,6 +105,20 @@ def cpu_baseline(host, sample, steps):
     return sample * steps / best, oracle.num_threads(), best
 
 
+def learned_vs_plain(nc, steps):
+    """The committed first-substep network against the plain controller: the same
+    config-0 columns (20 soil layers, BC_ZERO_FLUX, per-step infiltration
+    U(0, 2e-3) mm/s, held-out seed) advanced `steps` steps by each."""
+    from clmsoil import api, learned
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_ZERO_FLUX)
+    pred = learned.DtPredictor.load()
+    r = learned.compare_controllers(pred, nc, steps, 2024, cfg)
+    r["workload"] = (f"{nc} config-0 columns x {steps} steps, Richards only (BC_ZERO_FLUX), "
+                     f"prediction kernel included in the learned timing")
+    return r
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +129,8 @@ def main(argv=None):
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-vs-plain first-substep comparison")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -188,6 +204,8 @@ def main(argv=None):
         "mean_substep_iterations": nsub,
         "wall_s": wall,
     }
+    if not args.no_learned:
+        out["learned_vs_plain"] = learned_vs_plain(nc, args.steps)
     if not args.no_cpu_baseline:
         v, cores, secs = cpu_baseline(host, min(args.cpu_sample, nc), args.cpu_steps)
         out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": cores,

@@ -17,3 +17,19 @@ def declare(lib) -> None:
     lib.clmsoil_get_tuning.restype = I
     lib.clmsoil_set_richards_schedule.argtypes = [I]
     lib.clmsoil_set_richards_schedule.restype = I
+    _declare_learned(lib)
+
+
+def _declare_learned(lib) -> None:
+    """include/clmsoil_learned.h"""
+    P, I, L, F, D = C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_double
+    lib.clmsoil_richards_features.argtypes = [P, P, D, P, P, L, P]
+    lib.clmsoil_richards_features.restype = I
+    lib.clmsoil_mlp_predict_dtsub0.argtypes = [P, P, D, P, P, P, P]
+    lib.clmsoil_mlp_predict_dtsub0.restype = I
+    lib.clmsoil_mlp_forward.argtypes = [P, P, L, L, P, P]
+    lib.clmsoil_mlp_forward.restype = I
+    lib.clmsoil_mlp_grad.argtypes = [P, P, P, L, L, L, P, P, P]
+    lib.clmsoil_mlp_grad.restype = I
+    lib.clmsoil_mlp_adam.argtypes = [P, P, P, P, L, L, F, F, F, F, P, P]
+    lib.clmsoil_mlp_adam.restype = I

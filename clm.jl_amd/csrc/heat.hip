@@ -223,7 +223,6 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     const double thk1 = level_value<G, S>(thk, js1, k);
     const double z1 = level_value<G, S>(z, js1, k);
     const double t1 = level_value<G, S>(t, js1, k);
-    const double fact1 = level_value<G, S>(fact, js1, k);
     double tk_h2osfc;
     {
         const double zh2osfc = 1.0e-3 * (0.5 * h2osfc);

@@ -2,6 +2,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "../../include/clmsoil.h"
+
 namespace clmsoil {
 
 // Lanes per column (G) and workgroup size of the two fused kernels. Defaults are
@@ -22,5 +24,6 @@ struct DeviceState {
 };
 
 int current_device_state(DeviceState **out);
+int validate_levels(const clmsoil_levels *lev);
 
 }  // namespace clmsoil
