@@ -18,6 +18,7 @@ def declare(lib) -> None:
     lib.clmsoil_set_richards_schedule.argtypes = [I]
     lib.clmsoil_set_richards_schedule.restype = I
     _declare_learned(lib)
+    _declare_adjoint(lib)
 
 
 def _declare_learned(lib) -> None:
@@ -33,3 +34,16 @@ def _declare_learned(lib) -> None:
     lib.clmsoil_mlp_grad.restype = I
     lib.clmsoil_mlp_adam.argtypes = [P, P, P, P, L, L, F, F, F, F, P, P]
     lib.clmsoil_mlp_adam.restype = I
+
+
+def _declare_adjoint(lib) -> None:
+    """include/clmsoil_adjoint.h"""
+    P, I, L, D = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    lib.clmsoil_adjoint_workspace_bytes.argtypes = [L]
+    lib.clmsoil_adjoint_workspace_bytes.restype = L
+    lib.clmsoil_soilwm_moisture_form_adjoint.argtypes = [P, P, D, P, P, P, P, P, P, P, P, L, P]
+    lib.clmsoil_soilwm_moisture_form_adjoint.restype = I
+    lib.clmsoil_tridiagonal_multi_vjp.argtypes = [P] * 7 + [L, L, L] + [P] * 5
+    lib.clmsoil_tridiagonal_multi_vjp.restype = I
+    lib.clmsoil_batched_band_solve_vjp.argtypes = [P] * 6 + [I, I, I, L, L, L, P, P, P]
+    lib.clmsoil_batched_band_solve_vjp.restype = I

@@ -276,3 +276,45 @@ def production_day(nc=1 << 20, seed=2, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrn
         hk_l=np.full((nlevgrnd, nc), fill), qcharge=np.full(nc, fill),
         nsubsteps=np.full(nc, fill))
     return h
+
+
+def calibration_columns(nc=262144, seed=3, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrnd=NLEVGRND):
+    """Config 3: snow-free, unfrozen columns for the differentiable calibration.
+
+    Texture and pedotransfer as config 0, nbedrock ~ U{5..20}, liquid water
+    theta ~ U(0.3, 0.95) x watsat with no ice, t_soisno ~ U(283, 303) K, ground
+    heat flux forcing hs ~ U(0, 40) W/m2 (warming only, so no layer reaches
+    freezing within 30 days and the heat step never changes the water state),
+    dhsdT = -4*0.97*sigma*T_top^3. The Richards lower boundary is BC_FLUX (free
+    drainage), whose qcharge is the drainage the calibration loss uses.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sand, clay, om = _texture(rng, nc)
+    dzs, zs, zis = clm5_grid(nlevsoi, nlevgrnd)
+    props = pedotransfer(sand, clay, om, zs)
+    snl = np.zeros(nc, dtype=np.int64)
+    z, dz, zi = _geometry(nc, snl, np.zeros((nlevsno, nc)), nlevsno, nlevsoi, nlevgrnd, 0.0)
+    nlev = nlevsno + nlevgrnd
+    liq = np.zeros((nlev, nc))
+    liq[nlevsno:] = rng.uniform(0.3, 0.95, (nlevgrnd, nc)) * props["watsat"] * dzs[:, None] * DENH2O
+    t = np.zeros((nlev, nc))
+    t[nlevsno:] = rng.uniform(283.0, 303.0, (nlevgrnd, nc))
+    nb = rng.integers(5, nlevsoi + 1, nc).astype(np.int64)
+    hs = rng.uniform(0.0, 40.0, nc)
+    dhsdT = -4.0 * 0.97 * SB * t[nlevsno] ** 3
+    return dict(
+        nc=nc, nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd, dtime=1800.0,
+        snl=snl, nbedrock=nb, z=z, dz=dz, zi=zi, t_soisno=t, h2osoi_liq=liq,
+        h2osoi_ice=np.zeros((nlev, nc)), excess_ice=np.zeros((nlevgrnd, nc)),
+        watsat=props["watsat"], tkmg=props["tkmg"], tkdry=props["tkdry"],
+        csol=props["csol"], bsw=props["bsw"], sucsat=props["sucsat"], hksat=props["hksat"],
+        frac_sno_eff=np.zeros(nc), frac_h2osfc=np.zeros(nc),
+        hs_top_snow=hs.copy(), hs_soil=hs.copy(), hs_top=hs.copy(), hs_h2osfc=hs.copy(),
+        dhsdT=dhsdT, sabg_lyr_col=np.zeros((nlevsno + 1, nc)), eflx_bot=np.zeros(nc),
+        t_h2osfc=t[nlevsno].copy(), h2osfc=np.zeros(nc), h2osno_no_layers=np.zeros(nc),
+        snow_depth=np.zeros(nc), int_snow=np.zeros(nc),
+        lower_bc=1, qflx_infl=rng.uniform(0.0, 2.0e-3, nc), zwt=np.full(nc, 5.0),
+        icefrac=np.zeros((nlevgrnd, nc)), qflx_rootsoi=np.zeros((nlevsoi, nc)),
+        smpmin=np.full(nc, -1.0e8), smp_l=np.zeros((nlevgrnd, nc)),
+        hk_l=np.zeros((nlevgrnd, nc)), qcharge=np.zeros(nc), nsubsteps=np.zeros(nc),
+        nrejected=np.zeros(nc))

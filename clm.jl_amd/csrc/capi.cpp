@@ -14,6 +14,7 @@
 
 #include "../../include/clmsoil.h"
 #include "../../include/clmsoil_ext.h"
+#include "../../include/clmsoil_adjoint.h"
 #include "common.hpp"
 #include "heat.hpp"
 #include "richards.hpp"
@@ -286,6 +287,38 @@ int clmsoil_batched_band_solve(double *tvector, const double *bmatrix, const dou
     BandParams P{tvector, bmatrix, rvector, jtop, jbot, mask, kl, ku, nlevsno_off,
                  nc, nlev, ld, s->err};
     return launch_band_solve(P, (hipStream_t)stream) == hipSuccess ? CLMSOIL_OK : CLMSOIL_ERR_HIP;
+}
+
+int clmsoil_tridiagonal_multi_vjp(double *ubar, const double *u, const double *a, const double *b,
+                                  const double *c, const int64_t *jtop, const uint8_t *mask,
+                                  int64_t ncols, int64_t nlevs, int64_t ld, double *abar,
+                                  double *bbar, double *cbar, double *rbar, void *stream) {
+    if (ncols < 0 || ld < ncols || nlevs < 1 || nlevs > 64) return CLMSOIL_ERR_SIZE;
+    if (ncols == 0) return CLMSOIL_OK;
+    if (!ubar || !u || !a || !b || !c || !jtop) return CLMSOIL_ERR_NULL;
+    DeviceState *s = nullptr;
+    int st = current_device_state(&s);
+    if (st) return st;
+    TridiagVjpParams P{ubar, u, a, b, c, jtop, mask, ncols, nlevs, ld, abar, bbar, cbar, rbar,
+                       s->err};
+    return launch_tridiag_vjp(P, (hipStream_t)stream) == hipSuccess ? CLMSOIL_OK : CLMSOIL_ERR_HIP;
+}
+
+int clmsoil_batched_band_solve_vjp(double *tbar, const double *tvector, const double *bmatrix,
+                                   const int64_t *jtop, const int64_t *jbot, const uint8_t *mask,
+                                   int32_t kl, int32_t ku, int32_t nlevsno_off, int64_t nc,
+                                   int64_t nlev, int64_t ld, double *bmatrix_bar,
+                                   double *rvector_bar, void *stream) {
+    if (nc < 0 || ld < nc || nlev < 1) return CLMSOIL_ERR_SIZE;
+    if (kl < 1 || ku < 1 || kl > 2 || ku > 2) return CLMSOIL_ERR_ARG;
+    if (nc == 0) return CLMSOIL_OK;
+    if (!tbar || !tvector || !bmatrix || !jtop || !jbot) return CLMSOIL_ERR_NULL;
+    DeviceState *s = nullptr;
+    int st = current_device_state(&s);
+    if (st) return st;
+    BandVjpParams P{tbar, tvector, bmatrix, jtop, jbot, mask, kl, ku, nlevsno_off, nc, nlev, ld,
+                    bmatrix_bar, rvector_bar, s->err};
+    return launch_band_vjp(P, (hipStream_t)stream) == hipSuccess ? CLMSOIL_OK : CLMSOIL_ERR_HIP;
 }
 
 }  // extern "C"

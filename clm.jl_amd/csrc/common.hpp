@@ -134,6 +134,7 @@ enum DeviceError : unsigned {
     DEV_ERR_NBEDROCK = 1u,
     DEV_ERR_SNL = 2u,
     DEV_ERR_ITER = 4u,
+    DEV_ERR_TAPE = 8u,
 };
 
 }  // namespace clmsoil

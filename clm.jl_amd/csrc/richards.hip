@@ -28,350 +28,9 @@
 // as soon as its column is done (wavefront-aggregated atomics, one counter per
 // XCD over a contiguous column range so neighbouring columns share an L2; idle
 // groups move on to other XCDs' ranges at the end).
-#include "common.hpp"
-#include "richards.hpp"
+#include "rcolumn.hpp"
 
 namespace clmsoil {
-
-template <int G>
-__device__ __forceinline__ double nxt(const double *x, int i, int k, int S) {
-    // value of layer j+1 for the layer in slot i of lane k
-    double a = group_next<G>(x[i]);
-    double b = (i + 1 < S) ? group_next<G>(x[i + 1]) : 0.0;
-    return (k == G - 1) ? b : a;
-}
-template <int G>
-__device__ __forceinline__ double prv(const double *x, int i, int k) {
-    double a = group_prev<G>(x[i]);
-    double b = (i > 0) ? group_prev<G>(x[i - 1]) : 0.0;
-    return (k == 0) ? b : a;
-}
-
-// One column's state while a group works on it (all in registers).
-template <int G>
-struct RColumn {
-    static constexpr int NMAX = MAX_NLEVSOI;
-    static constexpr int S = (NMAX + G - 1) / G;
-    int64_t c;
-    int n, jwt, nsubstep, nrej, iter;
-    bool props_valid;
-    double zwt, qflx_infl, dtsub, dtdone, qc;
-    double liq[S], dz[S], watsat[S], khat[S], bsw[S], sucsat[S], root[S], den[S];
-    double hk[S], smp[S], dhkdw[S], qin[S], qout[S], dqidw0[S], dqidw1[S], dqodw1[S], dqodw2[S];
-    double gam[S], dwat[S];
-
-    // Load column `col`; returns false (and flags the error word) for a bad nbedrock.
-    __device__ __forceinline__ bool load(const RichardsParams &P, int64_t col, int k) {
-        c = col;
-        n = (int)P.nbedrock[c];
-        if (n < 1 || n > P.nlevsoi) {
-            if (k == 0) atomicOr(P.err, (unsigned)DEV_ERR_NBEDROCK);
-            return false;
-        }
-        const int64_t ld = P.ld;
-        const int joff = P.nlevsno, joff_zi = P.nlevsno + 1;
-        // water-table index (BC_WATERTABLE, :1346-1350)
-        jwt = P.nlevsoi;
-        zwt = 0.0;
-        if (P.lower_bc == CLMSOIL_BC_WATERTABLE || P.aquifer_pass) {
-            zwt = P.zwt[c];
-            for (int jj = 1; jj <= P.nlevsoi; ++jj)
-                if (zwt <= P.zi[c + (int64_t)(joff_zi + jj - 1) * ld]) { jwt = jj - 1; break; }
-        }
-#pragma unroll
-        for (int i = 0; i < S; ++i) {
-            const int j = i * G + k + 1;
-            if (j <= n) {
-                const int64_t o = c + (int64_t)(j - 1) * ld;           // soil-array offset
-                const int64_t os = c + (int64_t)(joff + j - 1) * ld;   // snow+soil offset
-                liq[i] = P.h2osoi_liq[os];
-                dz[i] = __builtin_nontemporal_load(&P.dz[os]);
-                watsat[i] = __builtin_nontemporal_load(&P.watsat[o]);
-                bsw[i] = __builtin_nontemporal_load(&P.bsw[o]);
-                sucsat[i] = __builtin_nontemporal_load(&P.sucsat[o]);
-                root[i] = __builtin_nontemporal_load(&P.qflx_rootsoi[o]);
-                const double hks = __builtin_nontemporal_load(&P.hksat[o]);
-                const double icf = __builtin_nontemporal_load(&P.icefrac[o]);
-                const double zj = __builtin_nontemporal_load(&P.z[os]);
-                double imp;
-                if (j == n) {
-                    imp = pow(10.0, -P.e_ice * icf);
-                    den[i] = M_TO_MM * (zwt - zj);                     // :1357 (WATERTABLE)
-                } else {
-                    const double icf1 = __builtin_nontemporal_load(&P.icefrac[o + ld]);
-                    imp = pow(10.0, -P.e_ice * (0.5 * (icf + icf1)));
-                    den[i] = M_TO_MM * (__builtin_nontemporal_load(&P.z[os + ld]) - zj);   // :1338
-                }
-                khat[i] = imp * hks;   // (imp * hksat) * s1^(2b+3), the reference's grouping
-            } else {
-                liq[i] = 0.0; dz[i] = 1.0; watsat[i] = 1.0; khat[i] = 0.0; bsw[i] = 1.0;
-                sucsat[i] = 0.0; root[i] = 0.0; den[i] = 1.0;
-            }
-            hk[i] = smp[i] = dhkdw[i] = qin[i] = qout[i] = 0.0;
-            dqidw0[i] = dqidw1[i] = dqodw1[i] = dqodw2[i] = gam[i] = dwat[i] = 0.0;
-        }
-        qflx_infl = P.qflx_infl[c];
-        nsubstep = 0;
-        nrej = 0;
-        iter = 0;
-        const double dtime = P.dtime;
-        dtsub = dtime;
-        if (P.dtsub0) {
-            const double d0 = P.dtsub0[c], lo = jl_min(P.dtmin, dtime);
-            dtsub = (d0 != d0) ? dtime : jl_min(jl_max(d0, lo), dtime);
-        }
-        dtdone = 0.0;
-        qc = 0.0;
-        props_valid = false;
-        return true;
-    }
-
-    // One pass of the reference's substep loop; returns true when the column's
-    // step is complete.
-    __device__ __forceinline__ bool iterate(const RichardsParams &P, int k) {
-        const int lower_bc = P.lower_bc;
-        const double dtime = P.dtime;
-        if (iter >= P.max_iter) {
-            if (k == 0) atomicOr(P.err, (unsigned)DEV_ERR_ITER);
-            return true;
-        }
-        ++iter;
-        nsubstep += 1;
-        if (!props_valid) {
-            // node saturation s2 (:1273-1286)
-            double s2[S];
-#pragma unroll
-            for (int i = 0; i < S; ++i) {
-                const double vwc = jl_max(liq[i], 1.0e-6) / (dz[i] * DENH2O);
-                double s = vwc / watsat[i];
-                s = jl_min(s, 1.0);
-                s = jl_max(0.01, s);
-                s2[i] = s;
-            }
-            double dsmpdw[S];
-            // Clapp-Hornberger at interface (hk) and node (smp) (:1287-1309)
-#pragma unroll
-            for (int i = 0; i < S; ++i) {
-                const int j = i * G + k + 1;
-                const double s2n = nxt<G>(s2, i, k, S);
-                double s1 = (j == n) ? s2[i] : 0.5 * (s2[i] + s2n);
-                s1 = jl_min(s1, 1.0);
-                s1 = jl_max(0.01, s1);
-                const double b = bsw[i];
-                const double hkv = khat[i] * pow(s1, 2.0 * b + 3.0);
-                const double dhkds = (2.0 * b + 3.0) * hkv / s1;
-                const double sn = s2[i];
-                const double smpv = -sucsat[i] * pow(sn, -b);
-                const double dsmpds = -b * smpv / sn;
-                hk[i] = hkv;
-                smp[i] = smpv;
-                dhkdw[i] = dhkds;
-                dsmpdw[i] = dsmpds / watsat[i];
-            }
-            // interface fluxes and derivatives (:1334-1367)
-#pragma unroll
-            for (int i = 0; i < S; ++i) {
-                const int j = i * G + k + 1;
-                const double smpn = nxt<G>(smp, i, k, S);
-                const double dsmpdwn = nxt<G>(dsmpdw, i, k, S);
-                const double watsatn = nxt<G>(watsat, i, k, S);
-                double qo = 0.0, d1 = 0.0, d2 = 0.0;
-                if (j < n) {
-                    const double dhkds1 = 0.5 * dhkdw[i] / watsat[i];
-                    const double dhkds2 = 0.5 * dhkdw[i] / watsatn;
-                    const double num = smpn - smp[i];
-                    const double dn = den[i];
-                    qo = -hk[i] * num / dn + hk[i];
-                    d1 = (hk[i] * dsmpdw[i] - dhkds1 * num) / dn + dhkds1;
-                    d2 = (-hk[i] * dsmpdwn - dhkds2 * num) / dn + dhkds2;
-                } else if (j == n) {
-                    if (lower_bc == CLMSOIL_BC_WATERTABLE) {
-                        if (!(j > jwt)) {
-                            const double dhkds1l = dhkdw[i] / watsat[i];
-                            const double num = -smp[i];
-                            const double dn = den[i];
-                            qo = -hk[i] * num / dn + hk[i];
-                            d1 = (hk[i] * dsmpdw[i] - dhkds1l * num) / dn + dhkds1l;
-                        }
-                    } else if (lower_bc == CLMSOIL_BC_FLUX) {
-                        qo = hk[i];
-                        d1 = dhkdw[i] / watsat[i];
-                    }
-                }
-                qout[i] = qo;
-                dqodw1[i] = d1;
-                dqodw2[i] = d2;
-            }
-            // inflow at the top of each layer (:1371-1378)
-#pragma unroll
-            for (int i = 0; i < S; ++i) {
-                const int j = i * G + k + 1;
-                const double po = prv<G>(qout, i, k);
-                const double p1 = prv<G>(dqodw1, i, k);
-                const double p2 = prv<G>(dqodw2, i, k);
-                qin[i] = (j == 1) ? qflx_infl : po;
-                dqidw0[i] = (j == 1) ? 0.0 : p1;
-                dqidw1[i] = (j == 1) ? 0.0 : p2;
-            }
-            props_valid = true;
-        }
-
-        // ---- RHS/LHS and Thomas forward sweep (:1381-1431) ----
-        double bet = 0.0, dwp = 0.0, cmxp = 0.0;
-#pragma unroll
-        for (int j = 1; j <= NMAX; ++j) {
-            const int i = (j - 1) / G, kk = (j - 1) % G;
-            if (j > n) break;
-            double ob = 0.0, od = 0.0, oc = 0.0;
-            if (k == kk) {
-                const double dt_dz = dtsub / (M_TO_MM * dz[i]);
-                const double fluxNet = qin[i] - qout[i] - root[i];
-                const double rmx = -fluxNet * dt_dz;
-                const double amx = (j == 1) ? 0.0 : dqidw0[i] * dt_dz;
-                const double bmx = -1.0 - (-dqidw1[i] + dqodw1[i]) * dt_dz;
-                const double cmx = (j == n) ? 0.0 : -dqodw2[i] * dt_dz;
-                double b2, d;
-                if (j == 1) {
-                    b2 = bmx;
-                    if (fabs(b2) < 1.0e-30) b2 = 1.0e-30;
-                    d = rmx / b2;
-                } else {
-                    const double g = cmxp / bet;
-                    gam[i] = g;
-                    b2 = bmx - amx * g;
-                    if (fabs(b2) < 1.0e-30) b2 = copysign(1.0e-30, b2 == 0.0 ? 1.0 : b2);
-                    d = (rmx - amx * dwp) / b2;
-                }
-                dwat[i] = d;
-                ob = b2; od = d; oc = cmx;
-            }
-            bet = group_bcast<G>(ob, kk);
-            dwp = group_bcast<G>(od, kk);
-            cmxp = group_bcast<G>(oc, kk);
-        }
-        // ---- back substitution (:1434-1438) ----
-        double gn = 0.0, dn = 0.0;
-#pragma unroll
-        for (int j = NMAX; j >= 1; --j) {
-            const int i = (j - 1) / G, kk = (j - 1) % G;
-            if (j > n) continue;
-            double og = 0.0, od = 0.0;
-            if (k == kk) {
-                if (j < n) dwat[i] = dwat[i] - gn * dn;
-                og = gam[i];
-                od = dwat[i];
-            }
-            gn = group_bcast<G>(og, kk);
-            dn = group_bcast<G>(od, kk);
-        }
-        // ---- error estimate (:1441-1457) ----
-        double errorMax = 0.0;
-#pragma unroll
-        for (int i = 0; i < S; ++i) {
-            const int j = i * G + k + 1;
-            double fluxNet0;
-            const double dt_dz = dtsub / (M_TO_MM * dz[i]);
-            if (P.expensive) {
-                const double dwm = prv<G>(dwat, i, k);
-                const double dwn = nxt<G>(dwat, i, k, S);
-                const double qin_test = (j == 1) ? qin[i] + dqidw1[i] * dwat[i]
-                                                 : qin[i] + dqidw0[i] * dwm + dqidw1[i] * dwat[i];
-                const double qout_test = (j == n) ? qout[i] + dqodw1[i] * dwat[i]
-                                                  : qout[i] + dqodw1[i] * dwat[i] + dqodw2[i] * dwn;
-                fluxNet0 = qin_test - qout_test - root[i];
-            } else {
-                fluxNet0 = dwat[i] / dt_dz;
-            }
-            const double fluxNet1 = qin[i] - qout[i] - root[i];
-            const double ev = fabs(fluxNet1 - fluxNet0) * dtsub * 0.5;
-            if (j <= n) errorMax = jl_max(errorMax, ev);
-        }
-        errorMax = group_max<G>(errorMax);
-
-        if (errorMax > P.xtu && dtsub > P.dtmin) {                 // :1460-1463
-            dtsub = jl_max(dtsub / 2.0, P.dtmin);
-            nrej += 1;
-            return false;
-        }
-        // accept (:1466-1478)
-#pragma unroll
-        for (int i = 0; i < S; ++i) liq[i] = liq[i] + dwat[i] * (M_TO_MM * dz[i]);
-        {
-            const int in = (n - 1) / G, kn = (n - 1) % G;
-            double qct = 0.0;
-#pragma unroll
-            for (int i = 0; i < S; ++i) {
-                if (i != in || k != kn) continue;
-                if (lower_bc == CLMSOIL_BC_FLUX) qct = hk[i] + dhkdw[i] * dwat[i];
-                else if (lower_bc == CLMSOIL_BC_ZERO_FLUX) qct = 0.0;
-                else qct = qout[i] + dqodw1[i] * dwat[i];
-            }
-            qct = group_bcast<G>(qct, kn);
-            qc += qct * (dtsub / dtime);
-        }
-        props_valid = false;
-        dtdone += dtsub;                                            // :1480-1483
-        if (fabs(dtime - dtdone) < P.verysmall) return true;
-        if (errorMax < P.xtl) dtsub = dtsub * 2.0;
-        dtsub = jl_min(dtsub, dtime - dtdone);
-        return false;
-    }
-
-    __device__ __forceinline__ void store(const RichardsParams &P, int k) {
-        const int64_t ld = P.ld;
-        const int joff = P.nlevsno;
-        const double dtime = P.dtime;
-#pragma unroll
-        for (int i = 0; i < S; ++i) {
-            const int j = i * G + k + 1;
-            if (j <= n) {
-                P.h2osoi_liq[c + (int64_t)(joff + j - 1) * ld] = liq[i];
-                P.smp_l[c + (int64_t)(j - 1) * ld] = smp[i];
-                P.hk_l[c + (int64_t)(j - 1) * ld] = hk[i];
-            }
-        }
-        // liquid and thickness of layer jwt+1, read from its owner lane's registers
-        double liq_w = 0.0, dz_w = 1.0;
-        if (P.aquifer_pass && jwt < P.nlevsoi && jwt + 1 <= n) {
-            const int jw1 = jwt + 1;
-#pragma unroll
-            for (int i = 0; i < S; ++i)
-                if (i * G + k + 1 == jw1) { liq_w = liq[i]; dz_w = dz[i]; }
-            liq_w = group_bcast<G>(liq_w, (jw1 - 1) % G);
-            dz_w = group_bcast<G>(dz_w, (jw1 - 1) % G);
-        }
-        if (k == 0) {
-            double qcharge = qc;
-            if (P.aquifer_pass && jwt < P.nlevsoi) {
-                // compute_qcharge! with zero smp/imped (:1153-1217 via :1575-1598)
-                const int jw1 = jwt + 1;
-                const int64_t o = c + (int64_t)(jw1 - 1) * ld;
-                double vwc = 0.0;
-                if (jw1 <= n) vwc = jl_max(liq_w, 1.0e-6) / (dz_w * DENH2O);
-                const double wh_zwt = -P.sucsat[o] - zwt * M_TO_MM;
-                double s1 = jl_max(vwc / P.watsat[o], 0.01);
-                s1 = jl_min(1.0, s1);
-                const double ka = 0.0 * P.hksat[o] * pow(s1, 2.0 * P.bsw[o] + 3.0);
-                const double smpmin = P.smpmin ? P.smpmin[c] : -1.0e8;
-                const double smp1 = jl_max(smpmin, 0.0);
-                const int jm = jwt > 1 ? jwt : 1;
-                const double wh = smp1 - P.z[c + (int64_t)(joff + jm - 1) * ld] * M_TO_MM;
-                double q;
-                if (jwt == 0)
-                    q = -ka * (wh_zwt - wh) / ((zwt + 1.0e-3) * M_TO_MM);
-                else
-                    q = -ka * (wh_zwt - wh) /
-                        ((zwt - P.z[c + (int64_t)(joff + jwt - 1) * ld]) * M_TO_MM * 2.0);
-                q = jl_max(-10.0 / dtime, q);
-                q = jl_min(10.0 / dtime, q);
-                qcharge = q;
-            }
-            P.qcharge[c] = qcharge;
-            P.nsubsteps[c] = (double)nsubstep;
-            if (P.nrejected) P.nrejected[c] = (double)nrej;
-        }
-    }
-};
 
 // One column per group for the group's lifetime.
 template <int G>

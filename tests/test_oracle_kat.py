@@ -333,3 +333,18 @@ def test_richards_config0_balance():
         dl = (d["h2osoi_liq"][ns:ns + 20] - liq0[ns:ns + 20]).sum(0)
         bal = (d["qflx_infl"] - d["qflx_rootsoi"].sum(0) - d["qcharge"]) * d["dtime"]
         assert np.abs(dl - bal).max() < 1e-9
+
+
+@pytest.mark.parametrize("kl,ku", [(1, 2), (2, 1)])
+def test_band_solve_unequal_bandwidths(kl, ku):
+    """band_diagonal.jl:96-100 reads A[i, j] for j - kl <= i <= j + ku: ku
+    sub-diagonals and kl super-diagonals. The oracle must solve such systems."""
+    rng = np.random.default_rng(40 + kl)
+    n = 11
+    A = np.zeros((n, n))
+    for d in range(-ku, kl + 1):
+        A += np.diag(rng.normal(size=n - abs(d)), d)
+    x = rng.normal(size=n)
+    bm, rv = band_system([A], [x], kl=kl, ku=ku)
+    tv = band_solve(bm, rv, np.zeros(1, dtype=np.int64), np.full(1, n - 1, dtype=np.int64), kl, ku)
+    np.testing.assert_allclose(tv[:, 0], x, rtol=1e-9, atol=1e-9)

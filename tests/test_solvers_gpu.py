@@ -28,26 +28,33 @@ def test_tridiagonal_multi(gpu):
         assert rel_err(t["u"].cpu().numpy(), u) <= 1e-12
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("kl", [1, 2])
-def test_batched_band_solve(gpu, kl):
-    import torch
-    rng = np.random.default_rng(4 + kl)
+def random_band_columns(rng, nc, n, kl, ku):
+    """Band matrices in the reference's convention: A[i, j] != 0 only for
+    j - kl <= i <= j + ku (ku sub-diagonals, kl super-diagonals)."""
     As, xs = [], []
-    nc, n = 777, 33
     for col in range(nc):
         A = np.zeros((n, n))
-        for d in range(-kl, kl + 1):
+        for d in range(-ku, kl + 1):           # np.diag offset d: A[i, i + d]
             A += np.diag(rng.normal(size=n - abs(d)), d)
         if col % 2:
             A += np.diag(np.full(n, 10.0))     # half the columns diagonally dominant
         As.append(A); xs.append(rng.normal(size=n))
-    bm, rv = band_system(As, xs, nlevs=40, jtop=-3, nlevsno_off=5, kl=kl, ku=kl)
+    return As, xs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kl,ku", [(1, 1), (2, 2), (1, 2), (2, 1)])
+def test_batched_band_solve(gpu, kl, ku):
+    import torch
+    rng = np.random.default_rng(4 + kl + 3 * ku)
+    nc, n = 777, 33
+    As, xs = random_band_columns(rng, nc, n, kl, ku)
+    bm, rv = band_system(As, xs, nlevs=40, jtop=-3, nlevsno_off=5, kl=kl, ku=ku)
     jtop = np.full(nc, -3, dtype=np.int64)
     jbot = np.full(nc, n - 4, dtype=np.int64)
-    want = band_solve(bm, rv, jtop, jbot, kl, kl, nlevsno_off=5)
+    want = band_solve(bm, rv, jtop, jbot, kl, ku, nlevsno_off=5)
     t = api.to_device(dict(bm=bm, rv=rv, jtop=jtop, jbot=jbot, tv=np.full((40, nc), np.nan)))
-    api.batched_band_solve(t["tv"], t["bm"], t["rv"], t["jtop"], t["jbot"], None, kl, kl, 5)
+    api.batched_band_solve(t["tv"], t["bm"], t["rv"], t["jtop"], t["jbot"], None, kl, ku, 5)
     torch.cuda.synchronize()
     got = t["tv"].cpu().numpy()
     assert rel_err(got, want, floor=1e-12) <= 1e-12
