@@ -17,8 +17,11 @@ def declare(lib) -> None:
     lib.clmsoil_get_tuning.restype = I
     lib.clmsoil_set_richards_schedule.argtypes = [I]
     lib.clmsoil_set_richards_schedule.restype = I
+    lib.clmsoil_set_richards_residency.argtypes = [I]
+    lib.clmsoil_set_richards_residency.restype = I
     _declare_learned(lib)
     _declare_adjoint(lib)
+    _declare_day(lib)
 
 
 def _declare_learned(lib) -> None:
@@ -47,3 +50,14 @@ def _declare_adjoint(lib) -> None:
     lib.clmsoil_tridiagonal_multi_vjp.restype = I
     lib.clmsoil_batched_band_solve_vjp.argtypes = [P] * 6 + [I, I, I, L, L, L, P, P, P]
     lib.clmsoil_batched_band_solve_vjp.restype = I
+
+
+def _declare_day(lib) -> None:
+    """include/clmsoil_day.h"""
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    lib.clmsoil_day_plan_create.argtypes = [P, P, P, D, P, P, P, P, P, C.c_int32, C.POINTER(P)]
+    lib.clmsoil_day_plan_create.restype = I
+    lib.clmsoil_day_plan_launch.argtypes = [P, P]
+    lib.clmsoil_day_plan_launch.restype = I
+    lib.clmsoil_day_plan_destroy.argtypes = [P]
+    lib.clmsoil_day_plan_destroy.restype = I

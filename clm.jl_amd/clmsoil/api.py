@@ -238,6 +238,12 @@ def set_richards_schedule(persistent: bool) -> None:
     _abi.check(L, L.clmsoil_set_richards_schedule(1 if persistent else 0), "set_richards_schedule")
 
 
+def set_richards_residency(blocks_per_cu: int) -> None:
+    """Blocks per CU of the resident Richards grid (0 = occupancy calculator)."""
+    L = _lib()
+    _abi.check(L, L.clmsoil_set_richards_residency(int(blocks_per_cu)), "set_richards_residency")
+
+
 def get_tuning() -> tuple:
     L = _lib()
     v = [C.c_int() for _ in range(4)]

@@ -132,7 +132,8 @@ int run_richards(const RichardsParams &P, hipStream_t stream) {
                            NUM_XCD);
         if (hipGetLastError() != hipSuccess) return CLMSOIL_ERR_HIP;
     }
-    return launch_richards(P, g_tuning.richards_g, g_tuning.richards_block, persistent, stream) ==
+    return launch_richards(P, g_tuning.richards_g, g_tuning.richards_block, persistent,
+                           g_tuning.richards_blocks_per_cu, stream) ==
                    hipSuccess
                ? CLMSOIL_OK
                : CLMSOIL_ERR_HIP;
@@ -213,6 +214,12 @@ int clmsoil_set_tuning(int richards_g, int richards_block, int heat_g, int heat_
 int clmsoil_set_richards_schedule(int persistent) {
     if (persistent != 0 && persistent != 1) return CLMSOIL_ERR_ARG;
     g_tuning.richards_persistent = persistent;
+    return CLMSOIL_OK;
+}
+
+int clmsoil_set_richards_residency(int blocks_per_cu) {
+    if (blocks_per_cu < 0 || blocks_per_cu > 32) return CLMSOIL_ERR_ARG;
+    g_tuning.richards_blocks_per_cu = blocks_per_cu;
     return CLMSOIL_OK;
 }
 

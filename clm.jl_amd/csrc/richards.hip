@@ -28,6 +28,8 @@
 // as soon as its column is done (wavefront-aggregated atomics, one counter per
 // XCD over a contiguous column range so neighbouring columns share an L2; idle
 // groups move on to other XCDs' ranges at the end).
+#include <atomic>
+
 #include "rcolumn.hpp"
 
 namespace clmsoil {
@@ -113,16 +115,35 @@ static hipError_t launch_g(const RichardsParams &P, hipStream_t stream, int bloc
     return hipGetLastError();
 }
 
+// Resident blocks per CU of richards_mf_persistent<G> at a block size, from the
+// occupancy calculator, cached per (device, G, block) after the first query.
 template <int G>
-static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int block) {
-    int dev = 0, cus = 0, nb = 0;
+static int resident_blocks(int dev, int block) {
+    static std::atomic<int> cache[16][3];
+    const int bi = block == 64 ? 0 : (block == 128 ? 1 : 2);
+    if (dev < 0 || dev >= 16) dev = 0;
+    int nb = cache[dev][bi].load(std::memory_order_relaxed);
+    if (nb > 0) return nb;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G>, block, 0) !=
+        hipSuccess)
+        return -1;
+    if (nb < 1) nb = 1;
+    cache[dev][bi].store(nb, std::memory_order_relaxed);
+    return nb;
+}
+
+template <int G>
+static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int block,
+                           int blocks_per_cu) {
+    int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return hipErrorUnknown;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return hipErrorUnknown;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G>, block, 0) !=
-        hipSuccess)
-        return hipErrorUnknown;
-    if (nb < 1) nb = 1;
+    int nb = blocks_per_cu;
+    if (nb <= 0) nb = resident_blocks<G>(dev, block);
+    if (nb <= 0) return hipErrorUnknown;
+    // Any grid size is correct: groups only take work from the counters and never
+    // wait on each other, so blocks that start late simply find less work.
     int64_t grid = (int64_t)nb * cus;
     grid = (grid + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     const int64_t need = (P.nc * G + block - 1) / block;
@@ -132,13 +153,13 @@ static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int bloc
 }
 
 hipError_t launch_richards(const RichardsParams &P, int G, int block, bool persistent,
-                           hipStream_t stream) {
+                           int blocks_per_cu, hipStream_t stream) {
     if (P.nc == 0) return hipSuccess;
     if (persistent) {
         switch (G) {
-            case 1: return launch_p<1>(P, stream, block);
-            case 2: return launch_p<2>(P, stream, block);
-            default: return launch_p<4>(P, stream, block);
+            case 1: return launch_p<1>(P, stream, block, blocks_per_cu);
+            case 2: return launch_p<2>(P, stream, block, blocks_per_cu);
+            default: return launch_p<4>(P, stream, block, blocks_per_cu);
         }
     }
     switch (G) {

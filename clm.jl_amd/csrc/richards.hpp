@@ -24,7 +24,9 @@ struct RichardsParams {
 
 // G = lanes per column (1, 2 or 4); block = threads per workgroup; persistent
 // selects the resident-grid launch whose groups refill from work counters.
+// blocks_per_cu sets the persistent grid (0 = as many blocks as the occupancy
+// calculator says fit on a CU).
 hipError_t launch_richards(const RichardsParams &P, int G, int block, bool persistent,
-                           hipStream_t stream);
+                           int blocks_per_cu, hipStream_t stream);
 
 }  // namespace clmsoil

@@ -14,6 +14,7 @@ struct Tuning {
     int heat_g = 4;
     int heat_block = 256;
     int richards_persistent = 1;   // resident grid with per-group column refill
+    int richards_blocks_per_cu = 0;  // persistent grid size; 0 = occupancy query
 };
 extern Tuning g_tuning;
 

@@ -33,6 +33,9 @@ int clmsoil_get_tuning(int *richards_g, int *richards_block, int *heat_g, int *h
  * take a new column from a work counter as soon as their column's substeps are
  * done; 0: one column per lane group for the launch. Results are identical. */
 int clmsoil_set_richards_schedule(int persistent);
+/* Blocks per CU of the resident Richards grid (0, the default: as many as the
+ * occupancy calculator says fit; 1..32 override it). Results are identical. */
+int clmsoil_set_richards_residency(int blocks_per_cu);
 
 #ifdef __cplusplus
 }
