@@ -3,7 +3,7 @@
 // The reference gets this gradient from Enzyme differentiating
 // _soilwm_moisture_form_kernel! (src/biogeophys/soil_water_movement.jl:1250-1489).
 // Here one lane owns one column:
-//   1. replay the primal step with RColumn<1, true>: the same arithmetic as the
+//   1. replay the primal step with RColumn<1, EXP, true>: the same arithmetic as the
 //      primal kernels, so the same accept/reject decisions, recording the start
 //      state and length of every accepted substep to a tape in the workspace;
 //   2. walk the tape backwards. For each accepted substep, recompute its
@@ -41,10 +41,17 @@ struct AdjointParams {
     int64_t tape_ld;
 };
 
+// One column's per-layer inputs (layer j at index j-1).
+struct ColIn {
+    int n;
+    double qflx_infl;
+    double mdz[NMAX], watsat[NMAX], khat[NMAX], den[NMAX], bsw[NMAX], sucsat[NMAX], root[NMAX];
+};
+
 // Reverse of one accepted substep of column `col` from state w (length dt).
 // wbar: in dL/dw after the substep, out dL/dw before it. gb, gk, gw accumulate
 // dL/dbsw, dL/dkhat (khat = impedance * hksat), dL/dwatsat per layer.
-__device__ void substep_vjp(const RColumn<1, true> &col, const RichardsParams &P,
+__device__ void substep_vjp(const ColIn &col, const RichardsParams &P,
                             const double *w, double dt, double qbar, double *wbar,
                             double *gb, double *gk, double *gw) {
     const int n = col.n;
@@ -55,7 +62,7 @@ __device__ void substep_vjp(const RColumn<1, true> &col, const RichardsParams &P
     bool act2[NMAX], act1[NMAX], wpos[NMAX];
     // ---- forward recompute ----
     for (int j = 0; j < n; ++j) {
-        const double vwc = jl_max(w[j], 1.0e-6) / (col.dz[j] * DENH2O);
+        const double vwc = jl_max(w[j], 1.0e-6) / col.mdz[j];   // dz * denh2o
         wpos[j] = w[j] > 1.0e-6;
         sr[j] = vwc / col.watsat[j];
         const double m = jl_min(sr[j], 1.0);
@@ -93,7 +100,7 @@ __device__ void substep_vjp(const RColumn<1, true> &col, const RichardsParams &P
         }
     }
     for (int j = 0; j < n; ++j) {
-        tz[j] = dt / (M_TO_MM * col.dz[j]);
+        tz[j] = dt / col.mdz[j];
         const double qin = j == 0 ? col.qflx_infl : qo[j - 1];
         const double di0 = j == 0 ? 0.0 : d1[j - 1];
         const double di1 = j == 0 ? 0.0 : d2[j - 1];
@@ -118,7 +125,7 @@ __device__ void substep_vjp(const RColumn<1, true> &col, const RichardsParams &P
     double lam[NMAX], hkb[NMAX], dhkb[NMAX], smpb[NMAX], dsmpb[NMAX], qob[NMAX], d1b[NMAX],
         d2b[NMAX], s2b[NMAX];
     for (int j = 0; j < n; ++j) {
-        lam[j] = wbar[j] * (M_TO_MM * col.dz[j]);   // xbar
+        lam[j] = wbar[j] * col.mdz[j];   // xbar
         hkb[j] = dhkb[j] = smpb[j] = dsmpb[j] = qob[j] = d1b[j] = d2b[j] = s2b[j] = 0.0;
     }
     if (flux_bc) {   // qct = hk[n] + dhkdw[n] * dwat[n] (:1471-1478)
@@ -205,15 +212,16 @@ __device__ void substep_vjp(const RColumn<1, true> &col, const RichardsParams &P
     for (int j = 0; j < n; ++j) {
         const double srb = act2[j] ? s2b[j] : 0.0;
         gw[j] -= srb * sr[j] / col.watsat[j];
-        if (wpos[j]) wbar[j] += (srb / col.watsat[j]) / (col.dz[j] * DENH2O);
+        if (wpos[j]) wbar[j] += (srb / col.watsat[j]) / col.mdz[j];
     }
 }
 
+template <bool EXP>
 __global__ __launch_bounds__(64) void richards_adjoint_kernel(RichardsParams P, AdjointParams A) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= P.nc) return;
     if (P.mask && !P.mask[c]) return;
-    RColumn<1, true> col;
+    RColumn<1, EXP, true> col;
     col.tape = A.tape;
     col.tape_ld = A.tape_ld;
     col.kmax = KMAX;
@@ -230,6 +238,19 @@ __global__ __launch_bounds__(64) void richards_adjoint_kernel(RichardsParams P, 
     const int n = col.n;
     const int64_t ld = P.ld;
     const int joff = P.nlevsno;
+    ColIn in;
+    in.n = n;
+    in.qflx_infl = col.qflx_infl;
+    for (int j = 0; j < n; ++j) {
+        const int64_t o = c + (int64_t)j * ld;
+        in.mdz[j] = col.mdz[j];
+        in.watsat[j] = col.watsat[j];
+        in.khat[j] = col.khat[j];
+        in.den[j] = col.den[j];
+        in.bsw[j] = P.bsw[o];
+        in.sucsat[j] = P.sucsat[o];
+        in.root[j] = P.qflx_rootsoi[o];
+    }
     double wbar[NMAX], gb[NMAX], gk[NMAX], gw[NMAX], w[NMAX];
     for (int j = 0; j < n; ++j) {
         wbar[j] = A.liq_bar[c + (int64_t)(joff + j) * ld];
@@ -240,20 +261,20 @@ __global__ __launch_bounds__(64) void richards_adjoint_kernel(RichardsParams P, 
         const int64_t row = (int64_t)k * (NMAX + 1);
         for (int j = 0; j < n; ++j) w[j] = A.tape[(row + j) * A.tape_ld + c];
         const double dt = A.tape[(row + NMAX) * A.tape_ld + c];
-        substep_vjp(col, P, w, dt, qbar, wbar, gb, gk, gw);
+        substep_vjp(in, P, w, dt, qbar, wbar, gb, gk, gw);
     }
     double sb = 0.0, sk = 0.0, sw = 0.0;
     for (int j = 0; j < n; ++j) {
         A.liq_bar[c + (int64_t)(joff + j) * ld] = wbar[j];
         const int64_t o = c + (int64_t)j * ld;
         const double hks = P.hksat[o];
-        const double gh = gk[j] * (col.khat[j] / hks);   // khat = impedance * hksat
+        const double gh = gk[j] * (in.khat[j] / hks);   // khat = impedance * hksat
         if (A.g_bsw) A.g_bsw[o] += gb[j];
         if (A.g_hksat) A.g_hksat[o] += gh;
         if (A.g_watsat) A.g_watsat[o] += gw[j];
-        sb += gb[j] * col.bsw[j];
-        sk += gk[j] * col.khat[j];
-        sw += gw[j] * col.watsat[j];
+        sb += gb[j] * in.bsw[j];
+        sk += gk[j] * in.khat[j];
+        sw += gw[j] * in.watsat[j];
     }
     if (A.g_scale) {
         A.g_scale[c] += sb;
@@ -310,8 +331,12 @@ int clmsoil_soilwm_moisture_form_adjoint(const clmsoil_mf_args *a, const clmsoil
     AdjointParams A{liq_bar, qcharge_bar, grad_bsw, grad_hksat, grad_watsat, grad_scale,
                     static_cast<double *>(workspace), a->ld};
     const int64_t grid = (a->nc + 63) / 64;
-    hipLaunchKernelGGL(richards_adjoint_kernel, dim3((unsigned)grid), dim3(64), 0,
-                       (hipStream_t)stream, P, A);
+    if (P.expensive)
+        hipLaunchKernelGGL(richards_adjoint_kernel<true>, dim3((unsigned)grid), dim3(64), 0,
+                           (hipStream_t)stream, P, A);
+    else
+        hipLaunchKernelGGL(richards_adjoint_kernel<false>, dim3((unsigned)grid), dim3(64), 0,
+                           (hipStream_t)stream, P, A);
     return hipGetLastError() == hipSuccess ? CLMSOIL_OK : CLMSOIL_ERR_HIP;
 }
 

@@ -18,7 +18,7 @@ constexpr double THIN_SFCLAYER = 1.0e-6;   // soil_temperature.jl:26
 
 // Minimum waves per SIMD the fused kernels are compiled for (register cap).
 #ifndef CLMSOIL_RICHARDS_WAVES
-#define CLMSOIL_RICHARDS_WAVES 1
+#define CLMSOIL_RICHARDS_WAVES 2
 #endif
 #ifndef CLMSOIL_HEAT_WAVES
 #define CLMSOIL_HEAT_WAVES 1

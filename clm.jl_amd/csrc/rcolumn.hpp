@@ -2,8 +2,22 @@
 // G lanes (layer j in lane (j-1) mod G, register slot (j-1) div G), shared by the
 // primal kernels and the adjoint's replay of the primal step.
 //
-// REC = true additionally records every accepted substep (its start state and
-// length) to a tape [k][j][c] for the adjoint (k < kmax; kacc counts them all).
+// Register budget. Across substeps a lane keeps, per slot, only what the substep
+// loop re-reads on every iteration: the water state, 1000*dz, the impedance-scaled
+// conductivity khat, the node-distance denominator den, porosity, and the four
+// flux terms of the linear system (qin - qout - root, dqidw0, -dqidw1 + dqodw1,
+// dqodw2). bsw, sucsat and the root sink are read again from memory when the
+// hydraulic properties are recomputed (after an accepted substep), and smp/hk are
+// written through to smp_l/hk_l at that moment instead of being held. The row
+// coefficients of the tridiagonal system are formed for all rows in parallel
+// before the serial sweep, which then carries only (bet, dwat) between rows.
+// Every value is computed with the reference's operation order, so the results
+// are the same bits as a straightforward restatement.
+//
+// EXP = true keeps the separate flux terms the "expensive" error estimate needs
+// (flux_calculation = 42). REC = true records every accepted substep (its start
+// state and length) to a tape [k][j][c] for the adjoint (k < kmax; kacc counts
+// them all).
 #pragma once
 #include "common.hpp"
 #include "richards.hpp"
@@ -24,27 +38,34 @@ __device__ __forceinline__ double prv(const double *x, int i, int k) {
     return (k == 0) ? b : a;
 }
 
-// One column's state while a group works on it (all in registers).
-template <int G, bool REC = false>
+template <int G, bool EXP = false, bool REC = false>
 struct RColumn {
     static constexpr int NMAX = MAX_NLEVSOI;
     static constexpr int S = (NMAX + G - 1) / G;
+    static constexpr int SE = EXP ? S : 1;
     // tape of accepted substeps (REC only): tape[(k*(NMAX+1) + j-1)*tape_ld + c] is
     // layer j's liquid water at the start of accepted substep k; row NMAX its length
     double *tape = nullptr;
     int64_t tape_ld = 0;
     int kacc = 0, kmax = 0;
+
     int64_t c;
     int n, jwt, nsubstep, nrej, iter;
     bool props_valid;
     double zwt, qflx_infl, dtsub, dtdone, qc;
-    double liq[S], dz[S], watsat[S], khat[S], bsw[S], sucsat[S], root[S], den[S];
-    double hk[S], smp[S], dhkdw[S], qin[S], qout[S], dqidw0[S], dqidw1[S], dqodw1[S], dqodw2[S];
-    double gam[S], dwat[S];
+    // bottom-layer drainage qct = qn_a + qn_b * dwat[n] (owner lane of layer n)
+    double qn_a, qn_b;
+    double liq[S], mdz[S], khat[S], den[S], watsat[S];
+    double fnet[S], a0[S], sb[S], c2[S];
+    double qin[SE], qout[SE], di1[SE], do1[SE];
+
+    __device__ __forceinline__ int64_t soil(int j) const { return c + (int64_t)(j - 1) * ld_; }
+    int64_t ld_;
 
     // Load column `col`; returns false (and flags the error word) for a bad nbedrock.
     __device__ __forceinline__ bool load(const RichardsParams &P, int64_t col, int k) {
         c = col;
+        ld_ = P.ld;
         n = (int)P.nbedrock[c];
         if (n < 1 || n > P.nlevsoi) {
             if (k == 0) atomicOr(P.err, (unsigned)DEV_ERR_NBEDROCK);
@@ -64,14 +85,11 @@ struct RColumn {
         for (int i = 0; i < S; ++i) {
             const int j = i * G + k + 1;
             if (j <= n) {
-                const int64_t o = c + (int64_t)(j - 1) * ld;           // soil-array offset
-                const int64_t os = c + (int64_t)(joff + j - 1) * ld;   // snow+soil offset
+                const int64_t o = soil(j);                               // soil-array offset
+                const int64_t os = c + (int64_t)(joff + j - 1) * ld;    // snow+soil offset
                 liq[i] = P.h2osoi_liq[os];
-                dz[i] = __builtin_nontemporal_load(&P.dz[os]);
+                mdz[i] = M_TO_MM * __builtin_nontemporal_load(&P.dz[os]);
                 watsat[i] = __builtin_nontemporal_load(&P.watsat[o]);
-                bsw[i] = __builtin_nontemporal_load(&P.bsw[o]);
-                sucsat[i] = __builtin_nontemporal_load(&P.sucsat[o]);
-                root[i] = __builtin_nontemporal_load(&P.qflx_rootsoi[o]);
                 const double hks = __builtin_nontemporal_load(&P.hksat[o]);
                 const double icf = __builtin_nontemporal_load(&P.icefrac[o]);
                 const double zj = __builtin_nontemporal_load(&P.z[os]);
@@ -86,12 +104,12 @@ struct RColumn {
                 }
                 khat[i] = imp * hks;   // (imp * hksat) * s1^(2b+3), the reference's grouping
             } else {
-                liq[i] = 0.0; dz[i] = 1.0; watsat[i] = 1.0; khat[i] = 0.0; bsw[i] = 1.0;
-                sucsat[i] = 0.0; root[i] = 0.0; den[i] = 1.0;
+                liq[i] = 0.0; mdz[i] = M_TO_MM; watsat[i] = 1.0; khat[i] = 0.0; den[i] = 1.0;
             }
-            hk[i] = smp[i] = dhkdw[i] = qin[i] = qout[i] = 0.0;
-            dqidw0[i] = dqidw1[i] = dqodw1[i] = dqodw2[i] = gam[i] = dwat[i] = 0.0;
+            fnet[i] = a0[i] = sb[i] = c2[i] = 0.0;
+            if constexpr (EXP) qin[i] = qout[i] = di1[i] = do1[i] = 0.0;
         }
+        qn_a = qn_b = 0.0;
         qflx_infl = P.qflx_infl[c];
         nsubstep = 0;
         nrej = 0;
@@ -108,6 +126,109 @@ struct RColumn {
         return true;
     }
 
+    // Hydraulic properties, interface fluxes and the flux terms of the system
+    // for the current water state (:1273-1378).
+    __device__ __forceinline__ void properties(const RichardsParams &P, int k) {
+        const int lower_bc = P.lower_bc;
+        // node saturation s2 (:1273-1286)
+        double s2[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            const double vwc = jl_max(liq[i], 1.0e-6) / mdz[i];   // dz * denh2o
+            double s = vwc / watsat[i];
+            s = jl_min(s, 1.0);
+            s = jl_max(0.01, s);
+            s2[i] = s;
+        }
+        // node suction and its derivative (:1300-1309)
+        double smp[S], dsmpdw[S], bsw[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            const int j = i * G + k + 1;
+            double b = 1.0, suc = 0.0;
+            if (j <= n) {
+                b = P.bsw[soil(j)];
+                suc = P.sucsat[soil(j)];
+            }
+            bsw[i] = b;
+            const double sn = s2[i];
+            const double smpv = -suc * pow(sn, -b);
+            const double dsmpds = -b * smpv / sn;
+            smp[i] = smpv;
+            dsmpdw[i] = dsmpds / watsat[i];
+            if (j <= n && P.smp_l) P.smp_l[soil(j)] = smpv;
+        }
+        // interface conductivity (:1287-1299) and fluxes (:1334-1367)
+        double qo[S], d1[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            const int j = i * G + k + 1;
+            const double s2n = nxt<G>(s2, i, k, S);
+            double s1 = (j == n) ? s2[i] : 0.5 * (s2[i] + s2n);
+            s1 = jl_min(s1, 1.0);
+            s1 = jl_max(0.01, s1);
+            const double b = bsw[i];
+            const double hkv = khat[i] * pow(s1, 2.0 * b + 3.0);
+            const double dhkdw = (2.0 * b + 3.0) * hkv / s1;
+            if (j <= n && P.hk_l) P.hk_l[soil(j)] = hkv;
+            const double smpn = nxt<G>(smp, i, k, S);
+            const double dsmpdwn = nxt<G>(dsmpdw, i, k, S);
+            const double watsatn = nxt<G>(watsat, i, k, S);
+            double q = 0.0, e1 = 0.0, e2 = 0.0;
+            if (j < n) {
+                const double dhkds1 = 0.5 * dhkdw / watsat[i];
+                const double dhkds2 = 0.5 * dhkdw / watsatn;
+                const double num = smpn - smp[i];
+                const double dn = den[i];
+                q = -hkv * num / dn + hkv;
+                e1 = (hkv * dsmpdw[i] - dhkds1 * num) / dn + dhkds1;
+                e2 = (-hkv * dsmpdwn - dhkds2 * num) / dn + dhkds2;
+            } else if (j == n) {
+                if (lower_bc == CLMSOIL_BC_WATERTABLE) {
+                    if (!(j > jwt)) {
+                        const double dhkds1l = dhkdw / watsat[i];
+                        const double num = -smp[i];
+                        const double dn = den[i];
+                        q = -hkv * num / dn + hkv;
+                        e1 = (hkv * dsmpdw[i] - dhkds1l * num) / dn + dhkds1l;
+                    }
+                    qn_a = q;          // qcharge = qout + dqodw1 * dwat (:1476-1478)
+                    qn_b = e1;
+                } else if (lower_bc == CLMSOIL_BC_FLUX) {
+                    q = hkv;
+                    e1 = dhkdw / watsat[i];
+                    qn_a = hkv;        // qcharge = hk + dhkdw * dwat (:1471-1472)
+                    qn_b = dhkdw;
+                }
+            }
+            qo[i] = q;
+            d1[i] = e1;
+            c2[i] = e2;
+        }
+        // inflow at the top of each layer (:1371-1378) and the rows' flux terms
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            const int j = i * G + k + 1;
+            const double po = prv<G>(qo, i, k);
+            const double p1 = prv<G>(d1, i, k);
+            const double p2 = prv<G>(c2, i, k);
+            const double qi = (j == 1) ? qflx_infl : po;
+            const double dqidw0 = (j == 1) ? 0.0 : p1;
+            const double dqidw1 = (j == 1) ? 0.0 : p2;
+            const double root = (j <= n) ? P.qflx_rootsoi[soil(j)] : 0.0;
+            fnet[i] = qi - qo[i] - root;
+            a0[i] = dqidw0;
+            sb[i] = -dqidw1 + d1[i];
+            if constexpr (EXP) {
+                qin[i] = qi;
+                qout[i] = qo[i];
+                di1[i] = dqidw1;
+                do1[i] = d1[i];
+            }
+        }
+        props_valid = true;
+    }
+
     // One pass of the reference's substep loop; returns true when the column's
     // step is complete.
     __device__ __forceinline__ bool iterate(const RichardsParams &P, int k) {
@@ -119,117 +240,51 @@ struct RColumn {
         }
         ++iter;
         nsubstep += 1;
-        if (!props_valid) {
-            // node saturation s2 (:1273-1286)
-            double s2[S];
-#pragma unroll
-            for (int i = 0; i < S; ++i) {
-                const double vwc = jl_max(liq[i], 1.0e-6) / (dz[i] * DENH2O);
-                double s = vwc / watsat[i];
-                s = jl_min(s, 1.0);
-                s = jl_max(0.01, s);
-                s2[i] = s;
-            }
-            double dsmpdw[S];
-            // Clapp-Hornberger at interface (hk) and node (smp) (:1287-1309)
-#pragma unroll
-            for (int i = 0; i < S; ++i) {
-                const int j = i * G + k + 1;
-                const double s2n = nxt<G>(s2, i, k, S);
-                double s1 = (j == n) ? s2[i] : 0.5 * (s2[i] + s2n);
-                s1 = jl_min(s1, 1.0);
-                s1 = jl_max(0.01, s1);
-                const double b = bsw[i];
-                const double hkv = khat[i] * pow(s1, 2.0 * b + 3.0);
-                const double dhkds = (2.0 * b + 3.0) * hkv / s1;
-                const double sn = s2[i];
-                const double smpv = -sucsat[i] * pow(sn, -b);
-                const double dsmpds = -b * smpv / sn;
-                hk[i] = hkv;
-                smp[i] = smpv;
-                dhkdw[i] = dhkds;
-                dsmpdw[i] = dsmpds / watsat[i];
-            }
-            // interface fluxes and derivatives (:1334-1367)
-#pragma unroll
-            for (int i = 0; i < S; ++i) {
-                const int j = i * G + k + 1;
-                const double smpn = nxt<G>(smp, i, k, S);
-                const double dsmpdwn = nxt<G>(dsmpdw, i, k, S);
-                const double watsatn = nxt<G>(watsat, i, k, S);
-                double qo = 0.0, d1 = 0.0, d2 = 0.0;
-                if (j < n) {
-                    const double dhkds1 = 0.5 * dhkdw[i] / watsat[i];
-                    const double dhkds2 = 0.5 * dhkdw[i] / watsatn;
-                    const double num = smpn - smp[i];
-                    const double dn = den[i];
-                    qo = -hk[i] * num / dn + hk[i];
-                    d1 = (hk[i] * dsmpdw[i] - dhkds1 * num) / dn + dhkds1;
-                    d2 = (-hk[i] * dsmpdwn - dhkds2 * num) / dn + dhkds2;
-                } else if (j == n) {
-                    if (lower_bc == CLMSOIL_BC_WATERTABLE) {
-                        if (!(j > jwt)) {
-                            const double dhkds1l = dhkdw[i] / watsat[i];
-                            const double num = -smp[i];
-                            const double dn = den[i];
-                            qo = -hk[i] * num / dn + hk[i];
-                            d1 = (hk[i] * dsmpdw[i] - dhkds1l * num) / dn + dhkds1l;
-                        }
-                    } else if (lower_bc == CLMSOIL_BC_FLUX) {
-                        qo = hk[i];
-                        d1 = dhkdw[i] / watsat[i];
-                    }
-                }
-                qout[i] = qo;
-                dqodw1[i] = d1;
-                dqodw2[i] = d2;
-            }
-            // inflow at the top of each layer (:1371-1378)
-#pragma unroll
-            for (int i = 0; i < S; ++i) {
-                const int j = i * G + k + 1;
-                const double po = prv<G>(qout, i, k);
-                const double p1 = prv<G>(dqodw1, i, k);
-                const double p2 = prv<G>(dqodw2, i, k);
-                qin[i] = (j == 1) ? qflx_infl : po;
-                dqidw0[i] = (j == 1) ? 0.0 : p1;
-                dqidw1[i] = (j == 1) ? 0.0 : p2;
-            }
-            props_valid = true;
-        }
+        if (!props_valid) properties(P, k);
 
-        // ---- RHS/LHS and Thomas forward sweep (:1381-1431) ----
-        double bet = 0.0, dwp = 0.0, cmxp = 0.0;
+        // ---- RHS/LHS of every row (:1381-1397), in parallel ----
+        double amx[S], bmx[S], cmx[S], rmx[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            const int j = i * G + k + 1;
+            const double dt_dz = dtsub / mdz[i];
+            rmx[i] = -fnet[i] * dt_dz;
+            amx[i] = (j == 1) ? 0.0 : a0[i] * dt_dz;
+            bmx[i] = -1.0 - sb[i] * dt_dz;
+            cmx[i] = (j == n) ? 0.0 : -c2[i] * dt_dz;
+        }
+        double cpv[S];   // the previous row's cmx
+#pragma unroll
+        for (int i = 0; i < S; ++i) cpv[i] = prv<G>(cmx, i, k);
+
+        // ---- Thomas forward sweep (:1421-1431) ----
+        double gam[S], dwat[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) gam[i] = dwat[i] = 0.0;
+        double bet = 0.0, dwp = 0.0;
 #pragma unroll
         for (int j = 1; j <= NMAX; ++j) {
             const int i = (j - 1) / G, kk = (j - 1) % G;
             if (j > n) break;
-            double ob = 0.0, od = 0.0, oc = 0.0;
+            double ob = 0.0, od = 0.0;
             if (k == kk) {
-                const double dt_dz = dtsub / (M_TO_MM * dz[i]);
-                const double fluxNet = qin[i] - qout[i] - root[i];
-                const double rmx = -fluxNet * dt_dz;
-                const double amx = (j == 1) ? 0.0 : dqidw0[i] * dt_dz;
-                const double bmx = -1.0 - (-dqidw1[i] + dqodw1[i]) * dt_dz;
-                const double cmx = (j == n) ? 0.0 : -dqodw2[i] * dt_dz;
                 double b2, d;
                 if (j == 1) {
-                    b2 = bmx;
+                    b2 = bmx[i];
                     if (fabs(b2) < 1.0e-30) b2 = 1.0e-30;
-                    d = rmx / b2;
+                    d = rmx[i] / b2;
                 } else {
-                    const double g = cmxp / bet;
+                    const double g = cpv[i] / bet;
                     gam[i] = g;
-                    b2 = bmx - amx * g;
+                    b2 = bmx[i] - amx[i] * g;
                     if (fabs(b2) < 1.0e-30) b2 = copysign(1.0e-30, b2 == 0.0 ? 1.0 : b2);
-                    d = (rmx - amx * dwp) / b2;
+                    d = (rmx[i] - amx[i] * dwp) / b2;
                 }
                 dwat[i] = d;
-                ob = b2; od = d; oc = cmx;
+                ob = b2; od = d;
             }
             bet = group_bcast<G>(ob, kk);
             dwp = group_bcast<G>(od, kk);
-            cmxp = group_bcast<G>(oc, kk);
         }
         // ---- back substitution (:1434-1438) ----
         double gn = 0.0, dn = 0.0;
@@ -252,19 +307,22 @@ struct RColumn {
         for (int i = 0; i < S; ++i) {
             const int j = i * G + k + 1;
             double fluxNet0;
-            const double dt_dz = dtsub / (M_TO_MM * dz[i]);
-            if (P.expensive) {
+            const double dt_dz = dtsub / mdz[i];
+            double fluxNet1;
+            if constexpr (EXP) {
+                const double root = (j <= n) ? P.qflx_rootsoi[soil(j)] : 0.0;
                 const double dwm = prv<G>(dwat, i, k);
                 const double dwn = nxt<G>(dwat, i, k, S);
-                const double qin_test = (j == 1) ? qin[i] + dqidw1[i] * dwat[i]
-                                                 : qin[i] + dqidw0[i] * dwm + dqidw1[i] * dwat[i];
-                const double qout_test = (j == n) ? qout[i] + dqodw1[i] * dwat[i]
-                                                  : qout[i] + dqodw1[i] * dwat[i] + dqodw2[i] * dwn;
-                fluxNet0 = qin_test - qout_test - root[i];
+                const double qin_test = (j == 1) ? qin[i] + di1[i] * dwat[i]
+                                                 : qin[i] + a0[i] * dwm + di1[i] * dwat[i];
+                const double qout_test = (j == n) ? qout[i] + do1[i] * dwat[i]
+                                                  : qout[i] + do1[i] * dwat[i] + c2[i] * dwn;
+                fluxNet0 = qin_test - qout_test - root;
+                fluxNet1 = qin[i] - qout[i] - root;
             } else {
                 fluxNet0 = dwat[i] / dt_dz;
+                fluxNet1 = fnet[i];
             }
-            const double fluxNet1 = qin[i] - qout[i] - root[i];
             const double ev = fabs(fluxNet1 - fluxNet0) * dtsub * 0.5;
             if (j <= n) errorMax = jl_max(errorMax, ev);
         }
@@ -289,16 +347,14 @@ struct RColumn {
             ++kacc;
         }
 #pragma unroll
-        for (int i = 0; i < S; ++i) liq[i] = liq[i] + dwat[i] * (M_TO_MM * dz[i]);
+        for (int i = 0; i < S; ++i) liq[i] = liq[i] + dwat[i] * mdz[i];
         {
             const int in = (n - 1) / G, kn = (n - 1) % G;
             double qct = 0.0;
 #pragma unroll
             for (int i = 0; i < S; ++i) {
                 if (i != in || k != kn) continue;
-                if (lower_bc == CLMSOIL_BC_FLUX) qct = hk[i] + dhkdw[i] * dwat[i];
-                else if (lower_bc == CLMSOIL_BC_ZERO_FLUX) qct = 0.0;
-                else qct = qout[i] + dqodw1[i] * dwat[i];
+                if (lower_bc != CLMSOIL_BC_ZERO_FLUX) qct = qn_a + qn_b * dwat[i];
             }
             qct = group_bcast<G>(qct, kn);
             qc += qct * (dtsub / dtime);
@@ -318,21 +374,17 @@ struct RColumn {
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             const int j = i * G + k + 1;
-            if (j <= n) {
-                P.h2osoi_liq[c + (int64_t)(joff + j - 1) * ld] = liq[i];
-                P.smp_l[c + (int64_t)(j - 1) * ld] = smp[i];
-                P.hk_l[c + (int64_t)(j - 1) * ld] = hk[i];
-            }
+            if (j <= n) P.h2osoi_liq[c + (int64_t)(joff + j - 1) * ld] = liq[i];
         }
         // liquid and thickness of layer jwt+1, read from its owner lane's registers
-        double liq_w = 0.0, dz_w = 1.0;
+        double liq_w = 0.0, mdz_w = M_TO_MM;
         if (P.aquifer_pass && jwt < P.nlevsoi && jwt + 1 <= n) {
             const int jw1 = jwt + 1;
 #pragma unroll
             for (int i = 0; i < S; ++i)
-                if (i * G + k + 1 == jw1) { liq_w = liq[i]; dz_w = dz[i]; }
+                if (i * G + k + 1 == jw1) { liq_w = liq[i]; mdz_w = mdz[i]; }
             liq_w = group_bcast<G>(liq_w, (jw1 - 1) % G);
-            dz_w = group_bcast<G>(dz_w, (jw1 - 1) % G);
+            mdz_w = group_bcast<G>(mdz_w, (jw1 - 1) % G);
         }
         if (k == 0) {
             double qcharge = qc;
@@ -341,7 +393,7 @@ struct RColumn {
                 const int jw1 = jwt + 1;
                 const int64_t o = c + (int64_t)(jw1 - 1) * ld;
                 double vwc = 0.0;
-                if (jw1 <= n) vwc = jl_max(liq_w, 1.0e-6) / (dz_w * DENH2O);
+                if (jw1 <= n) vwc = jl_max(liq_w, 1.0e-6) / mdz_w;   // dz * denh2o
                 const double wh_zwt = -P.sucsat[o] - zwt * M_TO_MM;
                 double s1 = jl_max(vwc / P.watsat[o], 0.01);
                 s1 = jl_min(1.0, s1);
@@ -366,6 +418,5 @@ struct RColumn {
         }
     }
 };
-
 
 }  // namespace clmsoil

@@ -35,14 +35,14 @@
 namespace clmsoil {
 
 // One column per group for the group's lifetime.
-template <int G>
+template <int G, bool EXP>
 __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kernel(RichardsParams P) {
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t c = gtid / G;
     const int k = (int)(threadIdx.x % G);
     if (c >= P.nc) return;
     if (P.mask && !P.mask[c]) return;
-    RColumn<G> col;
+    RColumn<G, EXP> col;
     if (!col.load(P, c, k)) return;
     while (!col.iterate(P, k)) {
     }
@@ -51,14 +51,14 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kerne
 
 // Persistent: a resident grid whose groups refill from per-XCD column ranges.
 // Every wavefront leaves the loop once all eight ranges are exhausted.
-template <int G>
+template <int G, bool EXP>
 __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persistent(RichardsParams P) {
     const int k = (int)(threadIdx.x % G);
     const int lane = (int)__lane_id();
     const int xcd = (int)(blockIdx.x % NUM_XCD);
     const int64_t chunk = (P.nc + NUM_XCD - 1) / NUM_XCD;
     unsigned *ctrs = P.work_counters;
-    RColumn<G> col;
+    RColumn<G, EXP> col;
     bool active = false;   // group holds a column
     bool done = false;     // no work left for this group
     int steal = 0;         // XCD ranges exhausted so far (wavefront-uniform)
@@ -107,24 +107,25 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persi
     }
 }
 
-template <int G>
+template <int G, bool EXP>
 static hipError_t launch_g(const RichardsParams &P, hipStream_t stream, int block) {
     const int64_t threads = P.nc * G;
     const int64_t grid = (threads + block - 1) / block;
-    hipLaunchKernelGGL(richards_mf_kernel<G>, dim3((unsigned)grid), dim3(block), 0, stream, P);
+    hipLaunchKernelGGL((richards_mf_kernel<G, EXP>), dim3((unsigned)grid), dim3(block), 0, stream,
+                       P);
     return hipGetLastError();
 }
 
 // Resident blocks per CU of richards_mf_persistent<G> at a block size, from the
 // occupancy calculator, cached per (device, G, block) after the first query.
-template <int G>
+template <int G, bool EXP>
 static int resident_blocks(int dev, int block) {
     static std::atomic<int> cache[16][3];
     const int bi = block == 64 ? 0 : (block == 128 ? 1 : 2);
     if (dev < 0 || dev >= 16) dev = 0;
     int nb = cache[dev][bi].load(std::memory_order_relaxed);
     if (nb > 0) return nb;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G>, block, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G, EXP>, block, 0) !=
         hipSuccess)
         return -1;
     if (nb < 1) nb = 1;
@@ -132,7 +133,7 @@ static int resident_blocks(int dev, int block) {
     return nb;
 }
 
-template <int G>
+template <int G, bool EXP>
 static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int block,
                            int blocks_per_cu) {
     int dev = 0, cus = 0;
@@ -140,7 +141,7 @@ static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int bloc
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return hipErrorUnknown;
     int nb = blocks_per_cu;
-    if (nb <= 0) nb = resident_blocks<G>(dev, block);
+    if (nb <= 0) nb = resident_blocks<G, EXP>(dev, block);
     if (nb <= 0) return hipErrorUnknown;
     // Any grid size is correct: groups only take work from the counters and never
     // wait on each other, so blocks that start late simply find less work.
@@ -148,25 +149,33 @@ static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int bloc
     grid = (grid + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     const int64_t need = (P.nc * G + block - 1) / block;
     if (grid > need) grid = (need + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
-    hipLaunchKernelGGL(richards_mf_persistent<G>, dim3((unsigned)grid), dim3(block), 0, stream, P);
+    hipLaunchKernelGGL((richards_mf_persistent<G, EXP>), dim3((unsigned)grid), dim3(block), 0,
+                       stream, P);
     return hipGetLastError();
+}
+
+template <bool EXP>
+static hipError_t launch_exp(const RichardsParams &P, int G, int block, bool persistent,
+                             int blocks_per_cu, hipStream_t stream) {
+    if (persistent) {
+        switch (G) {
+            case 1: return launch_p<1, EXP>(P, stream, block, blocks_per_cu);
+            case 2: return launch_p<2, EXP>(P, stream, block, blocks_per_cu);
+            default: return launch_p<4, EXP>(P, stream, block, blocks_per_cu);
+        }
+    }
+    switch (G) {
+        case 1: return launch_g<1, EXP>(P, stream, block);
+        case 2: return launch_g<2, EXP>(P, stream, block);
+        default: return launch_g<4, EXP>(P, stream, block);
+    }
 }
 
 hipError_t launch_richards(const RichardsParams &P, int G, int block, bool persistent,
                            int blocks_per_cu, hipStream_t stream) {
     if (P.nc == 0) return hipSuccess;
-    if (persistent) {
-        switch (G) {
-            case 1: return launch_p<1>(P, stream, block, blocks_per_cu);
-            case 2: return launch_p<2>(P, stream, block, blocks_per_cu);
-            default: return launch_p<4>(P, stream, block, blocks_per_cu);
-        }
-    }
-    switch (G) {
-        case 1: return launch_g<1>(P, stream, block);
-        case 2: return launch_g<2>(P, stream, block);
-        default: return launch_g<4>(P, stream, block);
-    }
+    return P.expensive ? launch_exp<true>(P, G, block, persistent, blocks_per_cu, stream)
+                       : launch_exp<false>(P, G, block, persistent, blocks_per_cu, stream);
 }
 
 }  // namespace clmsoil
