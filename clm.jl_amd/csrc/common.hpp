@@ -21,7 +21,7 @@ constexpr double THIN_SFCLAYER = 1.0e-6;   // soil_temperature.jl:26
 #define CLMSOIL_RICHARDS_WAVES 2
 #endif
 #ifndef CLMSOIL_HEAT_WAVES
-#define CLMSOIL_HEAT_WAVES 1
+#define CLMSOIL_HEAT_WAVES 2
 #endif
 
 // MI355X (gfx950) has 8 accelerator complex dies (XCDs); workgroups are

@@ -25,9 +25,18 @@
 // no row interchange occurs the dense elimination performs exactly the
 // operations of the banded sweep below (every other update subtracts f*0). The
 // sweep checks the reference's pivot test at every step; a column that would
-// interchange rows is finished by the pivoting pass (dense elimination as in the
-// reference) instead. Elimination and back-substitution carries move between the
-// lanes of a group by DPP broadcast.
+// interchange rows is re-solved by a banded partially pivoted elimination in its
+// wavefront's LDS area. Elimination and back-substitution carries move between
+// the lanes of a group by DPP broadcast.
+//
+// Registers: the kernel is compiled for two wavefronts per SIMD. Inputs are read
+// level by level and turned into conductivity and heat capacity at once; the
+// inputs needed again after the solve (water, ice, excess ice, dz, porosity, b,
+// psi_sat, z) are read from memory again there (they are unchanged until this
+// kernel writes its outputs); with G = 4 the original band rows wait for the
+// pivoting fallback in LDS; diagnostics are written as soon as they are known.
+#include <atomic>
+
 #include "common.hpp"
 #include "heat.hpp"
 
@@ -63,14 +72,25 @@ constexpr int BAND_ROWS = MAX_NLEVSNO + MAX_NLEVGRND + 1;
 constexpr int BAND_W = 7;
 constexpr int TILE_DOUBLES = BAND_ROWS * BAND_W + BAND_ROWS;
 
+// G = 4 keeps the rows for the pivoting fallback in LDS (3 values x S slots x 64
+// lanes per wavefront); G = 1 and 2 would need too much LDS for that.
+template <int G>
+constexpr bool heat_saves_rows() { return G == 4; }
+template <int G>
+constexpr int heat_lds_doubles_per_wave() {
+    constexpr int S = (MAX_NLEVSNO + MAX_NLEVGRND + G - 1) / G;
+    return TILE_DOUBLES + (heat_saves_rows<G>() ? 3 * S * 64 : 0);
+}
+
 
 }  // namespace
 
 template <int G>
 __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c, const int k,
-                                            double *band_tile) {
+                                            double *band_tile, double *saved_rows) {
     constexpr int NL = MAX_NLEVSNO + MAX_NLEVGRND;   // 37 snow+soil levels
     constexpr int S = (NL + G - 1) / G;
+    constexpr bool SAVE_ROWS = heat_saves_rows<G>();
     const clmsoil_heat_args &a = P.a;
     const int64_t ld = a.ld;
     const int nlevsno = P.nlevsno, nlevgrnd = P.nlevgrnd;
@@ -99,69 +119,61 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     double int_snow = a.int_snow ? a.int_snow[c] : 0.0;
     double snomelt_accum = a.snomelt_accum ? a.snomelt_accum[c] : 0.0;
 
-    // ---- per-level loads ----
-    double t[S], liq[S], ice[S], z[S], dz[S], zit[S];
-    double wsat[S], tkmg[S], tkdry[S], csol[S], bsw[S], sucsat[S], exc[S], sabg[S];
+    // ---- per-level loads, thermal conductivity thk (:647-716) and heat capacity
+    // cv (:781-872), one level at a time so that only t, z, zi, dz, sabg, thk and
+    // cv stay live afterwards ----
+    double t[S], z[S], dz[S], zit[S], sabg[S], thk[S], cv[S];
 #pragma unroll
     for (int i = 0; i < S; ++i) {
         const int jj = i * G + k + 1;
         const int j = jj - nlevsno;
         const bool act = jj >= jtop && jj <= nlev;
-        t[i] = liq[i] = ice[i] = 0.0;
+        double liq = 0.0, ice = 0.0, wsat = 0.0, tkmg = 0.0, tkdry = 0.0, csol = 0.0, exc = 0.0;
+        t[i] = 0.0;
         z[i] = zit[i] = 0.0;
         dz[i] = 1.0;
-        wsat[i] = tkmg[i] = tkdry[i] = csol[i] = bsw[i] = sucsat[i] = exc[i] = sabg[i] = 0.0;
+        sabg[i] = 0.0;
+        const int64_t o = c + (int64_t)(jj - 1) * ld;
         if (act) {
-            const int64_t o = c + (int64_t)(jj - 1) * ld;
             t[i] = a.t_soisno[o];
-            liq[i] = a.h2osoi_liq[o];
-            ice[i] = a.h2osoi_ice[o];
+            liq = a.h2osoi_liq[o];
+            ice = a.h2osoi_ice[o];
             z[i] = __builtin_nontemporal_load(&a.z[o]);
             dz[i] = __builtin_nontemporal_load(&a.dz[o]);
             zit[i] = __builtin_nontemporal_load(&a.zi[o]);          // zi[c, jj] (top interface)
             if (j >= 1) {
                 const int64_t os = c + (int64_t)(j - 1) * ld;
-                wsat[i] = __builtin_nontemporal_load(&a.watsat[os]);
-                tkmg[i] = __builtin_nontemporal_load(&a.tkmg[os]);
-                tkdry[i] = __builtin_nontemporal_load(&a.tkdry[os]);
-                csol[i] = __builtin_nontemporal_load(&a.csol[os]);
-                bsw[i] = __builtin_nontemporal_load(&a.bsw[os]);
-                sucsat[i] = __builtin_nontemporal_load(&a.sucsat[os]);
-                exc[i] = a.excess_ice[os];
+                wsat = __builtin_nontemporal_load(&a.watsat[os]);
+                tkmg = __builtin_nontemporal_load(&a.tkmg[os]);
+                tkdry = __builtin_nontemporal_load(&a.tkdry[os]);
+                csol = __builtin_nontemporal_load(&a.csol[os]);
+                exc = a.excess_ice[os];
             }
             if (a.sabg_lyr_col && jj <= nlevsno + 1)
                 sabg[i] = a.sabg_lyr_col[c + (int64_t)(jj - 1) * ld];
         }
-    }
-
-    // ---- thermal conductivity thk (:647-716) ----
-    double thk[S], bwv_s[S];
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-        const int jj = i * G + k + 1;
-        const int j = jj - nlevsno;
-        const bool act = jj >= jtop && jj <= nlev;
+        const double dzi = dz[i];
         double th = 0.0, bwv = 0.0;
         if (act && j >= 1) {
-            double satw = (liq[i] / DENH2O + ice[i] / DENICE + exc[i] / DENICE) / (dz[i] * wsat[i]);
+            double satw = (liq / DENH2O + ice / DENICE + exc / DENICE) / (dzi * wsat);
             satw = jl_min(1.0, satw);
             if (satw > 1.0e-7) {
                 double dke;
                 if (t[i] >= TFRZ) dke = jl_max(0.0, log10(satw) + 1.0);
                 else dke = satw;
-                const double fl_denom = liq[i] / (DENH2O * dz[i]) + ice[i] / (DENICE * dz[i]) +
-                                        exc[i] / (DENICE * dz[i]);
-                const double fl = (liq[i] / (DENH2O * dz[i])) / fl_denom;
-                const double dksat = tkmg[i] * pow(TKWAT, fl * wsat[i]) * pow(TKICE, (1.0 - fl) * wsat[i]);
-                th = dke * dksat + (1.0 - dke) * tkdry[i];
+                const double fl_denom = liq / (DENH2O * dzi) + ice / (DENICE * dzi) +
+                                        exc / (DENICE * dzi);
+                const double fl = (liq / (DENH2O * dzi)) / fl_denom;
+                const double dksat = tkmg * pow(TKWAT, fl * wsat) * pow(TKICE, (1.0 - fl) * wsat);
+                th = dke * dksat + (1.0 - dke) * tkdry;
             } else {
-                th = tkdry[i];
+                th = tkdry;
             }
             if (j > nbedrock) th = THK_BEDROCK;
         }
         if (act && snl + 1 < 1 && j <= 0) {
-            const double denom = jl_max(frac_sno, 1.0e-6) * jl_max(dz[i], 1.0e-6);
-            bwv = (ice[i] + liq[i]) / denom;
+            const double denom = jl_max(frac_sno, 1.0e-6) * jl_max(dzi, 1.0e-6);
+            bwv = (ice + liq) / denom;
             if (P.snow_method == 2) {
                 if (bwv <= 156.0) th = 0.023 + 0.234 * (bwv / 1000.0);
                 else th = 0.138 - 1.01 * (bwv / 1000.0) + 3.233 * ((bwv / 1000.0) * (bwv / 1000.0));
@@ -170,24 +182,23 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             }
         }
         thk[i] = th;
-        bwv_s[i] = bwv;
-    }
-    // ---- heat capacity cv (:781-872) ----
-    double cv[S];
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-        const int jj = i * G + k + 1;
-        const int j = jj - nlevsno;
+        if (act) {
+            if (a.thk) a.thk[o] = th;
+            if (a.bw && j <= 0) a.bw[o] = bwv;
+        }
         double v = 1.0;
         if (j >= 1) {
-            v = csol[i] * (1.0 - wsat[i]) * dz[i] + (ice[i] * CPICE + liq[i] * CPLIQ) + exc[i] * CPICE;
-            if (j > nbedrock) v = CSOL_BEDROCK * dz[i];
+            v = csol * (1.0 - wsat) * dzi + (ice * CPICE + liq * CPLIQ) + exc * CPICE;
+            if (j > nbedrock) v = CSOL_BEDROCK * dzi;
             if (j == 1 && h2osno_nl > 0.0) v += CPICE * h2osno_nl;
         } else if (jj >= jtop) {
-            if (frac_sno > 0.0) v = jl_max(THIN_SFCLAYER, (CPLIQ * liq[i] + CPICE * ice[i]) / frac_sno);
+            if (frac_sno > 0.0) v = jl_max(THIN_SFCLAYER, (CPLIQ * liq + CPICE * ice) / frac_sno);
             else v = THIN_SFCLAYER;
         }
         cv[i] = v;
+        // keep the next level's loads below this point: the other wavefront on the
+        // SIMD hides their latency, and registers stay free for two wavefronts
+        __builtin_amdgcn_sched_barrier(0);
     }
     // ---- interface conductivity tk, fact, fn (:605-629, :1271-1310) ----
     double tk[S], fact[S], fn[S];
@@ -308,6 +319,36 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     }
     // b4h stays in the soil-1 owner lane, which is the lane that eliminates that row
 
+    // fn is needed after the solve only for the soil heat flux diagnostics: keep
+    // soil layer 1's value and store -cnfac*fn, to which the post-solve part adds
+    // -(1-cnfac)*fn1 (the same two roundings as computing the sum at once).
+    const double fn_s1 = level_value<G, S>(fn, js1, k);
+    if (a.eflx_fgr) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            const int jj = i * G + k + 1;
+            const int j = jj - nlevsno;
+            if (jj >= jtop && jj <= nlev && j >= 1)
+                a.eflx_fgr[c + (int64_t)(j - 1) * ld] = (j < nlevgrnd) ? -cnfac * fn[i] : 0.0;
+        }
+    }
+    // Compiler-only barrier: the level inputs used again after the solve (water,
+    // ice, dz, soil properties, z) are reloaded from memory below instead of being
+    // held in registers through the elimination.
+    asm volatile("" ::: "memory");
+    // The original diagonal, right-hand side and lower coupling of every row are
+    // needed again only by the (rare) pivoting fallback below: with G = 4 they go
+    // to this wavefront's LDS area instead of staying in registers.
+    if constexpr (SAVE_ROWS) {
+        const int lane = (int)__lane_id();
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            saved_rows[(0 * S + i) * 64 + lane] = b3[i];
+            saved_rows[(1 * S + i) * 64 + lane] = rv[i];
+            saved_rows[(2 * S + i) * 64 + lane] = sub[i];
+        }
+    }
+
     // ---- solve ----
     double x[S];
     double x_h = 0.0;
@@ -392,18 +433,25 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
                     const int jj = i * G + k + 1;
                     if (jj < jtop || jj > nlev) continue;
                     const int r = jj <= nlevsno ? jj - jt + 1 : jj - jt + 2;
-                    BT(r, r) = b3[i];
-                    xt[r - 1] = rv[i];
+                    double b3i = b3[i], rvi = rv[i], subi = sub[i];
+                    if constexpr (SAVE_ROWS) {
+                        const int lane = (int)__lane_id();
+                        b3i = saved_rows[(0 * S + i) * 64 + lane];
+                        rvi = saved_rows[(1 * S + i) * 64 + lane];
+                        subi = saved_rows[(2 * S + i) * 64 + lane];
+                    }
+                    BT(r, r) = b3i;
+                    xt[r - 1] = rvi;
                     if (jj <= nlevsno) {
-                        if (jj > jtop) BT(r, r - 1) = sub[i];
+                        if (jj > jtop) BT(r, r - 1) = subi;
                         if (jj < nlevsno) BT(r, r + 1) = sup[i];
                         else BT(r, r + 2) = sup[i];
                     } else if (jj == js1) {
-                        if (jtop <= nlevsno) BT(r, r - 2) = sub[i];
+                        if (jtop <= nlevsno) BT(r, r - 2) = subi;
                         BT(r, r - 1) = b4h;
                         if (jj < nlev) BT(r, r + 1) = sup[i];
                     } else {
-                        BT(r, r - 1) = sub[i];
+                        BT(r, r - 1) = subi;
                         if (jj < nlev) BT(r, r + 1) = sup[i];
                     }
                 }
@@ -473,20 +521,39 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
 #pragma unroll
     for (int i = 0; i < S; ++i) {
         const int jj = i * G + k + 1;
-        tnew[i] = (jj >= jtop && jj <= nlev) ? x[i] : t[i];
+        tnew[i] = (jj >= jtop && jj <= nlev) ? x[i] : 0.0;   // t of inactive levels is 0
     }
     const double x_s1 = level_value<G, S>(tnew, js1, k);
     t_h2osfc = (frac_h2osfc == 0.0) ? x_s1 : x_h;
     double fn1[S];
+    {
+        double zr[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            const int jj = i * G + k + 1;
+            zr[i] = (jj >= jtop && jj <= nlev) ? a.z[c + (int64_t)(jj - 1) * ld] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            const int jj = i * G + k + 1;
+            const int j = jj - nlevsno;
+            const double tn = hnxt<G>(tnew, i, k, S);
+            const double zn = hnxt<G>(zr, i, k, S);
+            double v = 0.0;
+            if (jj >= jtop && j <= nlevgrnd - 1) v = tk[i] * (tn - tnew[i]) / (zn - zr[i]);
+            fn1[i] = v;
+        }
+    }
+    // water and ice of every level, read again (unchanged in memory until the end)
+    double liq[S], ice[S];
 #pragma unroll
     for (int i = 0; i < S; ++i) {
         const int jj = i * G + k + 1;
-        const int j = jj - nlevsno;
-        const double tn = hnxt<G>(tnew, i, k, S);
-        const double zn = hnxt<G>(z, i, k, S);
-        double v = 0.0;
-        if (jj >= jtop && j <= nlevgrnd - 1) v = tk[i] * (tn - tnew[i]) / (zn - z[i]);
-        fn1[i] = v;
+        liq[i] = ice[i] = 0.0;
+        if (jj >= jtop && jj <= nlev) {
+            liq[i] = a.h2osoi_liq[c + (int64_t)(jj - 1) * ld];
+            ice[i] = a.h2osoi_ice[c + (int64_t)(jj - 1) * ld];
+        }
     }
 
     // ---- phase_change_h2osfc! (:1757-1826) ----
@@ -571,7 +638,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     // ---- phase_change_beta! (:1929-2289), Float64 (hard) branch ----
     const double tfrz = TFRZ, hfus = HFUS;
     int imelt[S];
-    double xm_a[S], wice0[S], wexice0[S], wmass0[S], tinc[S], dxmf[S], qmelt_l[S], qfrz_l[S];
+    double dxmf[S], qmelt_l[S], qfrz_l[S];
     double h2osno_nl_1 = h2osno_nl, snow_depth_1 = snow_depth, qsm_set = 0.0;
     bool unresolved = false;
     bool tgt_written[S];
@@ -580,31 +647,49 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         const int jj = i * G + k + 1;
         const int j = jj - nlevsno;
         const bool act = jj >= jtop && jj <= nlev;
+        const int64_t o = c + (int64_t)(jj - 1) * ld;
         imelt[i] = 0;
-        wice0[i] = ice[i];
-        wexice0[i] = j >= 1 ? exc[i] : 0.0;
-        wmass0[i] = ice[i] + liq[i] + wexice0[i];
-        tinc[i] = 0.0;
-        xm_a[i] = 0.0;
         dxmf[i] = qmelt_l[i] = qfrz_l[i] = 0.0;
         tgt_written[i] = false;
-        if (!act) continue;
+        if (!act) {
+            // phase_change_h2osfc! writes t_soisno[c, joff] also without snow layers
+            if (jj == nlevsno && wrote_t0) a.t_soisno[o] = tnew[i];
+            if (j <= 0 && jj >= 1) {
+                a.qflx_snomelt_lyr[o] = 0.0;
+                a.qflx_snofrz_lyr[o] = 0.0;
+            }
+            continue;
+        }
+        // soil properties of this level, read again (unchanged in memory)
+        const int64_t os = c + (int64_t)(j - 1) * ld;
+        double exc_i = 0.0, wsat_i = 0.0, bsw_i = 0.0, sucsat_i = 0.0, dz_i = 1.0;
+        if (j >= 1) {
+            exc_i = a.excess_ice[os];
+            wsat_i = a.watsat[os];
+            bsw_i = a.bsw[os];
+            sucsat_i = a.sucsat[os];
+            dz_i = a.dz[o];
+        }
+        const double wice0 = ice[i];
+        const double wexice0 = j >= 1 ? exc_i : 0.0;
+        const double wmass0 = ice[i] + liq[i] + wexice0;
+        double tinc = 0.0;
         double tt = tnew[i];
         double supercool = 0.0;
         if (j <= 0) {
-            if (ice[i] > 0.0 && tt > tfrz) { imelt[i] = 1; tinc[i] = tfrz - tt; tt = tfrz; }
-            if (liq[i] > 0.0 && tt < tfrz) { imelt[i] = 2; tinc[i] = tfrz - tt; tt = tfrz; }
+            if (ice[i] > 0.0 && tt > tfrz) { imelt[i] = 1; tinc = tfrz - tt; tt = tfrz; }
+            if (liq[i] > 0.0 && tt < tfrz) { imelt[i] = 2; tinc = tfrz - tt; tt = tfrz; }
         } else {
-            if (ice[i] > 0.0 && tt > tfrz) { imelt[i] = 1; tinc[i] = tfrz - tt; tt = tfrz; }
-            if (exc[i] > 0.0 && tt > tfrz) { imelt[i] = 1; tinc[i] = tfrz - tt; tt = tfrz; }
+            if (ice[i] > 0.0 && tt > tfrz) { imelt[i] = 1; tinc = tfrz - tt; tt = tfrz; }
+            if (exc_i > 0.0 && tt > tfrz) { imelt[i] = 1; tinc = tfrz - tt; tt = tfrz; }
             if (tt < tfrz) {
                 const double smp = hfus * (tfrz - tt) / (GRAV * tt) * 1000.0;
-                supercool = wsat[i] * pow(smp / sucsat[i], -1.0 / bsw[i]);
-                supercool *= dz[i] * 1000.0;
+                supercool = wsat_i * pow(smp / sucsat_i, -1.0 / bsw_i);
+                supercool *= dz_i * 1000.0;
             }
-            if (liq[i] > supercool && tt < tfrz) { imelt[i] = 2; tinc[i] = tfrz - tt; tt = tfrz; }
+            if (liq[i] > supercool && tt < tfrz) { imelt[i] = 2; tinc = tfrz - tt; tt = tfrz; }
             if (h2osno_nl > 0.0 && j == 1) {
-                if (tt > tfrz) { imelt[i] = 1; tinc[i] = tfrz - tt; tt = tfrz; }
+                if (tt > tfrz) { imelt[i] = 1; tinc = tfrz - tt; tt = tfrz; }
             }
         }
         // energy surplus/loss and rate of phase change
@@ -612,19 +697,19 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         const double fct = fact[i];
         if (imelt[i] > 0) {
             if (jj == jtop) {
-                if (j > 0) hm = dhsdT * tinc[i] - tinc[i] / fct;
-                else hm = frac_sno * (dhsdT * tinc[i] - tinc[i] / fct);
-                if (j == 1 && frac_h2osfc != 0.0) hm -= frac_h2osfc * dhsdT * tinc[i];
+                if (j > 0) hm = dhsdT * tinc - tinc / fct;
+                else hm = frac_sno * (dhsdT * tinc - tinc / fct);
+                if (j == 1 && frac_h2osfc != 0.0) hm -= frac_h2osfc * dhsdT * tinc;
             } else if (j == 1) {
-                hm = (1.0 - frac_sno - frac_h2osfc) * dhsdT * tinc[i] - tinc[i] / fct;
+                hm = (1.0 - frac_sno - frac_h2osfc) * dhsdT * tinc - tinc / fct;
             } else {
-                if (j < 1) hm = -frac_sno * tinc[i] / fct;
-                else hm = -tinc[i] / fct;
+                if (j < 1) hm = -frac_sno * tinc / fct;
+                else hm = -tinc / fct;
             }
         }
         if (imelt[i] == 1 && hm < 0.0) { hm = 0.0; imelt[i] = 0; }
         if (imelt[i] == 2 && hm > 0.0) { hm = 0.0; imelt[i] = 0; }
-        double icev = ice[i], liqv = liq[i], excv = exc[i];
+        double icev = ice[i], liqv = liq[i], excv = exc_i;
         if (imelt[i] > 0 && fabs(hm) > 0.0) {
             double xm = hm * dtime / hfus;
             if (j == 1 && h2osno_nl > 0.0 && xm > 0.0) {
@@ -640,26 +725,26 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             }
             double heatr = 0.0;
             if (xm > 0.0) {
-                icev = jl_max(0.0, wice0[i] - xm);
-                heatr = hm - hfus * (wice0[i] - icev) / dtime;
+                icev = jl_max(0.0, wice0 - xm);
+                heatr = hm - hfus * (wice0 - icev) / dtime;
                 const double xm2 = xm - icev;
                 if (icev == 0.0) {
-                    if (wexice0[i] >= 0.0 && xm2 > 0.0 && j >= 2) {
-                        excv = jl_max(0.0, wexice0[i] - xm2);
-                        heatr = hm - hfus * (wexice0[i] - excv + wice0[i] - icev) / dtime;
+                    if (wexice0 >= 0.0 && xm2 > 0.0 && j >= 2) {
+                        excv = jl_max(0.0, wexice0 - xm2);
+                        heatr = hm - hfus * (wexice0 - excv + wice0 - icev) / dtime;
                     }
                 }
             } else if (xm < 0.0) {
                 if (j <= 0) {
-                    icev = jl_min(wmass0[i], wice0[i] - xm);
+                    icev = jl_min(wmass0, wice0 - xm);
                 } else {
-                    if (wmass0[i] - wexice0[i] < supercool) icev = 0.0;
-                    else icev = jl_min(wmass0[i] - wexice0[i] - supercool, wice0[i] - xm);
+                    if (wmass0 - wexice0 < supercool) icev = 0.0;
+                    else icev = jl_min(wmass0 - wexice0 - supercool, wice0 - xm);
                 }
-                heatr = hm - hfus * (wice0[i] - icev) / dtime;
+                heatr = hm - hfus * (wice0 - icev) / dtime;
             }
             const double ei_val = j >= 1 ? excv : 0.0;
-            liqv = jl_max(0.0, wmass0[i] - icev - ei_val);
+            liqv = jl_max(0.0, wmass0 - icev - ei_val);
             if (fabs(heatr) > 0.0) {
                 if (jj == jtop) {
                     if (j == 1) {
@@ -677,18 +762,31 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
                     if (liqv * icev > 0.0) tt = tfrz;
             }
             if (j >= 1) {
-                dxmf[i] = hfus * (wice0[i] - icev) / dtime + hfus * (wexice0[i] - excv) / dtime;
+                dxmf[i] = hfus * (wice0 - icev) / dtime + hfus * (wexice0 - excv) / dtime;
             } else {
-                dxmf[i] = hfus * (wice0[i] - icev) / dtime;
+                dxmf[i] = hfus * (wice0 - icev) / dtime;
             }
-            if (imelt[i] == 1 && j < 1) qmelt_l[i] = jl_max(0.0, wice0[i] - icev) / dtime;
-            if (imelt[i] == 2 && j < 1) qfrz_l[i] = jl_max(0.0, icev - wice0[i]) / dtime;
+            if (imelt[i] == 1 && j < 1) qmelt_l[i] = jl_max(0.0, wice0 - icev) / dtime;
+            if (imelt[i] == 2 && j < 1) qfrz_l[i] = jl_max(0.0, icev - wice0) / dtime;
             tgt_written[i] = true;
         }
         tnew[i] = tt;
-        ice[i] = icev;
-        liq[i] = liqv;
-        exc[i] = excv;
+        // write back this level (:1929-2289 outputs)
+        a.t_soisno[o] = tt;
+        a.h2osoi_liq[o] = liqv;
+        a.h2osoi_ice[o] = icev;
+        if (j >= 1 && excv != wexice0) a.excess_ice[os] = excv;
+        if (j <= 0) {
+            a.qflx_snomelt_lyr[o] = qmelt_l[i];
+            a.qflx_snofrz_lyr[o] = qfrz_l[i];
+        }
+        if (a.fact) a.fact[o] = fact[i];
+        if (a.imelt) a.imelt[o] = imelt[i];
+        if (j >= 1) {
+            if (a.eflx_fgr && j < nlevgrnd) a.eflx_fgr[os] = a.eflx_fgr[os] - (1.0 - cnfac) * fn1[i];
+            if (a.exice_subs)
+                a.exice_subs[os] = tgt_written[i] ? jl_max(0.0, (wexice0 - excv) / DENICE) : 0.0;
+        }
     }
     // column accumulators, summed over levels in the reference's order
     double xmf = 0.0, qflx_snomelt = 0.0, qflx_snofrz = 0.0, qflx_snow_drain = 0.0;
@@ -738,45 +836,9 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         if (frac_h2osfc != 0.0) tg = (1.0 - frac_h2osfc) * t_s1 + frac_h2osfc * t_h2osfc;
         else tg = t_s1;
     }
-    const double fn_s1 = level_value<G, S>(fn, js1, k);
     const double fn1_s1 = level_value<G, S>(fn1, js1, k);
 
-    // ---- write back ----
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-        const int jj = i * G + k + 1;
-        const int j = jj - nlevsno;
-        const bool act = jj >= jtop && jj <= nlev;
-        const int64_t o = c + (int64_t)(jj - 1) * ld;
-        if (act) {
-            a.t_soisno[o] = tnew[i];
-            a.h2osoi_liq[o] = liq[i];
-            a.h2osoi_ice[o] = ice[i];
-            if (j >= 1 && exc[i] != wexice0[i]) a.excess_ice[c + (int64_t)(j - 1) * ld] = exc[i];
-            if (j <= 0) {
-                a.qflx_snomelt_lyr[o] = qmelt_l[i];
-                a.qflx_snofrz_lyr[o] = qfrz_l[i];
-            }
-            if (a.fact) a.fact[o] = fact[i];
-            if (a.thk) a.thk[o] = thk[i];
-            if (a.imelt) a.imelt[o] = imelt[i];
-            if (a.bw && j <= 0) a.bw[o] = bwv_s[i];
-            if (j >= 1) {
-                const int64_t os = c + (int64_t)(j - 1) * ld;
-                if (a.eflx_fgr)
-                    a.eflx_fgr[os] = (j < nlevgrnd) ? -cnfac * fn[i] - (1.0 - cnfac) * fn1[i] : 0.0;
-                if (a.exice_subs)
-                    a.exice_subs[os] = tgt_written[i] ? jl_max(0.0, (wexice0[i] - exc[i]) / DENICE) : 0.0;
-            }
-        } else if (jj == nlevsno && wrote_t0) {
-            // phase_change_h2osfc! writes t_soisno[c, joff] also without snow layers
-            a.t_soisno[o] = tnew[i];
-        }
-        if (!act && j <= 0 && jj >= 1) {
-            a.qflx_snomelt_lyr[o] = 0.0;
-            a.qflx_snofrz_lyr[o] = 0.0;
-        }
-    }
+    // ---- write back the column scalars (levels were written above) ----
     if (k == 0) {
         a.t_h2osfc[c] = t_h2osfc;
         a.h2osfc[c] = h2osfc;
@@ -799,22 +861,35 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     }
 }
 
+// Dynamic LDS: one area of heat_lds_doubles_per_wave<G>() doubles per wavefront.
 template <int G>
 __global__ __launch_bounds__(256, CLMSOIL_HEAT_WAVES) void heat_kernel(HeatParams P) {
-    __shared__ double tiles[256 / 64][TILE_DOUBLES];
+    extern __shared__ double heat_lds[];
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t c = gtid / G;
     const int k = (int)(threadIdx.x % G);
     if (c >= P.a.nc) return;
     if (P.a.mask && !P.a.mask[c]) return;
-    heat_column<G>(P, c, k, tiles[threadIdx.x / 64]);
+    double *area = heat_lds + (int64_t)(threadIdx.x / 64) * heat_lds_doubles_per_wave<G>();
+    heat_column<G>(P, c, k, area, area + TILE_DOUBLES);
 }
 
 template <int G>
 static hipError_t launch_g(const HeatParams &P, hipStream_t stream, int block) {
     const int64_t threads = P.a.nc * G;
     const int64_t grid = (threads + block - 1) / block;
-    hipLaunchKernelGGL(heat_kernel<G>, dim3((unsigned)grid), dim3(block), 0, stream, P);
+    const size_t lds = (size_t)(block / 64) * heat_lds_doubles_per_wave<G>() * sizeof(double);
+    if (lds > 65536) {
+        static std::atomic<int> raised{0};
+        if (!raised.load(std::memory_order_relaxed)) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&heat_kernel<G>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)lds);
+            if (e != hipSuccess) return e;
+            raised.store(1, std::memory_order_relaxed);
+        }
+    }
+    hipLaunchKernelGGL(heat_kernel<G>, dim3((unsigned)grid), dim3(block), lds, stream, P);
     return hipGetLastError();
 }
 
