@@ -63,6 +63,16 @@ class HeatArgs(C.Structure):
     _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in HEAT_FIELDS]
 
 
+ZD_FIELDS = ["mask", "z", "dz", "zi", "watsat", "hksat", "bsw", "sucsat", "smpmin",
+             "h2osoi_liq", "h2osoi_ice", "h2osoi_vol", "t_soisno", "zwt", "qflx_infl",
+             "qflx_rootsoi", "icefrac", "smp_l", "hk_l", "qcharge", "qflx_deficit"]
+
+
+class ZdArgs(C.Structure):
+    """clmsoil_zd_args (include/clmsoil_zd09.h)."""
+    _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in ZD_FIELDS]
+
+
 class HeatConfig(C.Structure):
     _fields_ = [("snow_thermal_cond_method", C.c_int32), ("reserved", C.c_int32)]
 
@@ -77,8 +87,11 @@ def declare_common(lib: C.CDLL, prefix: str) -> C.CDLL:
     getattr(lib, prefix + "batched_band_solve").argtypes = [P, P, P, P, P, P, i32, i32, i32, i64, i64, i64] + \
         ([P] if prefix == "clmsoil_" else [])
     for f in ("tridiagonal_multi", "batched_band_solve", "soilwater_moisture_form",
-              "soil_temperature"):
+              "soil_temperature", "soilwater_zengdecker2009"):
         getattr(lib, prefix + f).restype = C.c_int
+    lib_zd = getattr(lib, prefix + "soilwater_zengdecker2009")
+    lib_zd.argtypes = [C.POINTER(ZdArgs), C.POINTER(Levels), dbl, C.POINTER(SwmConfig)] + \
+        ([P] if prefix == "clmsoil_" else [])
     lib_sw = getattr(lib, prefix + "soilwater_moisture_form")
     lib_sw.argtypes = [C.POINTER(MfArgs), C.POINTER(Levels), dbl, C.POINTER(SwmConfig)] + \
         ([P] if prefix == "clmsoil_" else [])

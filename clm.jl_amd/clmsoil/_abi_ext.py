@@ -19,6 +19,8 @@ def declare(lib) -> None:
     lib.clmsoil_set_richards_schedule.restype = I
     lib.clmsoil_set_richards_residency.argtypes = [I]
     lib.clmsoil_set_richards_residency.restype = I
+    lib.clmsoil_set_zd_block.argtypes = [I]
+    lib.clmsoil_set_zd_block.restype = I
     _declare_learned(lib)
     _declare_adjoint(lib)
     _declare_day(lib)

@@ -23,7 +23,7 @@ import ctypes as C
 from dataclasses import dataclass, field
 
 from . import _abi
-from .bind import byref, heat_args, heat_config, levels, mf_args, swm_config
+from .bind import byref, heat_args, heat_config, levels, mf_args, swm_config, zd_args
 
 ZENGDECKER_2009, MOISTURE_FORM = _abi.ZENGDECKER_2009, _abi.MOISTURE_FORM
 BC_HEAD, BC_FLUX, BC_ZERO_FLUX, BC_WATERTABLE, BC_AQUIFER = (
@@ -165,14 +165,42 @@ def soilwater_moisture_form(st: MfState, mfin: MfIn, mask, nbedrock, nlevsno: in
                                                     _stream()), "soilwater_moisture_form")
 
 
-def soil_water(st: MfState, mfin: MfIn, mask, nbedrock, nlevsno: int, nlevsoi: int, nlevgrnd: int,
+def soilwater_zengdecker2009(state: dict, dtime: float, cfg: SoilWaterMovementConfig,
+                             nlevsno: int = 12, nlevsoi: int = 20, nlevgrnd: int = 25) -> None:
+    """One Zeng-Decker 2009 step (soilwater_zengdecker2009!,
+    soil_water_movement.jl:1616-1764) for every masked column.
+
+    `state` maps the field names of clmsoil_zd_args (include/clmsoil_zd09.h) to
+    float64 device tensors (`mask` uint8 or absent); h2osoi_liq is updated and
+    icefrac, smp_l, hk_l (soil layers 1..nlevsoi), qcharge and qflx_deficit are
+    written."""
+    c = cfg.to_abi()
+    if c.soilwater_movement_method != ZENGDECKER_2009:
+        raise CLMSoilError("soilwater_zengdecker2009!: soilwater_movement_method must be ZENGDECKER_2009")
+    d = dict(state)
+    d.setdefault("nc", d["zwt"].shape[0])
+    d.update(nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd)
+    if d["nc"] == 0:
+        return
+    L = _lib()
+    a, lev = zd_args(d), levels(d)
+    _abi.check(L, L.clmsoil_soilwater_zengdecker2009(byref(a), byref(lev), float(dtime), byref(c),
+                                                     _stream()), "soilwater_zengdecker2009")
+
+
+def soil_water(st, mfin, mask, nbedrock, nlevsno: int, nlevsoi: int, nlevgrnd: int,
                cfg: SoilWaterMovementConfig, dtime: float) -> None:
-    """soil_water! dispatch (soil_water_movement.jl:1780-1810). Only the
-    moisture form runs on this backend; Zeng-Decker 2009 raises."""
+    """soil_water! dispatch (soil_water_movement.jl:1780-1810). For the moisture
+    form `st`/`mfin` are MfState/MfIn; for Zeng-Decker 2009 `st` is the field
+    dict of soilwater_zengdecker2009 (`mfin` and `nbedrock` are unused, a
+    non-None `mask` replaces the dict's)."""
     if cfg.soilwater_movement_method == MOISTURE_FORM:
         soilwater_moisture_form(st, mfin, mask, nbedrock, nlevsno, nlevsoi, nlevgrnd, dtime, cfg)
     elif cfg.soilwater_movement_method == ZENGDECKER_2009:
-        raise CLMSoilError("soil_water!: the Zeng-Decker 2009 method is not provided by this backend")
+        d = dict(st)
+        if mask is not None:
+            d["mask"] = mask
+        soilwater_zengdecker2009(d, dtime, cfg, nlevsno, nlevsoi, nlevgrnd)
     else:
         raise CLMSoilError("soil_water!: a SoilWater implementation must be specified!")
 
@@ -242,6 +270,12 @@ def set_richards_residency(blocks_per_cu: int) -> None:
     """Blocks per CU of the resident Richards grid (0 = occupancy calculator)."""
     L = _lib()
     _abi.check(L, L.clmsoil_set_richards_residency(int(blocks_per_cu)), "set_richards_residency")
+
+
+def set_zd_block(block: int) -> None:
+    """Workgroup size of the Zeng-Decker 2009 step (64, 128 or 256)."""
+    L = _lib()
+    _abi.check(L, L.clmsoil_set_zd_block(int(block)), "set_zd_block")
 
 
 def get_tuning() -> tuple:

@@ -10,7 +10,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import HEAT_FIELDS, MF_FIELDS, HeatArgs, HeatConfig, Levels, MfArgs, SwmConfig
+from ._abi import (HEAT_FIELDS, MF_FIELDS, ZD_FIELDS, HeatArgs, HeatConfig, Levels, MfArgs,
+                   SwmConfig, ZdArgs)
 
 _INT_FIELDS = {"nbedrock", "snl", "imelt"}
 _U8_FIELDS = {"mask"}
@@ -65,6 +66,18 @@ def heat_args(d: dict) -> HeatArgs:
     a.nc = nc
     a.ld = int(d.get("ld", nc))
     for f in HEAT_FIELDS:
+        x = d.get(f)
+        _check_dtype(f, x)
+        setattr(a, f, _ptr(x))
+    return a
+
+
+def zd_args(d: dict) -> ZdArgs:
+    nc = int(d["nc"])
+    a = ZdArgs()
+    a.nc = nc
+    a.ld = int(d.get("ld", nc))
+    for f in ZD_FIELDS:
         x = d.get(f)
         _check_dtype(f, x)
         setattr(a, f, _ptr(x))

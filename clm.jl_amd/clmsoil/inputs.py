@@ -318,3 +318,43 @@ def calibration_columns(nc=262144, seed=3, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nle
         smpmin=np.full(nc, -1.0e8), smp_l=np.zeros((nlevgrnd, nc)),
         hk_l=np.zeros((nlevgrnd, nc)), qcharge=np.zeros(nc), nsubsteps=np.zeros(nc),
         nrejected=np.zeros(nc))
+
+
+def zengdecker_columns(nc=4096, seed=4, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrnd=NLEVGRND,
+                       frozen_fraction=0.3, zwt_range=(0.3, 12.0)):
+    """Columns for the Zeng-Decker 2009 step (the reference's default method with
+    the BC_AQUIFER lower boundary): texture and pedotransfer as config 0, liquid
+    water theta ~ U(0.2, 0.95) x watsat, ice on a `frozen_fraction` of the
+    columns (ice fraction ~ U(0, 0.5) of the pore water, those columns' soil at
+    U(260, 273) K, others U(274, 300) K), water table ~ U(zwt_range) m (so both
+    'inside the soil column' and 'below it' occur), infiltration ~ U(0, 2e-3)
+    mm/s, root sink ~ U(0, 1e-5) mm/s per layer, smpmin = -1e8 mm. h2osoi_vol
+    is the layers' total volumetric water content."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sand, clay, om = _texture(rng, nc)
+    dzs, zs, zis = clm5_grid(nlevsoi, nlevgrnd)
+    props = pedotransfer(sand, clay, om, zs)
+    snl = np.zeros(nc, dtype=np.int64)
+    z, dz, zi = _geometry(nc, snl, np.zeros((nlevsno, nc)), nlevsno, nlevsoi, nlevgrnd, 0.0)
+    nlev = nlevsno + nlevgrnd
+    theta = rng.uniform(0.2, 0.95, (nlevgrnd, nc)) * props["watsat"]
+    frozen = rng.uniform(0.0, 1.0, nc) < frozen_fraction
+    fice = np.where(frozen[None, :], rng.uniform(0.0, 0.5, (nlevgrnd, nc)), 0.0)
+    liq = np.zeros((nlev, nc))
+    ice = np.zeros((nlev, nc))
+    liq[nlevsno:] = (1.0 - fice) * theta * dzs[:, None] * DENH2O
+    ice[nlevsno:] = fice * theta * dzs[:, None] * DENICE
+    t = np.zeros((nlev, nc))
+    t[nlevsno:] = np.where(frozen[None, :], rng.uniform(260.0, 273.0, (nlevgrnd, nc)),
+                           rng.uniform(274.0, 300.0, (nlevgrnd, nc)))
+    vol = liq[nlevsno:] / (dzs[:, None] * DENH2O) + ice[nlevsno:] / (dzs[:, None] * DENICE)
+    return dict(
+        nc=nc, nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd, dtime=1800.0,
+        z=z, dz=dz, zi=zi, watsat=props["watsat"], hksat=props["hksat"], bsw=props["bsw"],
+        sucsat=props["sucsat"], smpmin=np.full(nc, -1.0e8), h2osoi_liq=liq, h2osoi_ice=ice,
+        h2osoi_vol=np.ascontiguousarray(vol), t_soisno=t,
+        zwt=rng.uniform(zwt_range[0], zwt_range[1], nc),
+        qflx_infl=rng.uniform(0.0, 2.0e-3, nc),
+        qflx_rootsoi=rng.uniform(0.0, 1.0e-5, (nlevsoi, nc)),
+        icefrac=np.zeros((nlevgrnd, nc)), smp_l=np.zeros((nlevgrnd, nc)),
+        hk_l=np.zeros((nlevgrnd, nc)), qcharge=np.zeros(nc), qflx_deficit=np.zeros(nc))

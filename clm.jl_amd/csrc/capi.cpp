@@ -15,11 +15,13 @@
 #include "../../include/clmsoil.h"
 #include "../../include/clmsoil_ext.h"
 #include "../../include/clmsoil_adjoint.h"
+#include "../../include/clmsoil_zd09.h"
 #include "common.hpp"
 #include "heat.hpp"
 #include "richards.hpp"
 #include "solvers.hpp"
 #include "state.hpp"
+#include "zd09.hpp"
 
 using namespace clmsoil;
 
@@ -223,6 +225,12 @@ int clmsoil_set_richards_residency(int blocks_per_cu) {
     return CLMSOIL_OK;
 }
 
+int clmsoil_set_zd_block(int block) {
+    if (block != 64 && block != 128 && block != 256) return CLMSOIL_ERR_ARG;
+    g_tuning.zd_block = block;
+    return CLMSOIL_OK;
+}
+
 int clmsoil_get_tuning(int *richards_g, int *richards_block, int *heat_g, int *heat_block) {
     if (!richards_g || !richards_block || !heat_g || !heat_block) return CLMSOIL_ERR_NULL;
     *richards_g = g_tuning.richards_g;
@@ -326,6 +334,36 @@ int clmsoil_batched_band_solve_vjp(double *tbar, const double *tvector, const do
     BandVjpParams P{tbar, tvector, bmatrix, jtop, jbot, mask, kl, ku, nlevsno_off, nc, nlev, ld,
                     bmatrix_bar, rvector_bar, s->err};
     return launch_band_vjp(P, (hipStream_t)stream) == hipSuccess ? CLMSOIL_OK : CLMSOIL_ERR_HIP;
+}
+
+// Host checks of soil_water! (soil_water_movement.jl:1780-1810) for the
+// Zeng-Decker 2009 method. The aquifer row reads layer nlevsoi+1 of t_soisno and
+// zi, so the ground must extend below the soil column.
+int clmsoil_soilwater_zengdecker2009(const clmsoil_zd_args *a, const clmsoil_levels *lev,
+                                     double dtime, const clmsoil_swm_config *cfg,
+                                     void *stream) {
+    if (!a || !cfg) return CLMSOIL_ERR_NULL;
+    int st = validate_levels(lev);
+    if (st) return st;
+    if (lev->nlevgrnd < lev->nlevsoi + 1) return CLMSOIL_ERR_SIZE;
+    if (cfg->soilwater_movement_method != CLMSOIL_ZENGDECKER_2009) return CLMSOIL_ERR_METHOD;
+    if (a->nc < 0 || a->ld < a->nc) return CLMSOIL_ERR_SIZE;
+    if (!(dtime > 0.0) || !std::isfinite(dtime) || !std::isfinite(cfg->e_ice))
+        return CLMSOIL_ERR_ARG;
+    if (a->nc == 0) return CLMSOIL_OK;
+    if (!a->z || !a->dz || !a->zi || !a->watsat || !a->hksat || !a->bsw || !a->sucsat ||
+        !a->smpmin || !a->h2osoi_liq || !a->h2osoi_ice || !a->h2osoi_vol || !a->t_soisno ||
+        !a->zwt || !a->qflx_infl || !a->qflx_rootsoi || !a->icefrac || !a->smp_l ||
+        !a->hk_l || !a->qcharge || !a->qflx_deficit)
+        return CLMSOIL_ERR_NULL;
+    ZdParams P{a->nc, a->ld, a->mask, a->z, a->dz, a->zi, a->watsat, a->hksat, a->bsw,
+               a->sucsat, a->smpmin, a->h2osoi_liq, a->h2osoi_ice, a->h2osoi_vol,
+               a->t_soisno, a->zwt, a->qflx_infl, a->qflx_rootsoi, a->icefrac, a->smp_l,
+               a->hk_l, a->qcharge, a->qflx_deficit, lev->nlevsno, lev->nlevsoi,
+               lev->nlevgrnd, dtime, cfg->e_ice};
+    return launch_zengdecker2009(P, g_tuning.zd_block, (hipStream_t)stream) == hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
 }
 
 }  // extern "C"

@@ -15,6 +15,7 @@ struct Tuning {
     int heat_block = 256;
     int richards_persistent = 1;   // resident grid with per-group column refill
     int richards_blocks_per_cu = 0;  // persistent grid size; 0 = occupancy query
+    int zd_block = 128;              // Zeng-Decker 2009 step, one lane per column
 };
 extern Tuning g_tuning;
 

@@ -36,6 +36,9 @@ int clmsoil_set_richards_schedule(int persistent);
 /* Blocks per CU of the resident Richards grid (0, the default: as many as the
  * occupancy calculator says fit; 1..32 override it). Results are identical. */
 int clmsoil_set_richards_residency(int blocks_per_cu);
+/* Workgroup size (64, 128 — the default — or 256) of the Zeng-Decker 2009 step
+ * (include/clmsoil_zd09.h). Results are identical. */
+int clmsoil_set_zd_block(int block);
 
 #ifdef __cplusplus
 }

@@ -1225,3 +1225,303 @@ int orc_soil_temperature(const clmsoil_heat_args *a, const clmsoil_levels *lev,
     }
     return CLMSOIL_OK;
 }
+
+/* ---------------------------------------------------------------------------
+ * Zeng-Decker 2009 soil water step, soilwater_zengdecker2009!
+ * (soil_water_movement.jl:1616-1764), one column at a time in the order of its
+ * kernels; smooth_min/smooth_max/smooth_ifelse on Float64 are Julia's exact
+ * min/max/branch (smooth_ad.jl:91-149, 253-262). Scratch index j is the soil
+ * layer (1-based, column entry j); zimm[j] is Fortran zimm(c, j), j = 0..nlevsoi.
+ * ------------------------------------------------------------------------- */
+#define ZMAX 32
+/* Test hook of the Zeng-Decker step: with a non-zero seed every pow() of that
+ * step is moved one ulp up or down, the direction taken from a hash of its
+ * operands and the seed (so the result is deterministic and thread-safe). pow()
+ * is the one operation whose last bit differs between the host libm and the HIP
+ * device library; the parity tests use this to measure how far such a
+ * difference moves the step's results. Zero (the default) leaves pow() exact. */
+static uint64_t g_zd_pow_jitter = 0;
+void orc_set_zd_pow_jitter(uint64_t seed) { g_zd_pow_jitter = seed; }
+
+static double zd_pow(double x, double y) {
+    const double p = pow(x, y);
+    if (!g_zd_pow_jitter || !isfinite(p) || p == 0.0) return p;
+    uint64_t bx, by;
+    memcpy(&bx, &x, 8);
+    memcpy(&by, &y, 8);
+    uint64_t h = bx ^ (by * 0x9E3779B97F4A7C15ull) ^ g_zd_pow_jitter;
+    h ^= h >> 30; h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 27; h *= 0x94D049BB133111EBull;
+    h ^= h >> 31;
+    return nextafter(p, (h & 1u) ? INFINITY : -INFINITY);
+}
+
+static void zd09_column(const clmsoil_zd_args *a, const clmsoil_levels *lev, int64_t c,
+                        double dtime, double e_ice) {
+    const int64_t ld = a->ld;
+    const int n = lev->nlevsoi, ngrnd = lev->nlevgrnd;
+    const int joff = lev->nlevsno, joff_zi = lev->nlevsno + 1;
+    double zmm[ZMAX], dzmm[ZMAX], zimm[ZMAX], vol_ice[ZMAX], icefrac[ZMAX], vwc_liq[ZMAX];
+    double vol_eq[ZMAX], zq[ZMAX], hk[ZMAX], dhkdw[ZMAX], imped[ZMAX], smp[ZMAX], dsmpdw[ZMAX];
+    double qin[ZMAX], qout[ZMAX], dqidw0[ZMAX], dqidw1[ZMAX], dqodw1[ZMAX], dqodw2[ZMAX];
+    double rmx[ZMAX], amx[ZMAX], bmx[ZMAX], cmx[ZMAX], dwat2[ZMAX], cp[ZMAX], dp[ZMAX];
+    for (int j = 0; j < ZMAX; ++j)
+        zmm[j] = dzmm[j] = zimm[j] = vol_ice[j] = icefrac[j] = vwc_liq[j] = vol_eq[j] = zq[j] =
+            hk[j] = dhkdw[j] = imped[j] = smp[j] = dsmpdw[j] = qin[j] = qout[j] = dqidw0[j] =
+                dqidw1[j] = dqodw1[j] = dqodw2[j] = rmx[j] = amx[j] = bmx[j] = cmx[j] = dwat2[j] =
+                    0.0;
+    /* _soilwm_mm_icefrac_kernel! (:97-121) */
+    for (int j = 1; j <= n; ++j) {
+        zmm[j] = A(a->z, c, joff + j) * 1.0e3;
+        dzmm[j] = A(a->dz, c, joff + j) * 1.0e3;
+        zimm[j] = A(a->zi, c, joff_zi + j) * 1.0e3;
+        vol_ice[j] = JMIN(A(a->watsat, c, j), A(a->h2osoi_ice, c, joff + j) /
+                                                  (A(a->dz, c, joff + j) * DENICE));
+        icefrac[j] = JMIN(1.0, vol_ice[j] / A(a->watsat, c, j));
+        vwc_liq[j] = JMAX(A(a->h2osoi_liq, c, joff + j), 1.0e-6) /
+                     (A(a->dz, c, joff + j) * DENH2O);
+        A(a->icefrac, c, j) = icefrac[j];
+    }
+    /* _soilwm_zwtmm_kernel! (:143-150) */
+    zimm[0] = 0.0;
+    const double zwtmm = a->zwt[c] * 1.0e3;
+    /* _soilwm_jwt_vwczwt_kernel! (:289-324) */
+    int jw = n;
+    for (int j = 1; j <= n; ++j)
+        if (a->zwt[c] <= A(a->zi, c, joff_zi + j)) { jw = j - 1; break; }
+    double vwc_zwt = A(a->watsat, c, n);
+    if (A(a->t_soisno, c, joff + jw + 1) < TFRZ) {
+        double v = vwc_liq[n];
+        for (int j = n; j <= ngrnd; ++j) {
+            if (a->zwt[c] <= A(a->zi, c, joff_zi + j)) {
+                double smp1 = HFUS * (TFRZ - A(a->t_soisno, c, joff + j)) /
+                              (GRAV * A(a->t_soisno, c, joff + j)) * 1000.0;
+                smp1 = JMAX(A(a->sucsat, c, n), smp1);
+                v = A(a->watsat, c, n) * zd_pow(smp1 / A(a->sucsat, c, n), -1.0 / A(a->bsw, c, n));
+                v = JMIN(v, 0.5 * (A(a->watsat, c, n) + A(a->h2osoi_vol, c, n)));
+                break;
+            }
+        }
+        vwc_zwt = v;
+    }
+    /* _soilwm_voleq_zq_kernel! (:171-201) */
+    const double smpmin = a->smpmin[c];
+    for (int j = 1; j <= n; ++j) {
+        const double zimm_jm1 = zimm[j - 1], zimm_j = zimm[j];
+        const double ws = A(a->watsat, c, j), suc = A(a->sucsat, c, j), b = A(a->bsw, c, j);
+        const double zwt = zwtmm;
+        const double onem1b = 1.0 - 1.0 / b;
+        if (zwt <= zimm_jm1) {
+            vol_eq[j] = ws;
+        } else if (zwt < zimm_j && zwt > zimm_jm1) {
+            const double tempi = 1.0;
+            const double temp0 = zd_pow((suc + zwt - zimm_jm1) / suc, onem1b);
+            const double voleq1 = -suc * ws / onem1b / (zwt - zimm_jm1) * (tempi - temp0);
+            double v = (voleq1 * (zwt - zimm_jm1) + ws * (zimm_j - zwt)) / (zimm_j - zimm_jm1);
+            v = JMIN(ws, v);
+            vol_eq[j] = JMAX(v, 0.0);
+        } else {
+            const double tempi = zd_pow((suc + zwt - zimm_j) / suc, onem1b);
+            const double temp0 = zd_pow((suc + zwt - zimm_jm1) / suc, onem1b);
+            double v = -suc * ws / onem1b / (zimm_j - zimm_jm1) * (tempi - temp0);
+            v = JMAX(v, 0.0);
+            vol_eq[j] = JMIN(ws, v);
+        }
+        const double zqv = -suc * zd_pow(JMAX(vol_eq[j] / ws, 0.01), -b);
+        zq[j] = JMAX(smpmin, zqv);
+    }
+    /* _soilwm_voleq_zq_aqu_kernel! (:346-364) */
+    if (jw == n) {
+        const int j = n;
+        const double suc = A(a->sucsat, c, j), ws = A(a->watsat, c, j), b = A(a->bsw, c, j);
+        const double zwt = zwtmm, zimm_j = zimm[j];
+        const double onem1b = 1.0 - 1.0 / b;
+        const double tempi = 1.0;
+        const double temp0 = zd_pow((suc + zwt - zimm_j) / suc, onem1b);
+        double v = -suc * ws / onem1b / (zwt - zimm_j) * (tempi - temp0);
+        v = JMAX(v, 0.0);
+        v = JMIN(ws, v);
+        vol_eq[j + 1] = v;
+        const double zqv = -suc * zd_pow(JMAX(v / ws, 0.01), -b);
+        zq[j + 1] = JMAX(smpmin, zqv);
+    }
+    /* _soilwm_hk_smp_kernel! (:223-262) */
+    for (int j = 1; j <= n; ++j) {
+        const int jp1 = j + 1 < n ? j + 1 : n;
+        const double b = A(a->bsw, c, j);
+        double s1 = 0.5 * (vwc_liq[j] + vwc_liq[jp1]) /
+                    (0.5 * (A(a->watsat, c, j) + A(a->watsat, c, jp1)));
+        s1 = JMIN(1.0, s1);
+        const double s2 = A(a->hksat, c, j) * zd_pow(s1, 2.0 * b + 2.0);
+        const double imp = zd_pow(10.0, -e_ice * (0.5 * (icefrac[j] + icefrac[jp1])));
+        imped[j] = imp;
+        hk[j] = imp * s1 * s2;
+        dhkdw[j] = imp * (2.0 * b + 3.0) * s2 * (1.0 / (A(a->watsat, c, j) + A(a->watsat, c, jp1)));
+        double s_node = JMAX(vwc_liq[j] / A(a->watsat, c, j), 0.01);
+        s_node = JMIN(1.0, s_node);
+        double smpv = -A(a->sucsat, c, j) * zd_pow(s_node, -b);
+        smpv = JMAX(smpmin, smpv);
+        smp[j] = smpv;
+        dsmpdw[j] = -b * smpv / vwc_liq[j];
+        A(a->smp_l, c, j) = smpv;
+        A(a->hk_l, c, j) = hk[j];
+    }
+    /* _soilwm_aqu_zmm_kernel! (:385-396) */
+    zmm[n + 1] = 0.5 * (1.0e3 * a->zwt[c] + zmm[n]);
+    dzmm[n + 1] = jw < n ? dzmm[n] : 1.0e3 * a->zwt[c] - zmm[n];
+    /* _soilwm_tridiag_assemble_kernel! (:430-539), sdamp = 0 (:1700) */
+    const double sdamp = 0.0;
+    {
+        int j = 1;
+        qin[j] = a->qflx_infl[c];
+        double den = zmm[j + 1] - zmm[j];
+        double dzq = zq[j + 1] - zq[j];
+        double num = (smp[j + 1] - smp[j]) - dzq;
+        qout[j] = -hk[j] * num / den;
+        dqodw1[j] = -(-hk[j] * dsmpdw[j] + num * dhkdw[j]) / den;
+        dqodw2[j] = -(hk[j] * dsmpdw[j + 1] + num * dhkdw[j]) / den;
+        rmx[j] = qin[j] - qout[j] - A(a->qflx_rootsoi, c, j);
+        amx[j] = 0.0;
+        bmx[j] = dzmm[j] * (sdamp + 1.0 / dtime) + dqodw1[j];
+        cmx[j] = dqodw2[j];
+        for (j = 2; j <= n - 1; ++j) {
+            den = zmm[j] - zmm[j - 1];
+            dzq = zq[j] - zq[j - 1];
+            num = (smp[j] - smp[j - 1]) - dzq;
+            qin[j] = -hk[j - 1] * num / den;
+            dqidw0[j] = -(-hk[j - 1] * dsmpdw[j - 1] + num * dhkdw[j - 1]) / den;
+            dqidw1[j] = -(hk[j - 1] * dsmpdw[j] + num * dhkdw[j - 1]) / den;
+            den = zmm[j + 1] - zmm[j];
+            dzq = zq[j + 1] - zq[j];
+            num = (smp[j + 1] - smp[j]) - dzq;
+            qout[j] = -hk[j] * num / den;
+            dqodw1[j] = -(-hk[j] * dsmpdw[j] + num * dhkdw[j]) / den;
+            dqodw2[j] = -(hk[j] * dsmpdw[j + 1] + num * dhkdw[j]) / den;
+            rmx[j] = qin[j] - qout[j] - A(a->qflx_rootsoi, c, j);
+            amx[j] = -dqidw0[j];
+            bmx[j] = dzmm[j] / dtime - dqidw1[j] + dqodw1[j];
+            cmx[j] = dqodw2[j];
+        }
+        j = n;
+        if (j > jw) {
+            den = zmm[j] - zmm[j - 1];
+            dzq = zq[j] - zq[j - 1];
+            num = (smp[j] - smp[j - 1]) - dzq;
+            qin[j] = -hk[j - 1] * num / den;
+            dqidw0[j] = -(-hk[j - 1] * dsmpdw[j - 1] + num * dhkdw[j - 1]) / den;
+            dqidw1[j] = -(hk[j - 1] * dsmpdw[j] + num * dhkdw[j - 1]) / den;
+            qout[j] = 0.0;
+            dqodw1[j] = 0.0;
+            rmx[j] = qin[j] - qout[j] - A(a->qflx_rootsoi, c, j);
+            amx[j] = -dqidw0[j];
+            bmx[j] = dzmm[j] / dtime - dqidw1[j] + dqodw1[j];
+            cmx[j] = 0.0;
+            rmx[j + 1] = 0.0;
+            amx[j + 1] = 0.0;
+            bmx[j + 1] = dzmm[j + 1] / dtime;
+            cmx[j + 1] = 0.0;
+        } else {
+            double s_node = JMAX(0.5 * ((vwc_zwt + vwc_liq[j]) / A(a->watsat, c, j)), 0.01);
+            s_node = JMIN(1.0, s_node);
+            double smp1 = -A(a->sucsat, c, j) * zd_pow(s_node, -A(a->bsw, c, j));
+            smp1 = JMAX(smpmin, smp1);
+            const double dsmpdw1 = -A(a->bsw, c, j) * smp1 / (s_node * A(a->watsat, c, j));
+            den = zmm[j] - zmm[j - 1];
+            dzq = zq[j] - zq[j - 1];
+            num = (smp[j] - smp[j - 1]) - dzq;
+            qin[j] = -hk[j - 1] * num / den;
+            dqidw0[j] = -(-hk[j - 1] * dsmpdw[j - 1] + num * dhkdw[j - 1]) / den;
+            dqidw1[j] = -(hk[j - 1] * dsmpdw[j] + num * dhkdw[j - 1]) / den;
+            den = zmm[j + 1] - zmm[j];
+            dzq = zq[j + 1] - zq[j];
+            num = (smp1 - smp[j]) - dzq;
+            qout[j] = -hk[j] * num / den;
+            dqodw1[j] = -(-hk[j] * dsmpdw[j] + num * dhkdw[j]) / den;
+            dqodw2[j] = -(hk[j] * dsmpdw1 + num * dhkdw[j]) / den;
+            rmx[j] = qin[j] - qout[j] - A(a->qflx_rootsoi, c, j);
+            amx[j] = -dqidw0[j];
+            bmx[j] = dzmm[j] / dtime - dqidw1[j] + dqodw1[j];
+            cmx[j] = dqodw2[j];
+            qin[j + 1] = qout[j];
+            dqidw0[j + 1] = -(-hk[j] * dsmpdw[j] + num * dhkdw[j]) / den;
+            dqidw1[j + 1] = -(hk[j] * dsmpdw1 + num * dhkdw[j]) / den;
+            qout[j + 1] = 0.0;
+            dqodw1[j + 1] = 0.0;
+            rmx[j + 1] = qin[j + 1] - qout[j + 1];
+            amx[j + 1] = -dqidw0[j + 1];
+            bmx[j + 1] = dzmm[j + 1] / dtime - dqidw1[j + 1] + dqodw1[j + 1];
+            cmx[j + 1] = 0.0;
+        }
+    }
+    /* tridiagonal_multi!(dwat2, amx, bmx, cmx, rmx, jtop = 1, ..., nlevsoi + 1)
+     * (tridiagonal.jl:86-127) */
+    {
+        const int nl = n + 1;
+        const double tiny = 1e-30;
+        double bet = bmx[1];
+        if (fabs(bet) < tiny) bet = tiny;
+        double cp_prev = cmx[1] / bet, dp_prev = rmx[1] / bet;
+        cp[1] = cp_prev;
+        dp[1] = dp_prev;
+        for (int jj = 2; jj <= nl; ++jj) {
+            double denom = bmx[jj] - amx[jj] * cp_prev;
+            if (fabs(denom) < tiny) denom = copysign(tiny, denom == 0.0 ? 1.0 : denom);
+            cp_prev = cmx[jj] / denom;
+            dp_prev = (rmx[jj] - amx[jj] * dp_prev) / denom;
+            cp[jj] = cp_prev;
+            dp[jj] = dp_prev;
+        }
+        double u_next = dp_prev;
+        dwat2[nl] = u_next;
+        for (int jj = nl - 1; jj >= 1; --jj) {
+            u_next = dp[jj] - cp[jj] * u_next;
+            dwat2[jj] = u_next;
+        }
+    }
+    /* _soilwm_renew_qcharge_kernel! (:561-604) */
+    for (int j = 1; j <= n; ++j)
+        A(a->h2osoi_liq, c, joff + j) = A(a->h2osoi_liq, c, joff + j) + dwat2[j] * dzmm[j];
+    if (jw < n) {
+        const double wh_zwt = 0.0;
+        const double s_node = JMAX(A(a->h2osoi_vol, c, jw + 1) / A(a->watsat, c, jw + 1), 0.01);
+        const double s1 = JMIN(1.0, s_node);
+        const double ka = imped[jw + 1] * A(a->hksat, c, jw + 1) *
+                          zd_pow(s1, 2.0 * A(a->bsw, c, jw + 1) + 3.0);
+        const int jm = jw > 1 ? jw : 1;
+        const double smp1 = JMAX(smpmin, smp[jm]);
+        const double wh = smp1 - zq[jm];
+        double q;
+        if (jw == 0)
+            q = -ka * (wh_zwt - wh) / ((a->zwt[c] + 1.0e-3) * 1000.0);
+        else
+            q = -ka * (wh_zwt - wh) / ((a->zwt[c] - A(a->z, c, joff + jw)) * 1000.0 * 2.0);
+        q = JMAX(-10.0 / dtime, q);
+        q = JMIN(10.0 / dtime, q);
+        a->qcharge[c] = q;
+    } else {
+        a->qcharge[c] = dwat2[n + 1] * dzmm[n + 1] / dtime;
+    }
+    /* _soilwm_deficit_kernel! (:621-633) */
+    double acc = 0.0;
+    for (int j = 1; j <= n; ++j) {
+        const double l = A(a->h2osoi_liq, c, joff + j);
+        acc = acc + (l >= 0.0 ? 0.0 : -l);
+    }
+    a->qflx_deficit[c] = acc;
+}
+
+int orc_soilwater_zengdecker2009(const clmsoil_zd_args *a, const clmsoil_levels *lev,
+                                 double dtime, const clmsoil_swm_config *cfg) {
+    if (!a || !lev || !cfg) return CLMSOIL_ERR_NULL;
+    if (cfg->soilwater_movement_method != CLMSOIL_ZENGDECKER_2009) return CLMSOIL_ERR_METHOD;
+    if (lev->nlevsoi < 2 || lev->nlevsoi + 1 >= ZMAX || lev->nlevgrnd < lev->nlevsoi + 1)
+        return CLMSOIL_ERR_SIZE;
+    if (a->nc < 0 || a->ld < a->nc) return CLMSOIL_ERR_SIZE;
+    if (!(dtime > 0.0)) return CLMSOIL_ERR_ARG;
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < a->nc; ++c) {
+        if (a->mask && !a->mask[c]) continue;
+        zd09_column(a, lev, c, dtime, cfg->e_ice);
+    }
+    return CLMSOIL_OK;
+}

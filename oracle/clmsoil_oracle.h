@@ -14,6 +14,7 @@
 #define CLMSOIL_ORACLE_H
 
 #include "../include/clmsoil.h"
+#include "../include/clmsoil_zd09.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -73,6 +74,13 @@ int orc_soilwater_moisture_form(const clmsoil_mf_args *a, const clmsoil_levels *
 /* Non-urban soil_temperature!, soil_temperature.jl:358-593. */
 int orc_soil_temperature(const clmsoil_heat_args *a, const clmsoil_levels *lev,
                          double dtime, const clmsoil_heat_config *cfg);
+
+/* Test hook: a non-zero seed moves every pow() of the Zeng-Decker step one ulp
+ * (direction from a hash of the operands and the seed); 0 restores exact pow(). */
+void orc_set_zd_pow_jitter(uint64_t seed);
+/* Zeng-Decker 2009 step, soilwater_zengdecker2009! (soil_water_movement.jl:1616-1764). */
+int orc_soilwater_zengdecker2009(const clmsoil_zd_args *a, const clmsoil_levels *lev,
+                                 double dtime, const clmsoil_swm_config *cfg);
 
 /* Number of OpenMP threads the batched functions use. */
 int orc_num_threads(void);

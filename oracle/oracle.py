@@ -16,7 +16,7 @@ REPO = ORACLE_DIR.parent
 sys.path.insert(0, str(REPO / "clm.jl_amd"))
 
 from clmsoil import _abi  # noqa: E402
-from clmsoil.bind import byref, heat_args, levels, mf_args  # noqa: E402
+from clmsoil.bind import byref, heat_args, levels, mf_args, zd_args  # noqa: E402
 
 LIB_PATH = ORACLE_DIR / "liboracle_clmsoil.so"
 _lib = None
@@ -89,6 +89,24 @@ def soil_temperature(d: dict, hcfg) -> None:
     lev = levels(d)
     check(lib().orc_soil_temperature(byref(a), byref(lev), float(d["dtime"]), byref(hcfg)),
           "soil_temperature")
+
+
+def set_zd_pow_jitter(seed: int) -> None:
+    """Non-zero: every pow() of the Zeng-Decker step moves one ulp (direction
+    from a hash of its operands and `seed`); 0: exact pow()."""
+    L = lib()
+    L.orc_set_zd_pow_jitter.argtypes = [C.c_uint64]
+    L.orc_set_zd_pow_jitter.restype = None
+    L.orc_set_zd_pow_jitter(int(seed))
+
+
+def soilwater_zengdecker2009(d: dict, cfg) -> None:
+    """Zeng-Decker 2009 step on the host arrays of `d` (updated in place)."""
+    L = lib()
+    a = zd_args(d)
+    lev = levels(d)
+    check(L.orc_soilwater_zengdecker2009(byref(a), byref(lev), float(d["dtime"]), byref(cfg)),
+          "soilwater_zengdecker2009")
 
 
 def num_threads() -> int:
