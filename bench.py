@@ -119,6 +119,69 @@ def learned_vs_plain(nc, steps):
     return r
 
 
+def zd09_bytes_per_column(n):
+    """Algorithmic bytes of one Zeng-Decker 2009 column-step, each input read once
+    and each output written once (FP64): reads z, dz, zi, watsat, hksat, bsw,
+    sucsat, h2osoi_liq, h2osoi_ice, qflx_rootsoi per soil layer, smpmin, zwt,
+    qflx_infl and the two water-table-layer values (t_soisno, h2osoi_vol); writes
+    icefrac, smp_l, hk_l, h2osoi_liq per layer, qcharge and qflx_deficit."""
+    return 8 * (14 * n + 7)
+
+
+def zengdecker_2009(nc, steps, cpu_sample):
+    """The Zeng-Decker 2009 step (CLM.jl's default soil water method) on `nc`
+    columns: 20 soil layers, water table 0.3-12 m, 30 % of the columns frozen.
+    `steps` consecutive steps on the evolving state, each launch bracketed by HIP
+    events on its stream; the oracle is timed on the first `cpu_sample` columns."""
+    import numpy as np
+    import torch
+    from clmsoil import api, inputs
+    d = inputs.zengdecker_columns(nc=nc, seed=5)
+    t = api.to_device(d)
+    keys = ("z", "dz", "zi", "watsat", "hksat", "bsw", "sucsat", "smpmin", "h2osoi_liq",
+            "h2osoi_ice", "h2osoi_vol", "t_soisno", "zwt", "qflx_infl", "qflx_rootsoi",
+            "icefrac", "smp_l", "hk_l", "qcharge", "qflx_deficit")
+    state = {k: t[k] for k in keys}
+    cfg = api.init_soilwater_movement()
+    for _ in range(2):
+        api.soilwater_zengdecker2009(state, 1800.0, cfg)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    for a, b in ev:
+        a.record()
+        api.soilwater_zengdecker2009(state, 1800.0, cfg)
+        b.record()
+    torch.cuda.synchronize()
+    ms = sum(a.elapsed_time(b) for a, b in ev) / steps
+    assert bool(torch.isfinite(t["h2osoi_liq"]).all())
+    by = zd09_bytes_per_column(d["nlevsoi"]) * nc
+    out = {"workload": f"{nc} columns x {steps} steps, 20 soil layers, zwt 0.3-12 m, "
+                       f"30% frozen columns, 1800 s",
+           "ms_per_step": ms, "column_steps_per_s": nc / (ms / 1e3),
+           "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                        "bytes_per_column": zd09_bytes_per_column(d["nlevsoi"])}}
+    if cpu_sample:
+        sys.path.insert(0, str(REPO / "oracle"))
+        import oracle
+        m = min(cpu_sample, nc)
+        sub = {k: (np.ascontiguousarray(v[..., :m]) if isinstance(v, np.ndarray) else v)
+               for k, v in d.items()}
+        sub["nc"] = m
+        best = float("inf")
+        for _ in range(2):
+            s = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in sub.items()}
+            t0 = time.perf_counter()
+            for _ in range(2):
+                oracle.soilwater_zengdecker2009(s, cfg.to_abi())
+            best = min(best, time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": 2 * m / best, "unit": "column-steps/s",
+                               "cores": oracle.num_threads(), "kind": "port",
+                               "sample": f"first {m} columns x 2 steps, best of 2 ({best:.2f} s)"}
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +194,10 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-plain first-substep comparison")
+    ap.add_argument("--no-zd", action="store_true",
+                    help="skip the Zeng-Decker 2009 step section")
+    ap.add_argument("--zd-only", action="store_true",
+                    help="run only the Zeng-Decker 2009 section (for profiling its kernel)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -155,6 +222,10 @@ def main(argv=None):
         print(json.dumps({"copy_bytes_each_way": n * 8, "copies": 4}), flush=True)
         return 0
     nc = args.nc
+    if args.zd_only:
+        print(json.dumps({"zengdecker_2009": zengdecker_2009(
+            nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample)}), flush=True)
+        return 0
     host, dev, gen_s = make_state(nc, args.seed)
     print(f"# state ready: {nc} columns in {gen_s:.1f} s", file=sys.stderr, flush=True)
     from clmsoil import day, tuning
@@ -236,6 +307,9 @@ def main(argv=None):
     }
     if not args.no_learned:
         out["learned_vs_plain"] = learned_vs_plain(nc, args.steps)
+    if not args.no_zd:
+        out["zengdecker_2009"] = zengdecker_2009(
+            nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample)
     if not args.no_cpu_baseline:
         v, cores, secs = cpu_baseline(host, min(args.cpu_sample, nc), args.cpu_steps)
         out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": cores,
