@@ -73,6 +73,27 @@ class ZdArgs(C.Structure):
     _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in ZD_FIELDS]
 
 
+GHF_INT_FIELDS = ["patchi", "patchf", "patch_column", "patch_landunit", "col_itype", "snl"]
+GHF_U8_FIELDS = ["urbpoi", "mask_nolakec", "mask_nolakep"]
+GHF_FIELDS = (GHF_INT_FIELDS + GHF_U8_FIELDS + ["frac_veg_nosno"] +
+              ["emg", "t_grnd", "t_h2osfc", "forc_lwrad", "htvp", "frac_sno_eff", "t_soisno",
+               "sabg", "sabg_soil", "sabg_snow", "dlrad", "cgrnd", "eflx_sh_grnd", "eflx_sh_snow",
+               "eflx_sh_soil", "eflx_sh_h2osfc", "qflx_evap_soi", "qflx_ev_snow", "qflx_ev_soil",
+               "qflx_ev_h2osfc", "eflx_lwrad_net", "qflx_tran_veg", "wtcol", "sabg_lyr_patch",
+               "eflx_wasteheat_lun", "eflx_ventilation_lun", "eflx_heat_from_ac_lun",
+               "eflx_traffic_lun", "wtlunit_roof",
+               "eflx_gnet", "dgnetdT", "sabg_chk", "eflx_wasteheat", "eflx_ventilation",
+               "eflx_heat_from_ac", "eflx_traffic", "eflx_anthro",
+               "hs_top", "hs_top_snow", "hs_soil", "hs_h2osfc", "dhsdT", "sabg_lyr_col"])
+
+
+class GhfArgs(C.Structure):
+    """clmsoil_ghf_args (include/clmsoil_ghf.h)."""
+    _fields_ = ([(f, C.c_int64) for f in ("nc", "ldc", "np", "ldp", "nl")] +
+                [(f, _p) for f in GHF_FIELDS] +
+                [("building_temp_method", C.c_int32), ("reserved", C.c_int32)])
+
+
 class HeatConfig(C.Structure):
     _fields_ = [("snow_thermal_cond_method", C.c_int32), ("reserved", C.c_int32)]
 
@@ -87,8 +108,10 @@ def declare_common(lib: C.CDLL, prefix: str) -> C.CDLL:
     getattr(lib, prefix + "batched_band_solve").argtypes = [P, P, P, P, P, P, i32, i32, i32, i64, i64, i64] + \
         ([P] if prefix == "clmsoil_" else [])
     for f in ("tridiagonal_multi", "batched_band_solve", "soilwater_moisture_form",
-              "soil_temperature", "soilwater_zengdecker2009"):
+              "soil_temperature", "soilwater_zengdecker2009", "ground_heat_flux"):
         getattr(lib, prefix + f).restype = C.c_int
+    getattr(lib, prefix + "ground_heat_flux").argtypes = [C.POINTER(GhfArgs), C.POINTER(Levels)] + \
+        ([P] if prefix == "clmsoil_" else [])
     lib_zd = getattr(lib, prefix + "soilwater_zengdecker2009")
     lib_zd.argtypes = [C.POINTER(ZdArgs), C.POINTER(Levels), dbl, C.POINTER(SwmConfig)] + \
         ([P] if prefix == "clmsoil_" else [])

@@ -23,7 +23,7 @@ import ctypes as C
 from dataclasses import dataclass, field
 
 from . import _abi
-from .bind import byref, heat_args, heat_config, levels, mf_args, swm_config, zd_args
+from .bind import byref, ghf_args, heat_args, heat_config, levels, mf_args, swm_config, zd_args
 
 ZENGDECKER_2009, MOISTURE_FORM = _abi.ZENGDECKER_2009, _abi.MOISTURE_FORM
 BC_HEAD, BC_FLUX, BC_ZERO_FLUX, BC_WATERTABLE, BC_AQUIFER = (
@@ -186,6 +186,24 @@ def soilwater_zengdecker2009(state: dict, dtime: float, cfg: SoilWaterMovementCo
     a, lev = zd_args(d), levels(d)
     _abi.check(L, L.clmsoil_soilwater_zengdecker2009(byref(a), byref(lev), float(dtime), byref(c),
                                                      _stream()), "soilwater_zengdecker2009")
+
+
+def compute_ground_heat_flux_and_deriv(state: dict, nlevsno: int = 12, nlevgrnd: int = 25,
+                                       building_temp_method: int = 0) -> None:
+    """Ground heat flux and its temperature derivative per column
+    (compute_ground_heat_flux_and_deriv!, soil_temperature.jl:1200-1263).
+
+    `state` maps the field names of clmsoil_ghf_args (include/clmsoil_ghf.h) to
+    device tensors (index arrays int64 and 1-based, masks uint8) plus the counts
+    `nc`, `np`, `nl`; the patch and column outputs are written in place."""
+    d = dict(state)
+    d.update(nlevsno=nlevsno, nlevgrnd=nlevgrnd, nlevsoi=min(20, nlevgrnd),
+             building_temp_method=building_temp_method)
+    if int(d["nc"]) == 0 and int(d["np"]) == 0:
+        return
+    L = _lib()
+    _abi.check(L, L.clmsoil_ground_heat_flux(byref(ghf_args(d)), byref(levels(d)), _stream()),
+               "ground_heat_flux")
 
 
 def soil_water(st, mfin, mask, nbedrock, nlevsno: int, nlevsoi: int, nlevgrnd: int,

@@ -10,11 +10,11 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import (HEAT_FIELDS, MF_FIELDS, ZD_FIELDS, HeatArgs, HeatConfig, Levels, MfArgs,
-                   SwmConfig, ZdArgs)
+from ._abi import (GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_FIELDS, MF_FIELDS, ZD_FIELDS,
+                   GhfArgs, HeatArgs, HeatConfig, Levels, MfArgs, SwmConfig, ZdArgs)
 
-_INT_FIELDS = {"nbedrock", "snl", "imelt"}
-_U8_FIELDS = {"mask"}
+_INT_FIELDS = {"nbedrock", "imelt", "frac_veg_nosno"} | set(GHF_INT_FIELDS)
+_U8_FIELDS = {"mask"} | set(GHF_U8_FIELDS)
 
 
 def _ptr(x):
@@ -81,6 +81,21 @@ def zd_args(d: dict) -> ZdArgs:
         x = d.get(f)
         _check_dtype(f, x)
         setattr(a, f, _ptr(x))
+    return a
+
+
+def ghf_args(d: dict) -> GhfArgs:
+    """clmsoil_ghf_args from a dict of arrays; `nc`, `np`, `nl` and optionally
+    `ldc`/`ldp` and `building_temp_method` are integers."""
+    a = GhfArgs()
+    a.nc, a.np, a.nl = int(d["nc"]), int(d["np"]), int(d["nl"])
+    a.ldc = int(d.get("ldc", a.nc))
+    a.ldp = int(d.get("ldp", a.np))
+    for f in GHF_FIELDS:
+        x = d.get(f)
+        _check_dtype(f, x)
+        setattr(a, f, _ptr(x))
+    a.building_temp_method = int(d.get("building_temp_method", 0))
     return a
 
 

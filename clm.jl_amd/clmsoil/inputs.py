@@ -358,3 +358,98 @@ def zengdecker_columns(nc=4096, seed=4, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgr
         qflx_rootsoi=rng.uniform(0.0, 1.0e-5, (nlevsoi, nc)),
         icefrac=np.zeros((nlevgrnd, nc)), smp_l=np.zeros((nlevgrnd, nc)),
         hk_l=np.zeros((nlevgrnd, nc)), qcharge=np.zeros(nc), qflx_deficit=np.zeros(nc))
+
+
+# column_varcon.jl:7-11 and landunit_varcon.jl: soil/crop/urban/lake landunit kinds
+ICOL_ROOF, ICOL_SUNWALL, ICOL_SHADEWALL, ICOL_ROAD_IMPERV, ICOL_ROAD_PERV = 71, 72, 73, 74, 75
+ISTSOIL, ISTCROP, ISTDLAK = 1, 2, 5
+
+
+def ground_heat_flux_inputs(nc=4096, seed=6, nlevsno=NLEVSNO, nlevgrnd=NLEVGRND,
+                            max_patches=12, zero_weight_fraction=0.05,
+                            building_temp_method=0):
+    """Subgrid structure and forcing for the ground heat flux.
+
+    Columns are laid out as CLM's subgrid initialisation does (patches of a
+    column contiguous, in column order): 70 % natural-vegetation columns with
+    1..max_patches patches, 12 % crop columns with 1-2 patches, 10 % urban
+    columns (landunits of five columns: roof, sunwall, shadewall, impervious
+    and pervious road, one patch each) and 8 % lake columns (masked out). Patch
+    weights sum to one per column; a `zero_weight_fraction` of the natural
+    patches get weight 0. Snow layers 0..nlevsno per column, temperatures
+    250-300 K, radiative and turbulent fluxes in physical ranges."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kinds = rng.choice(4, size=nc, p=[0.70, 0.12, 0.10, 0.08])
+    col_itype = np.zeros(nc, dtype=np.int64)
+    col_lun = np.zeros(nc, dtype=np.int64)
+    lun_urb, lun_type = [], []
+    npatch = np.zeros(nc, dtype=np.int64)
+    n_urban = 0
+    for c in range(nc):
+        k = kinds[c]
+        if k == 2:
+            if n_urban % 5 == 0 or c == 0 or kinds[c - 1] != 2:
+                lun_urb.append(True)
+                lun_type.append(8)
+            col_itype[c] = ICOL_ROOF + n_urban % 5
+            n_urban += 1
+            npatch[c] = 1
+        else:
+            lun_urb.append(False)
+            lun_type.append({0: ISTSOIL, 1: ISTCROP, 3: ISTDLAK}[k])
+            col_itype[c] = {0: ISTSOIL, 1: ISTCROP, 3: ISTDLAK}[k]
+            npatch[c] = (rng.integers(1, max_patches + 1) if k == 0 else
+                         rng.integers(1, 3) if k == 1 else 1)
+        col_lun[c] = len(lun_urb)
+    nl = len(lun_urb)
+    npt = int(npatch.sum())
+    patchf = np.cumsum(npatch)
+    patchi = patchf - npatch + 1
+    patch_column = np.repeat(np.arange(1, nc + 1, dtype=np.int64), npatch)
+    patch_landunit = col_lun[patch_column - 1].astype(np.int64)
+    mask_nolakec = (kinds != 3).astype(np.uint8)
+    mask_nolakep = mask_nolakec[patch_column - 1].copy()
+    # weights: random simplex per column, some natural patches at zero weight
+    w = rng.gamma(1.0, 1.0, npt)
+    nat = kinds[patch_column - 1] == 0
+    zw = nat & (rng.uniform(size=npt) < zero_weight_fraction) & (npatch[patch_column - 1] > 1)
+    w[zw] = 0.0
+    sums = np.bincount(patch_column - 1, weights=w, minlength=nc)
+    w[patchi[sums == 0] - 1] = 1.0                 # keep one weighted patch per column
+    sums = np.bincount(patch_column - 1, weights=w, minlength=nc)
+    wtcol = w / np.where(sums[patch_column - 1] > 0, sums[patch_column - 1], 1.0)
+    snl = -rng.integers(0, nlevsno + 1, nc).astype(np.int64)
+    snl[kinds >= 2] = 0
+    nlev = nlevsno + nlevgrnd
+    t_soisno = rng.uniform(250.0, 300.0, (nlev, nc))
+    u = lambda lo, hi, n: rng.uniform(lo, hi, n)  # noqa: E731
+    d = dict(
+        nc=nc, np=npt, nl=nl, nlevsno=nlevsno, nlevsoi=min(20, nlevgrnd), nlevgrnd=nlevgrnd,
+        building_temp_method=building_temp_method,
+        patchi=patchi.astype(np.int64), patchf=patchf.astype(np.int64),
+        patch_column=patch_column, patch_landunit=patch_landunit, col_itype=col_itype, snl=snl,
+        urbpoi=np.array(lun_urb, dtype=np.uint8), mask_nolakec=mask_nolakec,
+        mask_nolakep=mask_nolakep,
+        frac_veg_nosno=rng.integers(0, 2, npt).astype(np.int64),
+        emg=u(0.9, 1.0, nc), t_grnd=u(250.0, 300.0, nc), t_h2osfc=u(265.0, 300.0, nc),
+        forc_lwrad=u(200.0, 420.0, nc), htvp=np.where(rng.uniform(size=nc) < 0.3, 2.8440e6, 2.501e6),
+        frac_sno_eff=np.where(snl < 0, u(0.2, 1.0, nc), u(0.0, 0.2, nc)), t_soisno=t_soisno,
+        sabg=u(0.0, 600.0, npt), sabg_soil=u(0.0, 600.0, npt), sabg_snow=u(0.0, 400.0, npt),
+        dlrad=u(0.0, 120.0, npt), cgrnd=u(0.0, 60.0, npt),
+        eflx_sh_grnd=u(-100.0, 250.0, npt), eflx_sh_snow=u(-100.0, 250.0, npt),
+        eflx_sh_soil=u(-100.0, 250.0, npt), eflx_sh_h2osfc=u(-100.0, 250.0, npt),
+        qflx_evap_soi=u(-5e-5, 2e-4, npt), qflx_ev_snow=u(-5e-5, 2e-4, npt),
+        qflx_ev_soil=u(-5e-5, 2e-4, npt), qflx_ev_h2osfc=u(-5e-5, 2e-4, npt),
+        eflx_lwrad_net=u(0.0, 150.0, npt), qflx_tran_veg=u(0.0, 1e-4, npt), wtcol=wtcol,
+        sabg_lyr_patch=rng.uniform(0.0, 300.0, (nlevsno + 1, npt)),
+        eflx_wasteheat_lun=u(0.0, 30.0, nl), eflx_ventilation_lun=u(-20.0, 20.0, nl),
+        eflx_heat_from_ac_lun=u(0.0, 10.0, nl), eflx_traffic_lun=u(0.0, 20.0, nl),
+        wtlunit_roof=u(0.1, 0.6, nl))
+    for k in ("eflx_gnet", "dgnetdT", "sabg_chk", "eflx_wasteheat", "eflx_heat_from_ac",
+              "eflx_traffic", "eflx_anthro"):
+        d[k] = np.zeros(npt)
+    d["eflx_ventilation"] = u(-5.0, 5.0, npt)     # kept when neither building method is set
+    for k in ("hs_top", "hs_top_snow", "hs_soil", "hs_h2osfc", "dhsdT"):
+        d[k] = np.zeros(nc)
+    d["sabg_lyr_col"] = np.zeros((nlevsno + 1, nc))
+    return d

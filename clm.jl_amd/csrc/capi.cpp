@@ -16,12 +16,14 @@
 #include "../../include/clmsoil_ext.h"
 #include "../../include/clmsoil_adjoint.h"
 #include "../../include/clmsoil_zd09.h"
+#include "../../include/clmsoil_ghf.h"
 #include "common.hpp"
 #include "heat.hpp"
 #include "richards.hpp"
 #include "solvers.hpp"
 #include "state.hpp"
 #include "zd09.hpp"
+#include "ghf.hpp"
 
 using namespace clmsoil;
 
@@ -362,6 +364,38 @@ int clmsoil_soilwater_zengdecker2009(const clmsoil_zd_args *a, const clmsoil_lev
                a->hk_l, a->qcharge, a->qflx_deficit, lev->nlevsno, lev->nlevsoi,
                lev->nlevgrnd, dtime, cfg->e_ice};
     return launch_zengdecker2009(P, g_tuning.zd_block, (hipStream_t)stream) == hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
+}
+
+// compute_ground_heat_flux_and_deriv! (soil_temperature.jl:1200-1263).
+int clmsoil_ground_heat_flux(const clmsoil_ghf_args *a, const clmsoil_levels *lev,
+                             void *stream) {
+    if (!a || !lev) return CLMSOIL_ERR_NULL;
+    if (lev->nlevsno < 1 || lev->nlevsno > MAX_NLEVSNO || lev->nlevgrnd < 1)
+        return CLMSOIL_ERR_SIZE;
+    if (a->nc < 0 || a->np < 0 || a->nl < 0 || a->ldc < a->nc || a->ldp < a->np)
+        return CLMSOIL_ERR_SIZE;
+    if (a->nc == 0 && a->np == 0) return CLMSOIL_OK;
+    const void *req[] = {
+        a->patchi, a->patchf, a->patch_column, a->patch_landunit, a->col_itype, a->snl,
+        a->urbpoi, a->mask_nolakec, a->mask_nolakep, a->frac_veg_nosno, a->emg, a->t_grnd,
+        a->t_h2osfc, a->forc_lwrad, a->htvp, a->frac_sno_eff, a->t_soisno, a->sabg,
+        a->sabg_soil, a->sabg_snow, a->dlrad, a->cgrnd, a->eflx_sh_grnd, a->eflx_sh_snow,
+        a->eflx_sh_soil, a->eflx_sh_h2osfc, a->qflx_evap_soi, a->qflx_ev_snow, a->qflx_ev_soil,
+        a->qflx_ev_h2osfc, a->eflx_lwrad_net, a->qflx_tran_veg, a->wtcol, a->sabg_lyr_patch,
+        a->eflx_wasteheat_lun, a->eflx_ventilation_lun, a->eflx_heat_from_ac_lun,
+        a->eflx_traffic_lun, a->wtlunit_roof, a->eflx_gnet, a->dgnetdT, a->sabg_chk,
+        a->eflx_wasteheat, a->eflx_ventilation, a->eflx_heat_from_ac, a->eflx_traffic,
+        a->eflx_anthro, a->hs_top, a->hs_top_snow, a->hs_soil, a->hs_h2osfc, a->dhsdT,
+        a->sabg_lyr_col};
+    for (const void *q : req)
+        if (!q) return CLMSOIL_ERR_NULL;
+    DeviceState *s = nullptr;
+    int st = current_device_state(&s);
+    if (st) return st;
+    GhfParams P{*a, lev->nlevsno, s->err};
+    return launch_ground_heat_flux(P, g_tuning.ghf_block, (hipStream_t)stream) == hipSuccess
                ? CLMSOIL_OK
                : CLMSOIL_ERR_HIP;
 }

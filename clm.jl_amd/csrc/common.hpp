@@ -135,6 +135,7 @@ enum DeviceError : unsigned {
     DEV_ERR_SNL = 2u,
     DEV_ERR_ITER = 4u,
     DEV_ERR_TAPE = 8u,
+    DEV_ERR_SUBGRID = 16u,
 };
 
 }  // namespace clmsoil

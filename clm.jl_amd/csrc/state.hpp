@@ -16,6 +16,7 @@ struct Tuning {
     int richards_persistent = 1;   // resident grid with per-group column refill
     int richards_blocks_per_cu = 0;  // persistent grid size; 0 = occupancy query
     int zd_block = 128;              // Zeng-Decker 2009 step, one lane per column
+    int ghf_block = 256;             // ground heat flux
 };
 extern Tuning g_tuning;
 

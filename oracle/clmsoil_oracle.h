@@ -15,6 +15,7 @@
 
 #include "../include/clmsoil.h"
 #include "../include/clmsoil_zd09.h"
+#include "../include/clmsoil_ghf.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -75,6 +76,11 @@ int orc_soilwater_moisture_form(const clmsoil_mf_args *a, const clmsoil_levels *
 int orc_soil_temperature(const clmsoil_heat_args *a, const clmsoil_levels *lev,
                          double dtime, const clmsoil_heat_config *cfg);
 
+/* Julia's x^4 for Float64 (Base.Math.pow_body: compensated power by squaring). */
+double orc_jl_pow4(double x);
+/* Ground heat flux, compute_ground_heat_flux_and_deriv! (soil_temperature.jl:1200-1263),
+ * in ghf_oracle.c. */
+int orc_ground_heat_flux(const clmsoil_ghf_args *a, const clmsoil_levels *lev);
 /* Test hook: a non-zero seed moves every pow() of the Zeng-Decker step one ulp
  * (direction from a hash of the operands and the seed); 0 restores exact pow(). */
 void orc_set_zd_pow_jitter(uint64_t seed);

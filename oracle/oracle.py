@@ -16,7 +16,7 @@ REPO = ORACLE_DIR.parent
 sys.path.insert(0, str(REPO / "clm.jl_amd"))
 
 from clmsoil import _abi  # noqa: E402
-from clmsoil.bind import byref, heat_args, levels, mf_args, zd_args  # noqa: E402
+from clmsoil.bind import byref, ghf_args, heat_args, levels, mf_args, zd_args  # noqa: E402
 
 LIB_PATH = ORACLE_DIR / "liboracle_clmsoil.so"
 _lib = None
@@ -89,6 +89,19 @@ def soil_temperature(d: dict, hcfg) -> None:
     lev = levels(d)
     check(lib().orc_soil_temperature(byref(a), byref(lev), float(d["dtime"]), byref(hcfg)),
           "soil_temperature")
+
+
+def ground_heat_flux(d: dict) -> None:
+    """compute_ground_heat_flux_and_deriv! on the host arrays of `d` (outputs in place)."""
+    L = lib()
+    check(L.orc_ground_heat_flux(byref(ghf_args(d)), byref(levels(d))), "ground_heat_flux")
+
+
+def jl_pow4(x: float) -> float:
+    L = lib()
+    L.orc_jl_pow4.argtypes = [C.c_double]
+    L.orc_jl_pow4.restype = C.c_double
+    return L.orc_jl_pow4(float(x))
 
 
 def set_zd_pow_jitter(seed: int) -> None:
