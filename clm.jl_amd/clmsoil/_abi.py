@@ -94,6 +94,20 @@ class GhfArgs(C.Structure):
                 [("building_temp_method", C.c_int32), ("reserved", C.c_int32)])
 
 
+SEB_FIELDS = ["patchi", "patchf", "patch_column", "col_itype", "snl", "mask_nolakec",
+              "mask_nolakep", "mask_urbanp", "eflx_soil_grnd", "wtcol", "xmf", "xmf_h2osfc",
+              "frac_h2osfc", "t_h2osfc", "t_h2osfc_bef", "c_h2osfc", "eflx_h2osfc_to_snow",
+              "eflx_building_heat_errsoi", "frac_sno_eff", "t_soisno", "t_ssbef", "fact",
+              "errsoi_patch", "errsoi_col"]
+
+
+class SebArgs(C.Structure):
+    """clmsoil_seb_args (include/clmsoil_ghf.h)."""
+    _fields_ = ([(f, C.c_int64) for f in ("nc", "ldc", "np")] +
+                [("nlevurb", C.c_int32), ("reserved", C.c_int32)] +
+                [(f, _p) for f in SEB_FIELDS])
+
+
 class HeatConfig(C.Structure):
     _fields_ = [("snow_thermal_cond_method", C.c_int32), ("reserved", C.c_int32)]
 
@@ -112,6 +126,9 @@ def declare_common(lib: C.CDLL, prefix: str) -> C.CDLL:
         getattr(lib, prefix + f).restype = C.c_int
     getattr(lib, prefix + "ground_heat_flux").argtypes = [C.POINTER(GhfArgs), C.POINTER(Levels)] + \
         ([P] if prefix == "clmsoil_" else [])
+    f = getattr(lib, prefix + "soil_energy_balance")
+    f.argtypes = [C.POINTER(SebArgs), C.POINTER(Levels), dbl] + ([P] if prefix == "clmsoil_" else [])
+    f.restype = C.c_int
     lib_zd = getattr(lib, prefix + "soilwater_zengdecker2009")
     lib_zd.argtypes = [C.POINTER(ZdArgs), C.POINTER(Levels), dbl, C.POINTER(SwmConfig)] + \
         ([P] if prefix == "clmsoil_" else [])

@@ -23,7 +23,8 @@ import ctypes as C
 from dataclasses import dataclass, field
 
 from . import _abi
-from .bind import byref, ghf_args, heat_args, heat_config, levels, mf_args, swm_config, zd_args
+from .bind import (byref, ghf_args, heat_args, heat_config, levels, mf_args, seb_args, swm_config,
+                   zd_args)
 
 ZENGDECKER_2009, MOISTURE_FORM = _abi.ZENGDECKER_2009, _abi.MOISTURE_FORM
 BC_HEAD, BC_FLUX, BC_ZERO_FLUX, BC_WATERTABLE, BC_AQUIFER = (
@@ -204,6 +205,21 @@ def compute_ground_heat_flux_and_deriv(state: dict, nlevsno: int = 12, nlevgrnd:
     L = _lib()
     _abi.check(L, L.clmsoil_ground_heat_flux(byref(ghf_args(d)), byref(levels(d)), _stream()),
                "ground_heat_flux")
+
+
+def soil_energy_balance(state: dict, dtime: float, nlevsno: int = 12, nlevgrnd: int = 25,
+                        nlevurb: int = 0) -> None:
+    """errsoi_patch and errsoi_col of soil_fluxes! (soil_fluxes.jl:380-447, 499-525).
+
+    `state` maps the field names of clmsoil_seb_args (include/clmsoil_ghf.h) to
+    device tensors plus the counts `nc` and `np`."""
+    d = dict(state)
+    d.update(nlevsno=nlevsno, nlevgrnd=nlevgrnd, nlevsoi=min(20, nlevgrnd), nlevurb=nlevurb)
+    if int(d["nc"]) == 0:
+        return
+    L = _lib()
+    _abi.check(L, L.clmsoil_soil_energy_balance(byref(seb_args(d)), byref(levels(d)), float(dtime),
+                                                _stream()), "soil_energy_balance")
 
 
 def soil_water(st, mfin, mask, nbedrock, nlevsno: int, nlevsoi: int, nlevgrnd: int,

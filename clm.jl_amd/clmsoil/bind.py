@@ -10,11 +10,12 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import (GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_FIELDS, MF_FIELDS, ZD_FIELDS,
-                   GhfArgs, HeatArgs, HeatConfig, Levels, MfArgs, SwmConfig, ZdArgs)
+from ._abi import (GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_FIELDS, MF_FIELDS, SEB_FIELDS,
+                   ZD_FIELDS, GhfArgs, HeatArgs, HeatConfig, Levels, MfArgs, SebArgs, SwmConfig,
+                   ZdArgs)
 
 _INT_FIELDS = {"nbedrock", "imelt", "frac_veg_nosno"} | set(GHF_INT_FIELDS)
-_U8_FIELDS = {"mask"} | set(GHF_U8_FIELDS)
+_U8_FIELDS = {"mask", "mask_urbanp"} | set(GHF_U8_FIELDS)
 
 
 def _ptr(x):
@@ -96,6 +97,19 @@ def ghf_args(d: dict) -> GhfArgs:
         _check_dtype(f, x)
         setattr(a, f, _ptr(x))
     a.building_temp_method = int(d.get("building_temp_method", 0))
+    return a
+
+
+def seb_args(d: dict) -> SebArgs:
+    """clmsoil_seb_args from a dict of arrays (`nc`, `np`, `nlevurb` integers)."""
+    a = SebArgs()
+    a.nc, a.np = int(d["nc"]), int(d["np"])
+    a.ldc = int(d.get("ldc", a.nc))
+    a.nlevurb = int(d.get("nlevurb", 0))
+    for f in SEB_FIELDS:
+        x = d.get(f)
+        _check_dtype(f, x)
+        setattr(a, f, _ptr(x))
     return a
 
 

@@ -453,3 +453,112 @@ def ground_heat_flux_inputs(nc=4096, seed=6, nlevsno=NLEVSNO, nlevgrnd=NLEVGRND,
         d[k] = np.zeros(nc)
     d["sabg_lyr_col"] = np.zeros((nlevsno + 1, nc))
     return d
+
+
+def energy_balance_inputs(nc=400, seed=14, nlevsno=NLEVSNO, nlevgrnd=NLEVGRND, nlevurb=10):
+    """Random multi-patch columns for the soil energy balance error alone: 1-4
+    patches per column, 30 % urban column types (71-75), 10 % of the patches
+    outside the non-lake mask (their prior errsoi is kept), 10 % of the columns
+    unmasked, random fluxes, temperatures, factors and prior errsoi values."""
+    rng = np.random.default_rng(seed)
+    npc = rng.integers(1, 5, nc)
+    patchf = np.cumsum(npc).astype(np.int64)
+    patchi = (patchf - npc + 1).astype(np.int64)
+    np_ = int(patchf[-1])
+    col = np.repeat(np.arange(1, nc + 1), npc).astype(np.int64)
+    itype = np.where(rng.uniform(size=nc) < 0.3, rng.integers(71, 76, nc), 1).astype(np.int64)
+    mask_p = np.ones(np_, np.uint8)
+    mask_p[rng.uniform(size=np_) < 0.1] = 0
+    mask_u = (np.isin(itype[col - 1], (71, 72, 73, 74, 75))).astype(np.uint8)
+    nl = nlevsno + max(nlevgrnd, nlevurb)
+    return dict(
+        nc=nc, np=np_, nlevsno=nlevsno, nlevsoi=min(20, nlevgrnd), nlevgrnd=nlevgrnd,
+        nlevurb=nlevurb, patchi=patchi, patchf=patchf, patch_column=col, col_itype=itype,
+        snl=-rng.integers(0, nlevsno + 1, nc).astype(np.int64),
+        mask_nolakec=(rng.uniform(size=nc) < 0.9).astype(np.uint8),
+        mask_nolakep=mask_p, mask_urbanp=mask_u,
+        eflx_soil_grnd=rng.normal(50, 100, np_), wtcol=rng.uniform(0, 1, np_),
+        xmf=rng.normal(0, 50, nc), xmf_h2osfc=rng.normal(0, 5, nc),
+        frac_h2osfc=rng.uniform(0, 0.3, nc), t_h2osfc=rng.uniform(270, 280, nc),
+        t_h2osfc_bef=rng.uniform(270, 280, nc), c_h2osfc=rng.uniform(1e3, 1e5, nc),
+        eflx_h2osfc_to_snow=rng.normal(0, 3, nc),
+        eflx_building_heat_errsoi=rng.normal(0, 10, nc),
+        frac_sno_eff=rng.uniform(0, 1, nc), t_soisno=rng.uniform(260, 290, (nl, nc)),
+        t_ssbef=rng.uniform(260, 290, (nl, nc)), fact=rng.uniform(1e-4, 1e-2, (nl, nc)),
+        errsoi_patch=rng.normal(0, 1, np_), errsoi_col=np.full(nc, -7.0))
+
+
+def coupled_surface_columns(nc=2000, seed=15, nlevsno=NLEVSNO, nlevgrnd=NLEVGRND):
+    """Heat-step columns (config 1 with surface water, partial snow cover) and a
+    patch layer over them for the ground heat flux: the subgrid and patch
+    forcing of ground_heat_flux_inputs with every column vegetated or crop and
+    unmasked, and the column temperatures, snow layers and fractions taken from
+    the heat columns (t_grnd the area-weighted surface temperature). Returns
+    (heat, ghf) dicts that share the column arrays."""
+    h = heat_config1(nc=nc, seed=seed, nlevsno=nlevsno, nlevgrnd=nlevgrnd, h2osfc_variant=True)
+    h.update(heat_outputs(h))
+    rng = np.random.default_rng(seed + 7)
+    ns = nlevsno
+    h["frac_sno_eff"] = np.where(h["snl"] < 0, rng.uniform(0.3, 0.7, nc), 0.0)
+    g = ground_heat_flux_inputs(nc=nc, seed=seed, nlevsno=nlevsno, nlevgrnd=nlevgrnd)
+    g["urbpoi"][:] = 0
+    g["col_itype"][:] = ISTSOIL
+    g["mask_nolakec"][:] = 1
+    g["mask_nolakep"][:] = 1
+    cols = np.arange(nc)
+    top = h["snl"] + ns
+    fs, fw = h["frac_sno_eff"], h["frac_h2osfc"]
+    g["snl"] = h["snl"]
+    g["t_soisno"] = np.nan_to_num(h["t_soisno"])
+    g["t_h2osfc"] = h["t_h2osfc"]
+    g["frac_sno_eff"] = fs
+    g["t_grnd"] = (fs * g["t_soisno"][top, cols] + (1.0 - fs - fw) * g["t_soisno"][ns] +
+                   fw * h["t_h2osfc"])
+    return h, g
+
+
+def applied_ground_flux(h0, h1):
+    """The ground heat flux a heat step applied, per column, from its inputs h0
+    and outputs h1: over the snow fraction the top-layer flux hs_top_snow +
+    dhsdT*dT_top plus the solar absorbed in the lower snow layers and soil layer
+    1; over bare soil hs_soil + dhsdT*dT_1; over surface water hs_h2osfc +
+    dhsdT*dT_h2osfc (soil_temperature.jl set_rhs_vec!, :1365-1542). Without snow
+    layers the top layer is soil layer 1 and takes hs_top_snow."""
+    ns, nc = h0["nlevsno"], h0["nc"]
+    cols = np.arange(nc)
+    top = h0["snl"] + ns
+    t0, t1 = h0["t_soisno"], h1["t_soisno"]
+    g = h0["dhsdT"]
+    dt_top = t1[top, cols] - t0[top, cols]
+    dt_1 = t1[ns] - t0[ns]
+    dt_h = h1["t_h2osfc"] - h0["t_h2osfc"]
+    fs, fw = h0["frac_sno_eff"], h0["frac_h2osfc"]
+    sab = h0["sabg_lyr_col"]
+    rows = np.arange(ns + 1)[:, None]                  # level k+1 is layer j = k+1-ns
+    below_top = (rows > top[None, :]) & (rows <= ns)
+    sab_lower = np.where(below_top, sab, 0.0).sum(0)
+    snow = (fs * (h0["hs_top_snow"] + g * dt_top + sab_lower) +
+            (1.0 - fs - fw) * (h0["hs_soil"] + g * dt_1) + fw * (h0["hs_h2osfc"] + g * dt_h))
+    bare = (h0["hs_top_snow"] + g * dt_1 - fw * (h0["hs_soil"] + g * dt_1) +
+            fw * (h0["hs_h2osfc"] + g * dt_h))
+    return np.where(h0["snl"] < 0, snow, bare)
+
+
+def energy_balance_from_heat(h0, h1):
+    """clmsoil_seb_args fields for one patch of weight 1 per column, the applied
+    ground flux as eflx_soil_grnd and the heat step's own outputs."""
+    nc, ns = h0["nc"], h0["nlevsno"]
+    one = np.arange(1, nc + 1, dtype=np.int64)
+    return dict(
+        nc=nc, np=nc, nlevsno=ns, nlevsoi=h0["nlevsoi"], nlevgrnd=h0["nlevgrnd"], nlevurb=0,
+        patchi=one, patchf=one.copy(), patch_column=one.copy(),
+        col_itype=np.full(nc, ISTSOIL, dtype=np.int64), snl=h0["snl"],
+        mask_nolakec=np.ones(nc, np.uint8), mask_nolakep=np.ones(nc, np.uint8),
+        mask_urbanp=np.zeros(nc, np.uint8), eflx_soil_grnd=applied_ground_flux(h0, h1),
+        wtcol=np.ones(nc), xmf=h1["xmf"], xmf_h2osfc=h1["xmf_h2osfc"],
+        frac_h2osfc=h1["frac_h2osfc"], t_h2osfc=h1["t_h2osfc"], t_h2osfc_bef=h0["t_h2osfc"],
+        c_h2osfc=h1["c_h2osfc"], eflx_h2osfc_to_snow=h1["eflx_h2osfc_to_snow"],
+        eflx_building_heat_errsoi=np.zeros(nc), frac_sno_eff=h0["frac_sno_eff"],
+        t_soisno=np.nan_to_num(h1["t_soisno"]), t_ssbef=np.nan_to_num(h0["t_soisno"]),
+        fact=np.where(np.isnan(h1["fact"]), 1.0, h1["fact"]),
+        errsoi_patch=np.zeros(nc), errsoi_col=np.zeros(nc))

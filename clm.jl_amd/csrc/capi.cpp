@@ -400,4 +400,30 @@ int clmsoil_ground_heat_flux(const clmsoil_ghf_args *a, const clmsoil_levels *le
                : CLMSOIL_ERR_HIP;
 }
 
+// errsoi of soil_fluxes! (soil_fluxes.jl:380-447, 499-525).
+int clmsoil_soil_energy_balance(const clmsoil_seb_args *a, const clmsoil_levels *lev,
+                                double dtime, void *stream) {
+    if (!a || !lev) return CLMSOIL_ERR_NULL;
+    if (lev->nlevsno < 1 || lev->nlevsno > MAX_NLEVSNO || lev->nlevgrnd < 1 || a->nlevurb < 0)
+        return CLMSOIL_ERR_SIZE;
+    if (a->nc < 0 || a->np < 0 || a->ldc < a->nc) return CLMSOIL_ERR_SIZE;
+    if (!(dtime > 0.0) || !std::isfinite(dtime)) return CLMSOIL_ERR_ARG;
+    if (a->nc == 0) return CLMSOIL_OK;
+    const void *req[] = {a->patchi, a->patchf, a->patch_column, a->col_itype, a->snl,
+                         a->mask_nolakec, a->mask_nolakep, a->mask_urbanp, a->eflx_soil_grnd,
+                         a->wtcol, a->xmf, a->xmf_h2osfc, a->frac_h2osfc, a->t_h2osfc,
+                         a->t_h2osfc_bef, a->c_h2osfc, a->eflx_h2osfc_to_snow,
+                         a->eflx_building_heat_errsoi, a->frac_sno_eff, a->t_soisno, a->t_ssbef,
+                         a->fact, a->errsoi_patch, a->errsoi_col};
+    for (const void *q : req)
+        if (!q) return CLMSOIL_ERR_NULL;
+    DeviceState *s = nullptr;
+    int st = current_device_state(&s);
+    if (st) return st;
+    return launch_soil_energy_balance(*a, lev->nlevsno, lev->nlevgrnd, dtime, s->err,
+                                      (hipStream_t)stream) == hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
+}
+
 }  // extern "C"

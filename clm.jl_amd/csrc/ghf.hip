@@ -224,7 +224,82 @@ __global__ void __launch_bounds__(256) ghf_kernel(GhfParams P) {
     if (bad) atomicOr(P.err, bad);
 }
 
+// ---- soil energy balance error (soil_fluxes.jl:380-447, 499-525)
+
+__device__ __forceinline__ bool is_wall_or_roof(int64_t it) {
+    return it == 72 || it == 73 || it == 71;   // ICOL_SUNWALL, ICOL_SHADEWALL, ICOL_ROOF
+}
+
+// errsoi of patch p (column c) after the base kernel and both layer passes, in
+// the reference's statement order; `e` enters as the patch's prior value.
+__device__ __forceinline__ double patch_errsoi(const clmsoil_seb_args &a, int64_t p, int64_t c,
+                                               int nlevsno, int nlevgrnd, double dtime,
+                                               double e, bool &written) {
+    const bool mp = a.mask_nolakep[p], wall = is_wall_or_roof(a.col_itype[c]);
+    const bool mu = a.mask_urbanp[p];
+    written = mp || (mu && wall);
+    if (mp) {
+        e = a.eflx_soil_grnd[p] - a.xmf[c] - a.xmf_h2osfc[c] -
+            a.frac_h2osfc[c] * (a.t_h2osfc[c] - a.t_h2osfc_bef[c]) * (a.c_h2osfc[c] / dtime);
+        e += a.eflx_h2osfc_to_snow[c];
+        if (wall) e += a.eflx_building_heat_errsoi[c];
+    }
+    const bool pass = (mp && !wall) || (mu && wall);
+    if (pass) {
+        const int jmax = wall ? a.nlevurb : nlevgrnd;
+        const int64_t ld = a.ldc;
+        const int64_t snl = a.snl[c];
+        for (int jf = -nlevsno + 1; jf <= jmax; ++jf) {
+            const int64_t o = c + (int64_t)(jf + nlevsno - 1) * ld;
+            const double dt = a.t_soisno[o] - a.t_ssbef[o];
+            if (jf >= snl + 1 && jf < 1) e -= a.frac_sno_eff[c] * dt / a.fact[o];
+            if (jf >= 1) e -= dt / a.fact[o];
+        }
+    }
+    return e;
+}
+
+__global__ void __launch_bounds__(256) seb_kernel(clmsoil_seb_args a, int nlevsno, int nlevgrnd,
+                                                  double dtime, unsigned *err) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nc) return;
+    unsigned bad = 0u;
+    const int64_t p0 = a.patchi[c], p1 = a.patchf[c];
+    const bool has = p0 >= 1 && p1 >= p0 && p1 <= a.np;
+    if (!has && p0 != -9999 && !(p0 >= 1 && p1 == p0 - 1)) bad |= DEV_ERR_SUBGRID;
+    const int64_t s = a.snl[c];
+    const bool snl_ok = s >= -nlevsno && s <= 0;
+    const bool mc = a.mask_nolakec[c];
+    double acc = 0.0;
+    for (int64_t p = has ? p0 - 1 : 0; has && p < p1; ++p) {
+        if (a.patch_column[p] != c + 1) {
+            bad |= DEV_ERR_SUBGRID;
+            continue;
+        }
+        bool written = false;
+        double e = a.errsoi_patch[p];
+        if (snl_ok) {
+            e = patch_errsoi(a, p, c, nlevsno, nlevgrnd, dtime, e, written);
+        } else if (a.mask_nolakep[p] || a.mask_urbanp[p]) {
+            bad |= DEV_ERR_SNL;
+        }
+        if (written) a.errsoi_patch[p] = e;
+        if (mc) acc += e * a.wtcol[p];
+    }
+    if (mc) a.errsoi_col[c] = acc;
+    if (bad) atomicOr(err, bad);
+}
+
 }  // namespace
+
+hipError_t launch_soil_energy_balance(const clmsoil_seb_args &a, int nlevsno, int nlevgrnd,
+                                      double dtime, unsigned *err, hipStream_t stream) {
+    const int block = 256;
+    const int64_t grid = (a.nc + block - 1) / block;
+    hipLaunchKernelGGL(seb_kernel, dim3((unsigned)grid), dim3(block), 0, stream, a, nlevsno,
+                       nlevgrnd, dtime, err);
+    return hipGetLastError();
+}
 
 hipError_t launch_ground_heat_flux(const GhfParams &P, int block, hipStream_t stream) {
     if (block != 64 && block != 128 && block != 256) block = 256;

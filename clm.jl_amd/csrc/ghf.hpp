@@ -14,5 +14,7 @@ struct GhfParams {
 };
 
 hipError_t launch_ground_heat_flux(const GhfParams &P, int block, hipStream_t stream);
+hipError_t launch_soil_energy_balance(const clmsoil_seb_args &a, int nlevsno, int nlevgrnd,
+                                      double dtime, unsigned *err, hipStream_t stream);
 
 }  // namespace clmsoil

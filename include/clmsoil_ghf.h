@@ -1,5 +1,6 @@
 /* clmsoil_ghf.h — ground heat flux and its temperature derivative, the input of
- * the heat step (clmsoil_soil_temperature).
+ * the heat step (clmsoil_soil_temperature), and the soil energy balance error
+ * computed after it.
  *
  * Replaces compute_ground_heat_flux_and_deriv! (src/biogeophys/soil_temperature.jl:
  * 1200-1263) and the kernels it launches: longwave emission per column (:944-960),
@@ -75,6 +76,38 @@ typedef struct clmsoil_ghf_args {
  * matrices; nlevsoi is not used. */
 int clmsoil_ground_heat_flux(const clmsoil_ghf_args *a, const clmsoil_levels *lev,
                              void *stream);
+
+/* ---- soil energy balance error --------------------------------------------
+ * Replaces the errsoi part of soil_fluxes! (src/biogeophys/soil_fluxes.jl:380-447,
+ * 499-525): per patch, the ground heat flux minus the phase-change and
+ * surface-water heat terms minus the heat stored in every snow and soil layer,
+ * sum_j (t_soisno - t_ssbef)/fact (snow layers weighted by frac_sno_eff; wall and
+ * roof columns use their own layer count nlevurb); then the weighted patch-to-
+ * column average over masked columns. As above, each column walks its own patch
+ * range patchi..patchf in ascending order (no atomics, the order of the
+ * reference's CPU path) and writes the errsoi of those patches. */
+typedef struct clmsoil_seb_args {
+    int64_t nc, ldc, np;
+    int32_t nlevurb;                     /* layers of wall and roof columns           */
+    int32_t reserved;
+    const int64_t *patchi, *patchf;      /* [nc] 1-based patch range of each column   */
+    const int64_t *patch_column;         /* [np] 1-based                              */
+    const int64_t *col_itype, *snl;      /* [nc]                                      */
+    const uint8_t *mask_nolakec;         /* [nc]                                      */
+    const uint8_t *mask_nolakep, *mask_urbanp;   /* [np]                              */
+    const double *eflx_soil_grnd, *wtcol;        /* [np] W/m2, weight in column       */
+    const double *xmf, *xmf_h2osfc;              /* [nc] phase-change heat, W/m2      */
+    const double *frac_h2osfc, *t_h2osfc, *t_h2osfc_bef, *c_h2osfc;   /* [nc]        */
+    const double *eflx_h2osfc_to_snow, *eflx_building_heat_errsoi;    /* [nc] W/m2   */
+    const double *frac_sno_eff;                  /* [nc]                              */
+    /* [ldc, nlevsno + max(nlevgrnd, nlevurb)] */
+    const double *t_soisno, *t_ssbef, *fact;
+    double *errsoi_patch;                /* [np] (patches that no pass touches keep their value) */
+    double *errsoi_col;                  /* [nc] written for masked columns           */
+} clmsoil_seb_args;
+
+int clmsoil_soil_energy_balance(const clmsoil_seb_args *a, const clmsoil_levels *lev,
+                                double dtime, void *stream);
 
 #ifdef __cplusplus
 }

@@ -81,6 +81,9 @@ double orc_jl_pow4(double x);
 /* Ground heat flux, compute_ground_heat_flux_and_deriv! (soil_temperature.jl:1200-1263),
  * in ghf_oracle.c. */
 int orc_ground_heat_flux(const clmsoil_ghf_args *a, const clmsoil_levels *lev);
+/* Soil energy balance error, soil_fluxes.jl:380-447, 499-525 (ghf_oracle.c). */
+int orc_soil_energy_balance(const clmsoil_seb_args *a, const clmsoil_levels *lev,
+                            double dtime);
 /* Test hook: a non-zero seed moves every pow() of the Zeng-Decker step one ulp
  * (direction from a hash of the operands and the seed); 0 restores exact pow(). */
 void orc_set_zd_pow_jitter(uint64_t seed);

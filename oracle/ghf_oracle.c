@@ -182,3 +182,52 @@ int orc_ground_heat_flux(const clmsoil_ghf_args *a, const clmsoil_levels *lev) {
     free(hs_h2osfc_patch);
     return st;
 }
+
+/* Soil energy balance error: _soilflux_errsoi_base_kernel! (soil_fluxes.jl:380-406),
+ * _soilflux_errsoi_layers_kernel! for the non-urban and the urban pass (:425-447,
+ * launched at :651-660) and the patch-to-column average (:499-525), in that order. */
+int orc_soil_energy_balance(const clmsoil_seb_args *a, const clmsoil_levels *lev,
+                            double dtime) {
+    if (!a || !lev) return CLMSOIL_ERR_NULL;
+    const int64_t nc = a->nc, np = a->np;
+    const int nlevsno = lev->nlevsno, nlevgrnd = lev->nlevgrnd, nlevurb = a->nlevurb;
+    const int64_t ICOL_ROOF = 71, ICOL_SUNWALL = 72, ICOL_SHADEWALL = 73;
+    for (int64_t p = 0; p < np; ++p) {
+        if (!a->mask_nolakep[p]) continue;
+        const int64_t c = a->patch_column[p] - 1;
+        a->errsoi_patch[p] = a->eflx_soil_grnd[p] - a->xmf[c] - a->xmf_h2osfc[c] -
+                             a->frac_h2osfc[c] * (a->t_h2osfc[c] - a->t_h2osfc_bef[c]) *
+                                 (a->c_h2osfc[c] / dtime);
+        a->errsoi_patch[p] += a->eflx_h2osfc_to_snow[c];
+        const int64_t it = a->col_itype[c];
+        if (it == ICOL_SUNWALL || it == ICOL_SHADEWALL || it == ICOL_ROOF)
+            a->errsoi_patch[p] += a->eflx_building_heat_errsoi[c];
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        const uint8_t *mask = pass == 0 ? a->mask_nolakep : a->mask_urbanp;
+        const int jmax = pass == 0 ? nlevgrnd : nlevurb;
+        for (int64_t p = 0; p < np; ++p) {
+            if (!mask[p]) continue;
+            const int64_t c = a->patch_column[p] - 1;
+            const int64_t it = a->col_itype[c];
+            const int wall = it == ICOL_SUNWALL || it == ICOL_SHADEWALL || it == ICOL_ROOF;
+            if (pass != wall) continue;
+            for (int jf = -nlevsno + 1; jf <= jmax; ++jf) {
+                const int jj = jf + nlevsno;
+                const double dt = CM(a->t_soisno, c, jj) - CM(a->t_ssbef, c, jj);
+                if (jf >= a->snl[c] + 1 && jf < 1)
+                    a->errsoi_patch[p] -= a->frac_sno_eff[c] * dt / CM(a->fact, c, jj);
+                if (jf >= 1) a->errsoi_patch[p] -= dt / CM(a->fact, c, jj);
+            }
+        }
+    }
+    for (int64_t c = 0; c < nc; ++c)
+        if (a->mask_nolakec[c]) a->errsoi_col[c] = 0.0;
+    for (int64_t p = 0; p < np; ++p) {
+        const int64_t c = a->patch_column[p];
+        if (c >= 1 && c <= nc && a->mask_nolakec[c - 1] && a->patchi[c - 1] <= p + 1 &&
+            p + 1 <= a->patchf[c - 1])
+            a->errsoi_col[c - 1] += a->errsoi_patch[p] * a->wtcol[p];
+    }
+    return CLMSOIL_OK;
+}

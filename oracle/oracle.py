@@ -16,7 +16,7 @@ REPO = ORACLE_DIR.parent
 sys.path.insert(0, str(REPO / "clm.jl_amd"))
 
 from clmsoil import _abi  # noqa: E402
-from clmsoil.bind import byref, ghf_args, heat_args, levels, mf_args, zd_args  # noqa: E402
+from clmsoil.bind import byref, ghf_args, heat_args, levels, mf_args, seb_args, zd_args  # noqa: E402
 
 LIB_PATH = ORACLE_DIR / "liboracle_clmsoil.so"
 _lib = None
@@ -95,6 +95,13 @@ def ground_heat_flux(d: dict) -> None:
     """compute_ground_heat_flux_and_deriv! on the host arrays of `d` (outputs in place)."""
     L = lib()
     check(L.orc_ground_heat_flux(byref(ghf_args(d)), byref(levels(d))), "ground_heat_flux")
+
+
+def soil_energy_balance(d: dict, dtime: float) -> None:
+    """errsoi_patch / errsoi_col of soil_fluxes! on the host arrays of `d`."""
+    L = lib()
+    check(L.orc_soil_energy_balance(byref(seb_args(d)), byref(levels(d)), float(dtime)),
+          "soil_energy_balance")
 
 
 def jl_pow4(x: float) -> float:
