@@ -182,6 +182,85 @@ def zengdecker_2009(nc, steps, cpu_sample):
     return out
 
 
+def ghf_bytes(d):
+    """Algorithmic bytes of one ground-heat-flux launch, each input read once and
+    each output written once: per non-lake patch 14 float64 forcing values and
+    its weight, 3 int64 indices, the absorbed solar of its snow layers and the
+    top soil layer, and 3 float64 outputs (urban patches: 8 inputs, 5 landunit
+    values, 8 outputs); per lake patch its mask and column index; per column 6
+    float64 inputs, two layer temperatures, 4 int64 indices and a mask, and
+    5 + nlevsno + 1 float64 outputs."""
+    import numpy as np
+    pc = d["patch_column"] - 1
+    on = d["mask_nolakep"] == 1
+    urb = on & (d["urbpoi"][d["patch_landunit"] - 1] == 1)
+    rur = on & ~urb
+    lyr = 1 - d["snl"][pc]
+    b = (rur * (8 * (15 + 3) + 1 + 8 * lyr + 8 * 3) + urb * (8 * (8 + 5 + 3) + 1 + 8 * 8) +
+         (~on) * (1 + 8))
+    cols = d["nc"] * (8 * 6 + 8 * 2 + 8 * 4 + 1 + 8 * (5 + d["nlevsno"] + 1))
+    return float(np.sum(b)) + cols
+
+
+def seb_bytes(d):
+    """Algorithmic bytes of one energy-balance launch: per column 4 int64 and a
+    mask, 9 float64 inputs, t_soisno, t_ssbef and fact over its snow and ground
+    layers, one output; per patch its index, masks, flux, weight, prior and
+    result."""
+    import numpy as np
+    ns = -d["snl"]
+    layers = ns + d["nlevgrnd"]
+    cols = np.sum(8 * 4 + 1 + 8 * 9 + 3 * 8 * layers + 8)
+    patches = d["np"] * (8 + 2 + 8 * 4)
+    return float(cols + patches)
+
+
+def timed_launches(fn, steps):
+    """Average ms per launch of `fn`, HIP events around each launch."""
+    import torch
+    fn()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    for a, b in ev:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in ev) / steps
+
+
+def surface_energy(nc, steps):
+    """The ground heat flux (subgrid patches summed per column) and the soil
+    energy balance error at `nc` columns, each launched `steps` times."""
+    from clmsoil import api, inputs
+    g = inputs.ground_heat_flux_inputs(nc=nc, seed=9)
+    tg = api.to_device(g)
+    ga = tg | {"nc": g["nc"], "np": g["np"], "nl": g["nl"]}
+    ms_g = timed_launches(lambda: api.compute_ground_heat_flux_and_deriv(ga, 12, 25, 0), steps)
+    assert api.device_errors() == 0
+    bg = ghf_bytes(g)
+    del tg, ga
+    e = inputs.energy_balance_inputs(nc=nc, seed=10)
+    te = api.to_device(e)
+    ea = te | {"nc": e["nc"], "np": e["np"]}
+    ms_e = timed_launches(lambda: api.soil_energy_balance(ea, 1800.0, 12, 25, e["nlevurb"]), steps)
+    assert api.device_errors() == 0
+    be = seb_bytes(e)
+
+    def line(ms, by, what):
+        return {"workload": what, "ms_per_launch": ms, "column_steps_per_s": nc / (ms / 1e3),
+                "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9,
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                             "bytes_per_launch": by}}
+    return {"ground_heat_flux": line(ms_g, bg, f"{nc} columns, {g['np']} patches "
+                                                "(70% vegetated with 1-12 patches, 12% crop, "
+                                                "10% urban, 8% lake), 0-12 snow layers"),
+            "soil_energy_balance": line(ms_e, be, f"{nc} columns, {e['np']} patches (1-4 per "
+                                                  "column), 0-12 snow layers + 25 ground layers")}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +275,10 @@ def main(argv=None):
                     help="skip the learned-vs-plain first-substep comparison")
     ap.add_argument("--no-zd", action="store_true",
                     help="skip the Zeng-Decker 2009 step section")
+    ap.add_argument("--no-surface", action="store_true",
+                    help="skip the ground heat flux / energy balance section")
+    ap.add_argument("--surface-only", action="store_true",
+                    help="run only the ground heat flux / energy balance section")
     ap.add_argument("--zd-only", action="store_true",
                     help="run only the Zeng-Decker 2009 section (for profiling its kernel)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -222,6 +305,9 @@ def main(argv=None):
         print(json.dumps({"copy_bytes_each_way": n * 8, "copies": 4}), flush=True)
         return 0
     nc = args.nc
+    if args.surface_only:
+        print(json.dumps({"surface_energy": surface_energy(nc, args.steps)}), flush=True)
+        return 0
     if args.zd_only:
         print(json.dumps({"zengdecker_2009": zengdecker_2009(
             nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample)}), flush=True)
@@ -307,6 +393,8 @@ def main(argv=None):
     }
     if not args.no_learned:
         out["learned_vs_plain"] = learned_vs_plain(nc, args.steps)
+    if not args.no_surface:
+        out["surface_energy"] = surface_energy(nc, args.steps)
     if not args.no_zd:
         out["zengdecker_2009"] = zengdecker_2009(
             nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample)
