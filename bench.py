@@ -24,7 +24,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE counters, each scaled by what the counter reads for a plain 2 GiB
 # device copy (fetch x2.0, write x1.0); recorded in
 # profiles/hbm_traffic_production_day_1M.md.
-MEASURED_TRAFFIC_1M = {"heat": 7.955e9, "richards": 9.573e9}
+MEASURED_TRAFFIC_1M = {"heat": 7.956e9, "richards": 4.243e9}
 METRIC = "column-steps/s @1M cols FP64 soil solves; HBM GB/s frac; learned vs plain dt"
 
 

@@ -38,7 +38,7 @@ __device__ __forceinline__ double prv(const double *x, int i, int k) {
     return (k == 0) ? b : a;
 }
 
-template <int G, bool EXP = false, bool REC = false>
+template <int G, bool EXP = false, bool REC = false, int LCS = 0>
 struct RColumn {
     static constexpr int NMAX = MAX_NLEVSOI;
     static constexpr int S = (NMAX + G - 1) / G;
@@ -58,6 +58,26 @@ struct RColumn {
     double liq[S], mdz[S], khat[S], den[S], watsat[S];
     double fnet[S], a0[S], sb[S], c2[S];
     double qin[SE], qout[SE], di1[SE], do1[SE];
+    // LDS cache of the column when LCS > 0 (LCS = the block size): bsw, sucsat and
+    // the root sink are copied in once per column instead of being read from memory
+    // at every property update, and smp/hk are kept there and written to smp_l/hk_l
+    // once, when the column is stored. Lane-private: field f, slot i at
+    // lc[(f*S + i) * LCS], so a wavefront's access is one contiguous, conflict-free
+    // line and every offset is an immediate.
+    static constexpr bool LC = LCS > 0;
+    double *lc = nullptr;
+    enum { LC_BSW = 0, LC_SUC = 1, LC_ROOT = 2, LC_SMP = 3, LC_HK = 4, LC_KHAT = 5, LC_DEN = 6,
+           LC_FIELDS = 7 };
+    __device__ __forceinline__ double &cache(int f, int i) const { return lc[(f * S + i) * LCS]; }
+    // khat and den are read once per property update: in LDS when cached
+    __device__ __forceinline__ double &kh(int i) {
+        if constexpr (LC) return cache(LC_KHAT, i);
+        else return khat[i];
+    }
+    __device__ __forceinline__ double &dn(int i) {
+        if constexpr (LC) return cache(LC_DEN, i);
+        else return den[i];
+    }
 
     __device__ __forceinline__ int64_t soil(int j) const { return c + (int64_t)(j - 1) * ld_; }
     int64_t ld_;
@@ -96,15 +116,20 @@ struct RColumn {
                 double imp;
                 if (j == n) {
                     imp = pow(10.0, -P.e_ice * icf);
-                    den[i] = M_TO_MM * (zwt - zj);                     // :1357 (WATERTABLE)
+                    dn(i) = M_TO_MM * (zwt - zj);                      // :1357 (WATERTABLE)
                 } else {
                     const double icf1 = __builtin_nontemporal_load(&P.icefrac[o + ld]);
                     imp = pow(10.0, -P.e_ice * (0.5 * (icf + icf1)));
-                    den[i] = M_TO_MM * (__builtin_nontemporal_load(&P.z[os + ld]) - zj);   // :1338
+                    dn(i) = M_TO_MM * (__builtin_nontemporal_load(&P.z[os + ld]) - zj);   // :1338
                 }
-                khat[i] = imp * hks;   // (imp * hksat) * s1^(2b+3), the reference's grouping
+                kh(i) = imp * hks;   // (imp * hksat) * s1^(2b+3), the reference's grouping
+                if constexpr (LC) {
+                    cache(LC_BSW, i) = P.bsw[o];
+                    cache(LC_SUC, i) = P.sucsat[o];
+                    cache(LC_ROOT, i) = P.qflx_rootsoi[o];
+                }
             } else {
-                liq[i] = 0.0; mdz[i] = M_TO_MM; watsat[i] = 1.0; khat[i] = 0.0; den[i] = 1.0;
+                liq[i] = 0.0; mdz[i] = M_TO_MM; watsat[i] = 1.0; kh(i) = 0.0; dn(i) = 1.0;
             }
             fnet[i] = a0[i] = sb[i] = c2[i] = 0.0;
             if constexpr (EXP) qin[i] = qout[i] = di1[i] = do1[i] = 0.0;
@@ -147,8 +172,13 @@ struct RColumn {
             const int j = i * G + k + 1;
             double b = 1.0, suc = 0.0;
             if (j <= n) {
-                b = P.bsw[soil(j)];
-                suc = P.sucsat[soil(j)];
+                if constexpr (LC) {
+                    b = cache(LC_BSW, i);
+                    suc = cache(LC_SUC, i);
+                } else {
+                    b = P.bsw[soil(j)];
+                    suc = P.sucsat[soil(j)];
+                }
             }
             bsw[i] = b;
             const double sn = s2[i];
@@ -156,7 +186,8 @@ struct RColumn {
             const double dsmpds = -b * smpv / sn;
             smp[i] = smpv;
             dsmpdw[i] = dsmpds / watsat[i];
-            if (j <= n && P.smp_l) P.smp_l[soil(j)] = smpv;
+            if constexpr (LC) cache(LC_SMP, i) = smpv;
+            else if (j <= n && P.smp_l) P.smp_l[soil(j)] = smpv;
         }
         // interface conductivity (:1287-1299) and fluxes (:1334-1367)
         double qo[S], d1[S];
@@ -168,9 +199,10 @@ struct RColumn {
             s1 = jl_min(s1, 1.0);
             s1 = jl_max(0.01, s1);
             const double b = bsw[i];
-            const double hkv = khat[i] * pow(s1, 2.0 * b + 3.0);
+            const double hkv = kh(i) * pow(s1, 2.0 * b + 3.0);
             const double dhkdw = (2.0 * b + 3.0) * hkv / s1;
-            if (j <= n && P.hk_l) P.hk_l[soil(j)] = hkv;
+            if constexpr (LC) cache(LC_HK, i) = hkv;
+            else if (j <= n && P.hk_l) P.hk_l[soil(j)] = hkv;
             const double smpn = nxt<G>(smp, i, k, S);
             const double dsmpdwn = nxt<G>(dsmpdw, i, k, S);
             const double watsatn = nxt<G>(watsat, i, k, S);
@@ -179,18 +211,18 @@ struct RColumn {
                 const double dhkds1 = 0.5 * dhkdw / watsat[i];
                 const double dhkds2 = 0.5 * dhkdw / watsatn;
                 const double num = smpn - smp[i];
-                const double dn = den[i];
-                q = -hkv * num / dn + hkv;
-                e1 = (hkv * dsmpdw[i] - dhkds1 * num) / dn + dhkds1;
-                e2 = (-hkv * dsmpdwn - dhkds2 * num) / dn + dhkds2;
+                const double dni = dn(i);
+                q = -hkv * num / dni + hkv;
+                e1 = (hkv * dsmpdw[i] - dhkds1 * num) / dni + dhkds1;
+                e2 = (-hkv * dsmpdwn - dhkds2 * num) / dni + dhkds2;
             } else if (j == n) {
                 if (lower_bc == CLMSOIL_BC_WATERTABLE) {
                     if (!(j > jwt)) {
                         const double dhkds1l = dhkdw / watsat[i];
                         const double num = -smp[i];
-                        const double dn = den[i];
-                        q = -hkv * num / dn + hkv;
-                        e1 = (hkv * dsmpdw[i] - dhkds1l * num) / dn + dhkds1l;
+                        const double dni = dn(i);
+                        q = -hkv * num / dni + hkv;
+                        e1 = (hkv * dsmpdw[i] - dhkds1l * num) / dni + dhkds1l;
                     }
                     qn_a = q;          // qcharge = qout + dqodw1 * dwat (:1476-1478)
                     qn_b = e1;
@@ -215,7 +247,11 @@ struct RColumn {
             const double qi = (j == 1) ? qflx_infl : po;
             const double dqidw0 = (j == 1) ? 0.0 : p1;
             const double dqidw1 = (j == 1) ? 0.0 : p2;
-            const double root = (j <= n) ? P.qflx_rootsoi[soil(j)] : 0.0;
+            double root = 0.0;
+            if (j <= n) {
+                if constexpr (LC) root = cache(LC_ROOT, i);
+                else root = P.qflx_rootsoi[soil(j)];
+            }
             fnet[i] = qi - qo[i] - root;
             a0[i] = dqidw0;
             sb[i] = -dqidw1 + d1[i];
@@ -310,7 +346,11 @@ struct RColumn {
             const double dt_dz = dtsub / mdz[i];
             double fluxNet1;
             if constexpr (EXP) {
-                const double root = (j <= n) ? P.qflx_rootsoi[soil(j)] : 0.0;
+                double root = 0.0;
+                if (j <= n) {
+                    if constexpr (LC) root = cache(LC_ROOT, i);
+                    else root = P.qflx_rootsoi[soil(j)];
+                }
                 const double dwm = prv<G>(dwat, i, k);
                 const double dwn = nxt<G>(dwat, i, k, S);
                 const double qin_test = (j == 1) ? qin[i] + di1[i] * dwat[i]
@@ -375,6 +415,10 @@ struct RColumn {
         for (int i = 0; i < S; ++i) {
             const int j = i * G + k + 1;
             if (j <= n) P.h2osoi_liq[c + (int64_t)(joff + j - 1) * ld] = liq[i];
+            if constexpr (LC) {
+                if (j <= n && P.smp_l) P.smp_l[soil(j)] = cache(LC_SMP, i);
+                if (j <= n && P.hk_l) P.hk_l[soil(j)] = cache(LC_HK, i);
+            }
         }
         // liquid and thickness of layer jwt+1, read from its owner lane's registers
         double liq_w = 0.0, mdz_w = M_TO_MM;

@@ -34,15 +34,32 @@
 
 namespace clmsoil {
 
+// LDS cache stride of a launch: the block size for G = 4 at 256 threads per block
+// (35 doubles per lane, 70 KB per block), none otherwise (it would not fit at
+// G = 1 or 2, and other block sizes keep the global-memory path).
+template <int G>
+constexpr int richards_cache_stride(int block) {
+    return (G == 4 && block == 256) ? 256 : 0;
+}
+template <int G, bool EXP, int LCS>
+static size_t richards_lds_bytes() {
+    return (size_t)RColumn<G, EXP, false, LCS>::LC_FIELDS * RColumn<G, EXP, false, LCS>::S * LCS *
+           sizeof(double);
+}
+
 // One column per group for the group's lifetime.
-template <int G, bool EXP>
+template <int G, bool EXP, int LCS>
 __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kernel(RichardsParams P) {
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t c = gtid / G;
     const int k = (int)(threadIdx.x % G);
     if (c >= P.nc) return;
     if (P.mask && !P.mask[c]) return;
-    RColumn<G, EXP> col;
+    RColumn<G, EXP, false, LCS> col;
+    if constexpr (LCS > 0) {
+        extern __shared__ double richards_lds[];
+        col.lc = richards_lds + threadIdx.x;
+    }
     if (!col.load(P, c, k)) return;
     while (!col.iterate(P, k)) {
     }
@@ -51,14 +68,18 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kerne
 
 // Persistent: a resident grid whose groups refill from per-XCD column ranges.
 // Every wavefront leaves the loop once all eight ranges are exhausted.
-template <int G, bool EXP>
+template <int G, bool EXP, int LCS>
 __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persistent(RichardsParams P) {
     const int k = (int)(threadIdx.x % G);
     const int lane = (int)__lane_id();
     const int xcd = (int)(blockIdx.x % NUM_XCD);
     const int64_t chunk = (P.nc + NUM_XCD - 1) / NUM_XCD;
     unsigned *ctrs = P.work_counters;
-    RColumn<G, EXP> col;
+    RColumn<G, EXP, false, LCS> col;
+    if constexpr (LCS > 0) {
+        extern __shared__ double richards_lds[];
+        col.lc = richards_lds + threadIdx.x;
+    }
     bool active = false;   // group holds a column
     bool done = false;     // no work left for this group
     int steal = 0;         // XCD ranges exhausted so far (wavefront-uniform)
@@ -107,25 +128,32 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persi
     }
 }
 
-template <int G, bool EXP>
+template <int G, bool EXP, int LCS>
 static hipError_t launch_g(const RichardsParams &P, hipStream_t stream, int block) {
     const int64_t threads = P.nc * G;
     const int64_t grid = (threads + block - 1) / block;
-    hipLaunchKernelGGL((richards_mf_kernel<G, EXP>), dim3((unsigned)grid), dim3(block), 0, stream,
-                       P);
+    if (richards_lds_bytes<G, EXP, LCS>() > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)richards_mf_kernel<G, EXP, LCS>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)richards_lds_bytes<G, EXP, LCS>());
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((richards_mf_kernel<G, EXP, LCS>), dim3((unsigned)grid), dim3(block),
+                       (richards_lds_bytes<G, EXP, LCS>()), stream, P);
     return hipGetLastError();
 }
 
-// Resident blocks per CU of richards_mf_persistent<G> at a block size, from the
-// occupancy calculator, cached per (device, G, block) after the first query.
-template <int G, bool EXP>
+// Resident blocks per CU of richards_mf_persistent at a block size, from the
+// occupancy calculator, cached per (device, kernel, block) after the first query.
+template <int G, bool EXP, int LCS>
 static int resident_blocks(int dev, int block) {
     static std::atomic<int> cache[16][3];
     const int bi = block == 64 ? 0 : (block == 128 ? 1 : 2);
     if (dev < 0 || dev >= 16) dev = 0;
     int nb = cache[dev][bi].load(std::memory_order_relaxed);
     if (nb > 0) return nb;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G, EXP>, block, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G, EXP, LCS>,
+                                                     block, richards_lds_bytes<G, EXP, LCS>()) !=
         hipSuccess)
         return -1;
     if (nb < 1) nb = 1;
@@ -133,15 +161,21 @@ static int resident_blocks(int dev, int block) {
     return nb;
 }
 
-template <int G, bool EXP>
+template <int G, bool EXP, int LCS>
 static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int block,
                            int blocks_per_cu) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return hipErrorUnknown;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return hipErrorUnknown;
+    if (richards_lds_bytes<G, EXP, LCS>() > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)richards_mf_persistent<G, EXP, LCS>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)richards_lds_bytes<G, EXP, LCS>());
+        if (e != hipSuccess) return e;
+    }
     int nb = blocks_per_cu;
-    if (nb <= 0) nb = resident_blocks<G, EXP>(dev, block);
+    if (nb <= 0) nb = resident_blocks<G, EXP, LCS>(dev, block);
     if (nb <= 0) return hipErrorUnknown;
     // Any grid size is correct: groups only take work from the counters and never
     // wait on each other, so blocks that start late simply find less work.
@@ -149,25 +183,30 @@ static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int bloc
     grid = (grid + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     const int64_t need = (P.nc * G + block - 1) / block;
     if (grid > need) grid = (need + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
-    hipLaunchKernelGGL((richards_mf_persistent<G, EXP>), dim3((unsigned)grid), dim3(block), 0,
-                       stream, P);
+    hipLaunchKernelGGL((richards_mf_persistent<G, EXP, LCS>), dim3((unsigned)grid), dim3(block),
+                       (richards_lds_bytes<G, EXP, LCS>()), stream, P);
     return hipGetLastError();
 }
 
 template <bool EXP>
 static hipError_t launch_exp(const RichardsParams &P, int G, int block, bool persistent,
                              int blocks_per_cu, hipStream_t stream) {
+    const bool cached = richards_cache_stride<4>(block) > 0;
     if (persistent) {
         switch (G) {
-            case 1: return launch_p<1, EXP>(P, stream, block, blocks_per_cu);
-            case 2: return launch_p<2, EXP>(P, stream, block, blocks_per_cu);
-            default: return launch_p<4, EXP>(P, stream, block, blocks_per_cu);
+            case 1: return launch_p<1, EXP, 0>(P, stream, block, blocks_per_cu);
+            case 2: return launch_p<2, EXP, 0>(P, stream, block, blocks_per_cu);
+            default:
+                return cached ? launch_p<4, EXP, 256>(P, stream, block, blocks_per_cu)
+                              : launch_p<4, EXP, 0>(P, stream, block, blocks_per_cu);
         }
     }
     switch (G) {
-        case 1: return launch_g<1, EXP>(P, stream, block);
-        case 2: return launch_g<2, EXP>(P, stream, block);
-        default: return launch_g<4, EXP>(P, stream, block);
+        case 1: return launch_g<1, EXP, 0>(P, stream, block);
+        case 2: return launch_g<2, EXP, 0>(P, stream, block);
+        default:
+            return cached ? launch_g<4, EXP, 256>(P, stream, block)
+                          : launch_g<4, EXP, 0>(P, stream, block);
     }
 }
 
