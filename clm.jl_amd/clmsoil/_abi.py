@@ -108,6 +108,22 @@ class SebArgs(C.Structure):
                 [(f, _p) for f in SEB_FIELDS])
 
 
+FRAC_FIELDS = ["mask", "watsat", "dz", "h2osoi_ice", "excess_ice", "eff_porosity", "icefrac"]
+WT_FIELDS = ["mask", "dz", "z", "zi", "t_soisno", "watsat", "sucsat", "bsw", "qcharge",
+             "h2osoi_liq", "h2osoi_ice", "zwt", "wa", "zwt_perched", "frost_table", "h2osoi_vol",
+             "qflx_drain", "qflx_rsub_sat", "qflx_drain_perched"]
+
+
+class FractionsArgs(C.Structure):
+    """clmsoil_fractions_args (include/clmsoil_hydro.h)."""
+    _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in FRAC_FIELDS]
+
+
+class WaterTableArgs(C.Structure):
+    """clmsoil_water_table_args (include/clmsoil_hydro.h)."""
+    _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in WT_FIELDS]
+
+
 class HeatConfig(C.Structure):
     _fields_ = [("snow_thermal_cond_method", C.c_int32), ("reserved", C.c_int32)]
 
@@ -126,6 +142,13 @@ def declare_common(lib: C.CDLL, prefix: str) -> C.CDLL:
         getattr(lib, prefix + f).restype = C.c_int
     getattr(lib, prefix + "ground_heat_flux").argtypes = [C.POINTER(GhfArgs), C.POINTER(Levels)] + \
         ([P] if prefix == "clmsoil_" else [])
+    f = getattr(lib, prefix + "set_soil_water_fractions")
+    f.argtypes = [C.POINTER(FractionsArgs), C.POINTER(Levels)] + ([P] if prefix == "clmsoil_" else [])
+    f.restype = C.c_int
+    f = getattr(lib, prefix + "water_table")
+    f.argtypes = [C.POINTER(WaterTableArgs), C.POINTER(Levels), dbl, dbl, C.c_int] + \
+        ([P] if prefix == "clmsoil_" else [])
+    f.restype = C.c_int
     f = getattr(lib, prefix + "soil_energy_balance")
     f.argtypes = [C.POINTER(SebArgs), C.POINTER(Levels), dbl] + ([P] if prefix == "clmsoil_" else [])
     f.restype = C.c_int

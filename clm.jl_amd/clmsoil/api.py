@@ -23,8 +23,8 @@ import ctypes as C
 from dataclasses import dataclass, field
 
 from . import _abi
-from .bind import (byref, ghf_args, heat_args, heat_config, levels, mf_args, seb_args, swm_config,
-                   zd_args)
+from .bind import (byref, fractions_args, ghf_args, heat_args, heat_config, levels, mf_args,
+                   seb_args, swm_config, water_table_args, zd_args)
 
 ZENGDECKER_2009, MOISTURE_FORM = _abi.ZENGDECKER_2009, _abi.MOISTURE_FORM
 BC_HEAD, BC_FLUX, BC_ZERO_FLUX, BC_WATERTABLE, BC_AQUIFER = (
@@ -220,6 +220,38 @@ def soil_energy_balance(state: dict, dtime: float, nlevsno: int = 12, nlevgrnd: 
     L = _lib()
     _abi.check(L, L.clmsoil_soil_energy_balance(byref(seb_args(d)), byref(levels(d)), float(dtime),
                                                 _stream()), "soil_energy_balance")
+
+
+def set_soil_water_fractions(state: dict, nlevsno: int = 12, nlevsoi: int = 20,
+                             nlevgrnd: int = 25) -> None:
+    """eff_porosity and icefrac per soil layer (set_soil_water_fractions!,
+    soil_hydrology.jl:394-415). `state` maps the clmsoil_fractions_args fields
+    (include/clmsoil_hydro.h) to device tensors, plus `nc`."""
+    d = dict(state)
+    d.setdefault("nc", d["watsat"].shape[-1])
+    d.update(nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd)
+    if int(d["nc"]) == 0:
+        return
+    L = _lib()
+    _abi.check(L, L.clmsoil_set_soil_water_fractions(byref(fractions_args(d)), byref(levels(d)),
+                                                     _stream()), "set_soil_water_fractions")
+
+
+def water_table(state: dict, dtime: float, nlevsno: int = 12, nlevsoi: int = 20,
+                nlevgrnd: int = 25, aq_sp_yield_min: float = 0.01,
+                recompute_frost_table: bool = True) -> None:
+    """water_table! (soil_hydrology.jl:849-893): the step's aquifer recharge moves
+    the water table; frost and perched water tables. `state` maps the
+    clmsoil_water_table_args fields to device tensors, plus `nc`."""
+    d = dict(state)
+    d.setdefault("nc", d["zwt"].shape[0])
+    d.update(nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd)
+    if int(d["nc"]) == 0:
+        return
+    L = _lib()
+    _abi.check(L, L.clmsoil_water_table(byref(water_table_args(d)), byref(levels(d)), float(dtime),
+                                        float(aq_sp_yield_min), int(bool(recompute_frost_table)),
+                                        _stream()), "water_table")
 
 
 def soil_water(st, mfin, mask, nbedrock, nlevsno: int, nlevsoi: int, nlevgrnd: int,

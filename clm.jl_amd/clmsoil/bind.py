@@ -10,9 +10,9 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import (GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_FIELDS, MF_FIELDS, SEB_FIELDS,
-                   ZD_FIELDS, GhfArgs, HeatArgs, HeatConfig, Levels, MfArgs, SebArgs, SwmConfig,
-                   ZdArgs)
+from ._abi import (FRAC_FIELDS, GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_FIELDS, MF_FIELDS,
+                   SEB_FIELDS, WT_FIELDS, ZD_FIELDS, FractionsArgs, GhfArgs, HeatArgs, HeatConfig,
+                   Levels, MfArgs, SebArgs, SwmConfig, WaterTableArgs, ZdArgs)
 
 _INT_FIELDS = {"nbedrock", "imelt", "frac_veg_nosno"} | set(GHF_INT_FIELDS)
 _U8_FIELDS = {"mask", "mask_urbanp"} | set(GHF_U8_FIELDS)
@@ -111,6 +111,26 @@ def seb_args(d: dict) -> SebArgs:
         _check_dtype(f, x)
         setattr(a, f, _ptr(x))
     return a
+
+
+def _simple_args(cls, fields, d: dict):
+    nc = int(d["nc"])
+    a = cls()
+    a.nc = nc
+    a.ld = int(d.get("ld", nc))
+    for f in fields:
+        x = d.get(f)
+        _check_dtype(f, x)
+        setattr(a, f, _ptr(x))
+    return a
+
+
+def fractions_args(d: dict) -> FractionsArgs:
+    return _simple_args(FractionsArgs, FRAC_FIELDS, d)
+
+
+def water_table_args(d: dict) -> WaterTableArgs:
+    return _simple_args(WaterTableArgs, WT_FIELDS, d)
 
 
 def swm_config(lower_bc=2, upper_bc=1, method=1, flux_calculation=1, dtmin=60.0,

@@ -562,3 +562,33 @@ def energy_balance_from_heat(h0, h1):
         t_soisno=np.nan_to_num(h1["t_soisno"]), t_ssbef=np.nan_to_num(h0["t_soisno"]),
         fact=np.where(np.isnan(h1["fact"]), 1.0, h1["fact"]),
         errsoi_patch=np.zeros(nc), errsoi_col=np.zeros(nc))
+
+
+def water_table_columns(nc=4096, seed=16, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrnd=NLEVGRND,
+                        zwt_range=(0.3, 12.0), frozen_fraction=0.3):
+    """Columns for the water-table update after an aquifer step: the soil and
+    water state of zengdecker_columns, a recharge qcharge ~ U(-2e-4, 2e-4) mm/s
+    (half of the columns draining, half filling, enough to move the table across
+    layer interfaces over 1800 s), aquifer water wa ~ U(3000, 5000) mm, excess
+    ice on a fifth of the columns, and frozen layers below unfrozen ones on the
+    frozen columns (so a frost table exists)."""
+    d = zengdecker_columns(nc=nc, seed=seed, nlevsno=nlevsno, nlevsoi=nlevsoi,
+                           nlevgrnd=nlevgrnd, frozen_fraction=frozen_fraction,
+                           zwt_range=zwt_range)
+    rng = np.random.default_rng(seed + 3)
+    ns = nlevsno
+    t = d["t_soisno"]
+    frozen = d["h2osoi_ice"][ns:ns + nlevsoi].sum(0) > 0
+    kf = rng.integers(1, nlevsoi, nc)
+    lay = np.arange(1, nlevgrnd + 1)[:, None]
+    t[ns:] = np.where(frozen[None, :] & (lay >= kf[None, :]), rng.uniform(260.0, 273.0, (nlevgrnd, nc)),
+                      rng.uniform(274.0, 290.0, (nlevgrnd, nc)))
+    exc = np.zeros((nlevgrnd, nc))
+    has = rng.uniform(size=nc) < 0.2
+    exc[:, has] = rng.uniform(0.0, 30.0, (nlevgrnd, int(has.sum())))
+    d.update(qcharge=rng.uniform(-2e-4, 2e-4, nc) * rng.uniform(0.0, 20.0, nc),
+             wa=rng.uniform(3000.0, 5000.0, nc), zwt_perched=np.zeros(nc), frost_table=np.zeros(nc),
+             h2osoi_vol=d["h2osoi_vol"].copy(), qflx_drain=np.full(nc, 7.0),
+             qflx_rsub_sat=np.full(nc, 7.0), qflx_drain_perched=np.full(nc, 7.0),
+             excess_ice=exc, eff_porosity=np.zeros((nlevgrnd, nc)))
+    return d

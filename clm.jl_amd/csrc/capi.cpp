@@ -17,6 +17,7 @@
 #include "../../include/clmsoil_adjoint.h"
 #include "../../include/clmsoil_zd09.h"
 #include "../../include/clmsoil_ghf.h"
+#include "../../include/clmsoil_hydro.h"
 #include "common.hpp"
 #include "heat.hpp"
 #include "richards.hpp"
@@ -24,6 +25,7 @@
 #include "state.hpp"
 #include "zd09.hpp"
 #include "ghf.hpp"
+#include "hydro.hpp"
 
 using namespace clmsoil;
 
@@ -422,6 +424,46 @@ int clmsoil_soil_energy_balance(const clmsoil_seb_args *a, const clmsoil_levels 
     if (st) return st;
     return launch_soil_energy_balance(*a, lev->nlevsno, lev->nlevgrnd, dtime, s->err,
                                       (hipStream_t)stream) == hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
+}
+
+// set_soil_water_fractions! (soil_hydrology.jl:394-415).
+int clmsoil_set_soil_water_fractions(const clmsoil_fractions_args *a, const clmsoil_levels *lev,
+                                     void *stream) {
+    if (!a) return CLMSOIL_ERR_NULL;
+    int st = validate_levels(lev);
+    if (st) return st;
+    if (a->nc < 0 || a->ld < a->nc) return CLMSOIL_ERR_SIZE;
+    if (a->nc == 0) return CLMSOIL_OK;
+    if (!a->watsat || !a->dz || !a->h2osoi_ice || !a->eff_porosity || !a->icefrac)
+        return CLMSOIL_ERR_NULL;
+    return launch_soil_water_fractions(*a, lev->nlevsno, lev->nlevsoi, (hipStream_t)stream) ==
+                   hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
+}
+
+// water_table! (soil_hydrology.jl:849-893).
+int clmsoil_water_table(const clmsoil_water_table_args *a, const clmsoil_levels *lev,
+                        double dtime, double aq_sp_yield_min, int recompute_frost_table,
+                        void *stream) {
+    if (!a) return CLMSOIL_ERR_NULL;
+    int st = validate_levels(lev);
+    if (st) return st;
+    if (a->nc < 0 || a->ld < a->nc) return CLMSOIL_ERR_SIZE;
+    if (!(dtime > 0.0) || !std::isfinite(dtime) || !std::isfinite(aq_sp_yield_min))
+        return CLMSOIL_ERR_ARG;
+    if (recompute_frost_table != 0 && recompute_frost_table != 1) return CLMSOIL_ERR_ARG;
+    if (a->nc == 0) return CLMSOIL_OK;
+    const void *req[] = {a->dz, a->z, a->zi, a->t_soisno, a->watsat, a->sucsat, a->bsw,
+                         a->qcharge, a->h2osoi_liq, a->h2osoi_ice, a->zwt, a->wa, a->zwt_perched,
+                         a->frost_table, a->h2osoi_vol, a->qflx_drain, a->qflx_rsub_sat,
+                         a->qflx_drain_perched};
+    for (const void *q : req)
+        if (!q) return CLMSOIL_ERR_NULL;
+    return launch_water_table(*a, lev->nlevsno, lev->nlevsoi, dtime, aq_sp_yield_min,
+                              recompute_frost_table, (hipStream_t)stream) == hipSuccess
                ? CLMSOIL_OK
                : CLMSOIL_ERR_HIP;
 }

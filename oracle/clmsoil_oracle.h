@@ -16,6 +16,7 @@
 #include "../include/clmsoil.h"
 #include "../include/clmsoil_zd09.h"
 #include "../include/clmsoil_ghf.h"
+#include "../include/clmsoil_hydro.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -84,6 +85,11 @@ int orc_ground_heat_flux(const clmsoil_ghf_args *a, const clmsoil_levels *lev);
 /* Soil energy balance error, soil_fluxes.jl:380-447, 499-525 (ghf_oracle.c). */
 int orc_soil_energy_balance(const clmsoil_seb_args *a, const clmsoil_levels *lev,
                             double dtime);
+/* Soil water fractions and water table, soil_hydrology.jl:110-136, 676-893
+ * (hydro_oracle.c). */
+int orc_set_soil_water_fractions(const clmsoil_fractions_args *a, const clmsoil_levels *lev);
+int orc_water_table(const clmsoil_water_table_args *a, const clmsoil_levels *lev, double dtime,
+                    double aq_sp_yield_min, int recompute_frost_table);
 /* Test hook: a non-zero seed moves every pow() of the Zeng-Decker step one ulp
  * (direction from a hash of the operands and the seed); 0 restores exact pow(). */
 void orc_set_zd_pow_jitter(uint64_t seed);

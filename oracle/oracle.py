@@ -16,7 +16,8 @@ REPO = ORACLE_DIR.parent
 sys.path.insert(0, str(REPO / "clm.jl_amd"))
 
 from clmsoil import _abi  # noqa: E402
-from clmsoil.bind import byref, ghf_args, heat_args, levels, mf_args, seb_args, zd_args  # noqa: E402
+from clmsoil.bind import (byref, fractions_args, ghf_args, heat_args, levels, mf_args,  # noqa: E402
+                          seb_args, water_table_args, zd_args)
 
 LIB_PATH = ORACLE_DIR / "liboracle_clmsoil.so"
 _lib = None
@@ -102,6 +103,20 @@ def soil_energy_balance(d: dict, dtime: float) -> None:
     L = lib()
     check(L.orc_soil_energy_balance(byref(seb_args(d)), byref(levels(d)), float(dtime)),
           "soil_energy_balance")
+
+
+def set_soil_water_fractions(d: dict) -> None:
+    L = lib()
+    check(L.orc_set_soil_water_fractions(byref(fractions_args(d)), byref(levels(d))),
+          "set_soil_water_fractions")
+
+
+def water_table(d: dict, dtime: float, aq_sp_yield_min: float = 0.01,
+                recompute_frost_table: bool = True) -> None:
+    L = lib()
+    check(L.orc_water_table(byref(water_table_args(d)), byref(levels(d)), float(dtime),
+                            float(aq_sp_yield_min), int(bool(recompute_frost_table))),
+          "water_table")
 
 
 def jl_pow4(x: float) -> float:

@@ -21,6 +21,8 @@ STRUCTS = {
     "clmsoil_zd_args": _abi.ZdArgs,
     "clmsoil_ghf_args": _abi.GhfArgs,
     "clmsoil_seb_args": _abi.SebArgs,
+    "clmsoil_fractions_args": _abi.FractionsArgs,
+    "clmsoil_water_table_args": _abi.WaterTableArgs,
 }
 
 
@@ -31,6 +33,7 @@ def c_layout(tmp_path_factory):
         pytest.skip("no C compiler")
     lines = ["#include <stdio.h>", "#include <stddef.h>",
              '#include "clmsoil.h"', '#include "clmsoil_zd09.h"', '#include "clmsoil_ghf.h"',
+             '#include "clmsoil_hydro.h"',
              "int main(void) {"]
     for cname, py in STRUCTS.items():
         lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
