@@ -129,39 +129,71 @@ def zd09_bytes_per_column(n):
 
 
 def zengdecker_2009(nc, steps, cpu_sample):
-    """The Zeng-Decker 2009 step (CLM.jl's default soil water method) on `nc`
-    columns: 20 soil layers, water table 0.3-12 m, 30 % of the columns frozen.
-    `steps` consecutive steps on the evolving state, each launch bracketed by HIP
-    events on its stream; the oracle is timed on the first `cpu_sample` columns."""
+    """The aquifer configuration of the soil water: per 1800 s step the soil
+    water fractions, the Zeng-Decker 2009 step (CLM.jl's default soil water
+    method) and the water table update, on `nc` columns with 20 soil layers, the
+    water table at 0.3-12 m and 30 % of the columns frozen; `steps` consecutive
+    steps on the evolving state, each launch bracketed by HIP events on its
+    stream; the oracle is timed on the Zeng-Decker step of the first
+    `cpu_sample` columns."""
     import numpy as np
     import torch
     from clmsoil import api, inputs
-    d = inputs.zengdecker_columns(nc=nc, seed=5)
+    d = inputs.water_table_columns(nc=nc, seed=5)
     t = api.to_device(d)
     keys = ("z", "dz", "zi", "watsat", "hksat", "bsw", "sucsat", "smpmin", "h2osoi_liq",
             "h2osoi_ice", "h2osoi_vol", "t_soisno", "zwt", "qflx_infl", "qflx_rootsoi",
             "icefrac", "smp_l", "hk_l", "qcharge", "qflx_deficit")
     state = {k: t[k] for k in keys}
+    frac = {k: t[k] for k in ("watsat", "dz", "h2osoi_ice", "excess_ice", "eff_porosity",
+                              "icefrac")} | {"nc": nc}
+    wt = {k: t[k] for k in ("dz", "z", "zi", "t_soisno", "watsat", "sucsat", "bsw", "qcharge",
+                            "h2osoi_liq", "h2osoi_ice", "zwt", "wa", "zwt_perched",
+                            "frost_table", "h2osoi_vol", "qflx_drain", "qflx_rsub_sat",
+                            "qflx_drain_perched")} | {"nc": nc}
     cfg = api.init_soilwater_movement()
+
+    def step(ev=None):
+        if ev:
+            ev[0].record()
+        api.set_soil_water_fractions(frac)
+        if ev:
+            ev[1].record()
+        api.soilwater_zengdecker2009(state, 1800.0, cfg)
+        if ev:
+            ev[2].record()
+        api.water_table(wt, 1800.0)
+        if ev:
+            ev[3].record()
     for _ in range(2):
-        api.soilwater_zengdecker2009(state, 1800.0, cfg)
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(steps)]
+        step()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
     torch.cuda.synchronize()
-    for a, b in ev:
-        a.record()
-        api.soilwater_zengdecker2009(state, 1800.0, cfg)
-        b.record()
+    for e in ev:
+        step(e)
     torch.cuda.synchronize()
-    ms = sum(a.elapsed_time(b) for a, b in ev) / steps
-    assert bool(torch.isfinite(t["h2osoi_liq"]).all())
+    ms_f = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    ms = sum(e[1].elapsed_time(e[2]) for e in ev) / steps
+    ms_w = sum(e[2].elapsed_time(e[3]) for e in ev) / steps
+    assert bool(torch.isfinite(t["h2osoi_liq"]).all()) and bool(torch.isfinite(t["zwt"]).all())
     by = zd09_bytes_per_column(d["nlevsoi"]) * nc
+    by_f = 8 * 6 * d["nlevsoi"] * nc      # watsat, dz, ice, excess ice in; two fractions out
     out = {"workload": f"{nc} columns x {steps} steps, 20 soil layers, zwt 0.3-12 m, "
                        f"30% frozen columns, 1800 s",
            "ms_per_step": ms, "column_steps_per_s": nc / (ms / 1e3),
            "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
-                        "bytes_per_column": zd09_bytes_per_column(d["nlevsoi"])}}
+                        "bytes_per_column": zd09_bytes_per_column(d["nlevsoi"])},
+           "soil_water_fractions": {
+               "ms_per_launch": ms_f,
+               "roofline": {"bound": "hbm", "achieved": by_f / (ms_f / 1e3) / 1e9,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": by_f / (ms_f / 1e3) / 1e9 / HBM_PEAK_GBS,
+                            "bytes_per_column": 8 * 6 * d["nlevsoi"]}},
+           "water_table": {"ms_per_launch": ms_w,
+                           "note": "per-column searches and layer-by-layer moves; "
+                                   "bound by dependent loads, not bandwidth"},
+           "aquifer_step_ms": ms_f + ms + ms_w}
     if cpu_sample:
         sys.path.insert(0, str(REPO / "oracle"))
         import oracle
