@@ -191,8 +191,10 @@ def zengdecker_2009(nc, steps, cpu_sample):
                             "frac": by_f / (ms_f / 1e3) / 1e9 / HBM_PEAK_GBS,
                             "bytes_per_column": 8 * 6 * d["nlevsoi"]}},
            "water_table": {"ms_per_launch": ms_w,
-                           "note": "per-column searches and layer-by-layer moves; "
-                                   "bound by dependent loads, not bandwidth"},
+                           "note": "per-column searches and layer-by-layer moves: the bytes read "
+                                   "depend on the state (0.96 GB of HBM traffic per "
+                                   "launch at 1M columns measured with FETCH_SIZE/"
+                                   "WRITE_SIZE)"},
            "aquifer_step_ms": ms_f + ms + ms_w}
     if cpu_sample:
         sys.path.insert(0, str(REPO / "oracle"))
