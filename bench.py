@@ -295,6 +295,38 @@ def surface_energy(nc, steps):
                                                   "column), 0-12 snow layers + 25 ground layers")}
 
 
+def water_tables_moisture_form(nc, steps):
+    """The water tables after a step without an aquifer layer (perched,
+    theta-based, bedrock clamp in one launch) on `nc` columns of 20 soil layers
+    with frozen layers under unfrozen ones on 30 % of the columns and the 90 %
+    saturation level anywhere in the column; `steps` launches on the same state,
+    each bracketed by HIP events on its stream."""
+    import torch
+    from clmsoil import api, inputs
+    d = inputs.moisture_water_table_columns(nc=nc, seed=52)
+    t = api.to_device(d)
+    st = {k: t[k] for k in ("nbedrock", "dz", "z", "zi", "t_soisno", "watsat", "h2osoi_liq",
+                            "h2osoi_ice", "zwt", "zwt_perched", "frost_table", "h2osoi_vol")}
+    st["nc"] = nc
+    zwt0 = t["zwt"].clone()
+    for _ in range(2):
+        t["zwt"].copy_(zwt0)
+        api.water_table_moisture_form(st)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
+    for e in ev:
+        t["zwt"].copy_(zwt0)
+        e[0].record()
+        api.water_table_moisture_form(st)
+        e[1].record()
+    torch.cuda.synchronize()
+    assert api.device_errors() == 0
+    ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    return {"workload": f"{nc} columns, 20 soil layers, nbedrock 3-20, 30% frozen columns",
+            "ms_per_launch": ms, "columns_per_s": nc / (ms / 1e3),
+            "note": "per-column searches up and down the saturation and temperature "
+                    "profiles: the bytes read depend on the state"}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -315,6 +347,9 @@ def main(argv=None):
                     help="run only the ground heat flux / energy balance section")
     ap.add_argument("--zd-only", action="store_true",
                     help="run only the Zeng-Decker 2009 section (for profiling its kernel)")
+    ap.add_argument("--hydro-only", action="store_true",
+                    help="run only the soil hydrology sections (aquifer step, water tables "
+                         "without an aquifer)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -345,6 +380,12 @@ def main(argv=None):
     if args.zd_only:
         print(json.dumps({"zengdecker_2009": zengdecker_2009(
             nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample)}), flush=True)
+        return 0
+    if args.hydro_only:
+        print(json.dumps({"zengdecker_2009": zengdecker_2009(
+            nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample),
+            "water_tables_moisture_form": water_tables_moisture_form(nc, args.steps)}),
+            flush=True)
         return 0
     host, dev, gen_s = make_state(nc, args.seed)
     print(f"# state ready: {nc} columns in {gen_s:.1f} s", file=sys.stderr, flush=True)
@@ -432,6 +473,7 @@ def main(argv=None):
     if not args.no_zd:
         out["zengdecker_2009"] = zengdecker_2009(
             nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample)
+        out["water_tables_moisture_form"] = water_tables_moisture_form(nc, args.steps)
     if not args.no_cpu_baseline:
         v, cores, secs = cpu_baseline(host, min(args.cpu_sample, nc), args.cpu_steps)
         out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": cores,

@@ -113,6 +113,9 @@ WT_FIELDS = ["mask", "dz", "z", "zi", "t_soisno", "watsat", "sucsat", "bsw", "qc
              "h2osoi_liq", "h2osoi_ice", "zwt", "wa", "zwt_perched", "frost_table", "h2osoi_vol",
              "qflx_drain", "qflx_rsub_sat", "qflx_drain_perched"]
 
+THETA_WT_FIELDS = ["mask", "nbedrock", "dz", "z", "zi", "t_soisno", "watsat", "h2osoi_liq",
+                   "h2osoi_ice", "zwt", "zwt_perched", "frost_table", "h2osoi_vol"]
+
 
 class FractionsArgs(C.Structure):
     """clmsoil_fractions_args (include/clmsoil_hydro.h)."""
@@ -122,6 +125,11 @@ class FractionsArgs(C.Structure):
 class WaterTableArgs(C.Structure):
     """clmsoil_water_table_args (include/clmsoil_hydro.h)."""
     _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in WT_FIELDS]
+
+
+class ThetaWtArgs(C.Structure):
+    """clmsoil_theta_wt_args (include/clmsoil_hydro.h)."""
+    _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in THETA_WT_FIELDS]
 
 
 class HeatConfig(C.Structure):
@@ -149,6 +157,14 @@ def declare_common(lib: C.CDLL, prefix: str) -> C.CDLL:
     f.argtypes = [C.POINTER(WaterTableArgs), C.POINTER(Levels), dbl, dbl, C.c_int] + \
         ([P] if prefix == "clmsoil_" else [])
     f.restype = C.c_int
+    for name in ("perched_water_table", "theta_based_water_table", "clamp_zwt_to_bedrock"):
+        f = getattr(lib, prefix + name)
+        f.argtypes = [C.POINTER(ThetaWtArgs), C.POINTER(Levels)] + ([P] if prefix == "clmsoil_" else [])
+        f.restype = C.c_int
+    if prefix == "clmsoil_":
+        f = lib.clmsoil_water_table_moisture_form
+        f.argtypes = [C.POINTER(ThetaWtArgs), C.POINTER(Levels), P]
+        f.restype = C.c_int
     f = getattr(lib, prefix + "soil_energy_balance")
     f.argtypes = [C.POINTER(SebArgs), C.POINTER(Levels), dbl] + ([P] if prefix == "clmsoil_" else [])
     f.restype = C.c_int

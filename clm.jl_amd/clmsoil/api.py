@@ -24,7 +24,7 @@ from dataclasses import dataclass, field
 
 from . import _abi
 from .bind import (byref, fractions_args, ghf_args, heat_args, heat_config, levels, mf_args,
-                   seb_args, swm_config, water_table_args, zd_args)
+                   seb_args, swm_config, theta_wt_args, water_table_args, zd_args)
 
 ZENGDECKER_2009, MOISTURE_FORM = _abi.ZENGDECKER_2009, _abi.MOISTURE_FORM
 BC_HEAD, BC_FLUX, BC_ZERO_FLUX, BC_WATERTABLE, BC_AQUIFER = (
@@ -252,6 +252,47 @@ def water_table(state: dict, dtime: float, nlevsno: int = 12, nlevsoi: int = 20,
     _abi.check(L, L.clmsoil_water_table(byref(water_table_args(d)), byref(levels(d)), float(dtime),
                                         float(aq_sp_yield_min), int(bool(recompute_frost_table)),
                                         _stream()), "water_table")
+
+
+def _theta_wt(entry: str, state: dict, nlevsno: int, nlevsoi: int, nlevgrnd: int) -> None:
+    d = dict(state)
+    d.setdefault("nc", d["zwt"].shape[0])
+    d.update(nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd)
+    if int(d["nc"]) == 0:
+        return
+    L = _lib()
+    _abi.check(L, getattr(L, "clmsoil_" + entry)(byref(theta_wt_args(d)), byref(levels(d)),
+                                                 _stream()), entry)
+
+
+def perched_water_table(state: dict, nlevsno: int = 12, nlevsoi: int = 20,
+                        nlevgrnd: int = 25) -> None:
+    """perched_water_table! (soil_hydrology.jl:1604-1653): frost table at the top
+    of the first frozen layer under an unfrozen one, and the perched water table
+    above it. `state` maps the clmsoil_theta_wt_args fields
+    (include/clmsoil_hydro.h) to device tensors, plus `nc`."""
+    _theta_wt("perched_water_table", state, nlevsno, nlevsoi, nlevgrnd)
+
+
+def theta_based_water_table(state: dict, nlevsno: int = 12, nlevsoi: int = 20,
+                            nlevgrnd: int = 25) -> None:
+    """theta_based_water_table! (soil_hydrology.jl:1732-1765): the water table
+    where the saturation profile above bedrock falls to 90 %."""
+    _theta_wt("theta_based_water_table", state, nlevsno, nlevsoi, nlevgrnd)
+
+
+def clamp_zwt_to_bedrock(state: dict, nlevsno: int = 12, nlevsoi: int = 20,
+                         nlevgrnd: int = 25) -> None:
+    """clamp_zwt_to_bedrock! (src/infrastructure/kernels.jl:209-227)."""
+    _theta_wt("clamp_zwt_to_bedrock", state, nlevsno, nlevsoi, nlevgrnd)
+
+
+def water_table_moisture_form(state: dict, nlevsno: int = 12, nlevsoi: int = 20,
+                              nlevgrnd: int = 25) -> None:
+    """The water tables after a step without an aquifer layer
+    (clm_driver.jl:2076-2092): perched, theta-based, then the bedrock clamp, in
+    one launch."""
+    _theta_wt("water_table_moisture_form", state, nlevsno, nlevsoi, nlevgrnd)
 
 
 def soil_water(st, mfin, mask, nbedrock, nlevsno: int, nlevsoi: int, nlevgrnd: int,

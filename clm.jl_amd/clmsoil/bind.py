@@ -11,8 +11,9 @@ import ctypes as C
 import numpy as np
 
 from ._abi import (FRAC_FIELDS, GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_FIELDS, MF_FIELDS,
-                   SEB_FIELDS, WT_FIELDS, ZD_FIELDS, FractionsArgs, GhfArgs, HeatArgs, HeatConfig,
-                   Levels, MfArgs, SebArgs, SwmConfig, WaterTableArgs, ZdArgs)
+                   SEB_FIELDS, THETA_WT_FIELDS, WT_FIELDS, ZD_FIELDS, FractionsArgs, GhfArgs,
+                   HeatArgs, HeatConfig, Levels, MfArgs, SebArgs, SwmConfig, ThetaWtArgs,
+                   WaterTableArgs, ZdArgs)
 
 _INT_FIELDS = {"nbedrock", "imelt", "frac_veg_nosno"} | set(GHF_INT_FIELDS)
 _U8_FIELDS = {"mask", "mask_urbanp"} | set(GHF_U8_FIELDS)
@@ -131,6 +132,10 @@ def fractions_args(d: dict) -> FractionsArgs:
 
 def water_table_args(d: dict) -> WaterTableArgs:
     return _simple_args(WaterTableArgs, WT_FIELDS, d)
+
+
+def theta_wt_args(d: dict) -> ThetaWtArgs:
+    return _simple_args(ThetaWtArgs, THETA_WT_FIELDS, d)
 
 
 def swm_config(lower_bc=2, upper_bc=1, method=1, flux_calculation=1, dtmin=60.0,

@@ -18,6 +18,7 @@ import numpy as np
 NLEVSNO, NLEVSOI, NLEVGRND = 12, 20, 25
 SB = 5.67e-8
 DENH2O, DENICE = 1000.0, 917.0
+TFRZ = 273.15
 
 
 def clm5_grid(nlevsoi: int = NLEVSOI, nlevgrnd: int = NLEVGRND):
@@ -591,4 +592,34 @@ def water_table_columns(nc=4096, seed=16, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlev
              h2osoi_vol=d["h2osoi_vol"].copy(), qflx_drain=np.full(nc, 7.0),
              qflx_rsub_sat=np.full(nc, 7.0), qflx_drain_perched=np.full(nc, 7.0),
              excess_ice=exc, eff_porosity=np.zeros((nlevgrnd, nc)))
+    return d
+
+
+def moisture_water_table_columns(nc=4096, seed=40, nlevsno=NLEVSNO, nlevsoi=NLEVSOI,
+                                 nlevgrnd=NLEVGRND):
+    """Columns for the water tables diagnosed from the moisture profile after a
+    step without an aquifer layer (perched, theta-based, bedrock clamp): the
+    geometry, soil and temperature state of water_table_columns (frozen layers
+    below unfrozen ones on 30 % of the columns), bedrock at layer nbedrock ~
+    U{3..nlevsoi}, and a saturation profile that is wet (saturation ~ U(0.91, 1))
+    from a per-column layer k_sat ~ U{1..nlevsoi+1} down and drier (~ U(0.3, 0.89))
+    above it, so the 90 % level falls inside, above and below the column. Frozen
+    layers hold 30-100 % of their water as ice."""
+    d = water_table_columns(nc=nc, seed=seed, nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd)
+    rng = np.random.default_rng(seed + 7)
+    ns = nlevsno
+    lay = np.arange(1, nlevgrnd + 1)[:, None]
+    k_sat = rng.integers(1, nlevsoi + 2, nc)
+    sat = np.where(lay >= k_sat[None, :], rng.uniform(0.91, 1.0, (nlevgrnd, nc)),
+                   rng.uniform(0.3, 0.89, (nlevgrnd, nc)))
+    vol = sat * d["watsat"]
+    dz = d["dz"][ns:]
+    frozen = d["t_soisno"][ns:] <= TFRZ
+    f_ice = np.where(frozen, rng.uniform(0.3, 1.0, (nlevgrnd, nc)), 0.0)
+    d["h2osoi_ice"][ns:] = f_ice * vol * dz * DENICE
+    d["h2osoi_liq"][ns:] = (1.0 - f_ice) * vol * dz * DENH2O
+    d["nbedrock"] = rng.integers(3, nlevsoi + 1, nc).astype(np.int64)
+    d["zwt_perched"] = np.full(nc, -1.0)
+    d["frost_table"] = np.full(nc, -1.0)
+    d["h2osoi_vol"] = np.full((nlevgrnd, nc), -1.0)
     return d

@@ -444,6 +444,53 @@ int clmsoil_set_soil_water_fractions(const clmsoil_fractions_args *a, const clms
                : CLMSOIL_ERR_HIP;
 }
 
+// perched_water_table! / theta_based_water_table! / clamp_zwt_to_bedrock!
+// (soil_hydrology.jl:1604-1653, 1732-1765; kernels.jl:209-227), ops as in
+// launch_theta_water_table.
+static int theta_water_table(const clmsoil_theta_wt_args *a, const clmsoil_levels *lev, int ops,
+                             void *stream) {
+    if (!a) return CLMSOIL_ERR_NULL;
+    int st = validate_levels(lev);
+    if (st) return st;
+    if (a->nc < 0 || a->ld < a->nc) return CLMSOIL_ERR_SIZE;
+    if (a->nc == 0) return CLMSOIL_OK;
+    if (!a->zi || !a->zwt) return CLMSOIL_ERR_NULL;
+    if ((ops & 3) && (!a->dz || !a->watsat || !a->h2osoi_liq || !a->h2osoi_ice || !a->h2osoi_vol))
+        return CLMSOIL_ERR_NULL;
+    if ((ops & 1) && (!a->z || !a->t_soisno || !a->zwt_perched || !a->frost_table))
+        return CLMSOIL_ERR_NULL;
+    if ((ops & 2) && !a->z) return CLMSOIL_ERR_NULL;
+    if ((ops & 6) && !a->nbedrock) return CLMSOIL_ERR_NULL;
+    DeviceState *s = nullptr;
+    st = current_device_state(&s);
+    if (st) return st;
+    return launch_theta_water_table(*a, lev->nlevsno, lev->nlevsoi, ops, s->err,
+                                    (hipStream_t)stream) == hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
+}
+
+int clmsoil_perched_water_table(const clmsoil_theta_wt_args *a, const clmsoil_levels *lev,
+                                void *stream) {
+    return theta_water_table(a, lev, 1, stream);
+}
+
+int clmsoil_theta_based_water_table(const clmsoil_theta_wt_args *a, const clmsoil_levels *lev,
+                                    void *stream) {
+    return theta_water_table(a, lev, 2, stream);
+}
+
+int clmsoil_clamp_zwt_to_bedrock(const clmsoil_theta_wt_args *a, const clmsoil_levels *lev,
+                                 void *stream) {
+    return theta_water_table(a, lev, 4, stream);
+}
+
+// The non-aquifer water tables of clm_driver.jl:2076-2092 in one launch.
+int clmsoil_water_table_moisture_form(const clmsoil_theta_wt_args *a,
+                                      const clmsoil_levels *lev, void *stream) {
+    return theta_water_table(a, lev, 7, stream);
+}
+
 // water_table! (soil_hydrology.jl:849-893).
 int clmsoil_water_table(const clmsoil_water_table_args *a, const clmsoil_levels *lev,
                         double dtime, double aq_sp_yield_min, int recompute_frost_table,

@@ -90,6 +90,11 @@ int orc_soil_energy_balance(const clmsoil_seb_args *a, const clmsoil_levels *lev
 int orc_set_soil_water_fractions(const clmsoil_fractions_args *a, const clmsoil_levels *lev);
 int orc_water_table(const clmsoil_water_table_args *a, const clmsoil_levels *lev, double dtime,
                     double aq_sp_yield_min, int recompute_frost_table);
+/* Water tables from the moisture profile, soil_hydrology.jl:1522-1715 and
+ * src/infrastructure/kernels.jl:209-218 */
+int orc_perched_water_table(const clmsoil_theta_wt_args *a, const clmsoil_levels *lev);
+int orc_theta_based_water_table(const clmsoil_theta_wt_args *a, const clmsoil_levels *lev);
+int orc_clamp_zwt_to_bedrock(const clmsoil_theta_wt_args *a, const clmsoil_levels *lev);
 /* Test hook: a non-zero seed moves every pow() of the Zeng-Decker step one ulp
  * (direction from a hash of the operands and the seed); 0 restores exact pow(). */
 void orc_set_zd_pow_jitter(uint64_t seed);

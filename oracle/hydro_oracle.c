@@ -123,3 +123,110 @@ int orc_water_table(const clmsoil_water_table_args *a, const clmsoil_levels *lev
     }
     return CLMSOIL_OK;
 }
+
+/* ---- water tables diagnosed from the moisture profile (no aquifer layer) ---- */
+
+#define T_(X, c, j) (X)[(size_t)(c) + ((size_t)(j) - 1u) * (size_t)t->ld]
+
+static double t_vol(const clmsoil_theta_wt_args *t, int64_t c, int k, int joff) {
+    return T_(t->h2osoi_liq, c, k + joff) / (T_(t->dz, c, k + joff) * H_DENH2O) +
+           T_(t->h2osoi_ice, c, k + joff) / (T_(t->dz, c, k + joff) * H_DENICE);
+}
+
+static int theta_nbedrock_ok(const clmsoil_theta_wt_args *t, int n) {
+    for (int64_t c = 0; c < t->nc; ++c)
+        if ((!t->mask || t->mask[c]) && (t->nbedrock[c] < 1 || t->nbedrock[c] > n)) return 0;
+    return 1;
+}
+
+/* _perched_water_table_kernel! (soil_hydrology.jl:1522-1587) */
+int orc_perched_water_table(const clmsoil_theta_wt_args *t, const clmsoil_levels *lev) {
+    if (!t || !lev) return CLMSOIL_ERR_NULL;
+    const int n = lev->nlevsoi, joff = lev->nlevsno, joff_zi = lev->nlevsno + 1;
+    const double sat_lev = 0.9;
+    for (int64_t c = 0; c < t->nc; ++c) {
+        if (t->mask && !t->mask[c]) continue;
+        int k_frz = T_(t->t_soisno, c, 1 + joff) > H_TFRZ ? n : 1;
+        for (int k = 2; k <= n; ++k) {
+            if (T_(t->t_soisno, c, k - 1 + joff) > H_TFRZ && T_(t->t_soisno, c, k + joff) <= H_TFRZ) {
+                k_frz = k;
+                break;
+            }
+        }
+        t->frost_table[c] = T_(t->zi, c, k_frz - 1 + joff_zi);
+        t->zwt_perched[c] = t->frost_table[c];
+        if (t->zwt[c] < t->frost_table[c] && T_(t->t_soisno, c, k_frz + joff) <= H_TFRZ) {
+            /* water table above the frost table: nothing to do */
+        } else if (k_frz > 1) {
+            int k_perch = 1;
+            for (int k = k_frz; k >= 1; --k) {
+                T_(t->h2osoi_vol, c, k) = t_vol(t, c, k, joff);
+                if (T_(t->h2osoi_vol, c, k) / T_(t->watsat, c, k) <= sat_lev) {
+                    k_perch = k;
+                    break;
+                }
+            }
+            if (T_(t->t_soisno, c, k_frz + joff) > H_TFRZ) k_perch = k_frz;
+            if (k_frz > k_perch) {
+                const double s1 = t_vol(t, c, k_perch, joff) / T_(t->watsat, c, k_perch);
+                const double s2 = t_vol(t, c, k_perch + 1, joff) / T_(t->watsat, c, k_perch + 1);
+                if (s1 > s2) {
+                    t->zwt_perched[c] = T_(t->zi, c, k_perch - 1 + joff_zi);
+                } else {
+                    const double m = (T_(t->z, c, k_perch + 1 + joff) - T_(t->z, c, k_perch + joff)) /
+                                     (s2 - s1);
+                    const double b = T_(t->z, c, k_perch + 1 + joff) - m * s2;
+                    t->zwt_perched[c] = JMAX(0.0, m * sat_lev + b);
+                }
+            }
+        }
+    }
+    return CLMSOIL_OK;
+}
+
+/* _theta_based_water_table_kernel! (soil_hydrology.jl:1670-1715) */
+int orc_theta_based_water_table(const clmsoil_theta_wt_args *t, const clmsoil_levels *lev) {
+    if (!t || !lev) return CLMSOIL_ERR_NULL;
+    const int n = lev->nlevsoi, joff = lev->nlevsno, joff_zi = lev->nlevsno + 1;
+    const double sat_lev = 0.9;
+    if (!theta_nbedrock_ok(t, n)) return CLMSOIL_ERR_SIZE;
+    for (int64_t c = 0; c < t->nc; ++c) {
+        if (t->mask && !t->mask[c]) continue;
+        const int nbed = (int)t->nbedrock[c];
+        t->zwt[c] = T_(t->zi, c, n + joff_zi);
+        int k_zwt = nbed, sat_flag = 1;
+        for (int k = nbed; k >= 1; --k) {
+            T_(t->h2osoi_vol, c, k) = t_vol(t, c, k, joff);
+            if (T_(t->h2osoi_vol, c, k) / T_(t->watsat, c, k) <= sat_lev) {
+                k_zwt = k;
+                sat_flag = 0;
+                break;
+            }
+        }
+        if (sat_flag == 1) k_zwt = 1;
+        if (k_zwt == 1) {
+            t->zwt[c] = T_(t->zi, c, 1 + joff_zi);
+        } else if (k_zwt < nbed) {
+            const double s1 = t_vol(t, c, k_zwt, joff) / T_(t->watsat, c, k_zwt);
+            const double s2 = t_vol(t, c, k_zwt + 1, joff) / T_(t->watsat, c, k_zwt + 1);
+            const double m = (T_(t->z, c, k_zwt + 1 + joff) - T_(t->z, c, k_zwt + joff)) / (s2 - s1);
+            const double b = T_(t->z, c, k_zwt + 1 + joff) - m * s2;
+            t->zwt[c] = JMAX(0.0, m * sat_lev + b);
+        } else {
+            t->zwt[c] = T_(t->zi, c, nbed + joff_zi);
+        }
+    }
+    return CLMSOIL_OK;
+}
+
+/* _clamp_zwt_bedrock_kernel! (src/infrastructure/kernels.jl:209-218) */
+int orc_clamp_zwt_to_bedrock(const clmsoil_theta_wt_args *t, const clmsoil_levels *lev) {
+    if (!t || !lev) return CLMSOIL_ERR_NULL;
+    if (!theta_nbedrock_ok(t, lev->nlevsoi)) return CLMSOIL_ERR_SIZE;
+    for (int64_t c = 0; c < t->nc; ++c) {
+        if (t->mask && !t->mask[c]) continue;
+        const double zb = T_(t->zi, c, t->nbedrock[c] + lev->nlevsno + 1);
+        if (t->zwt[c] > zb) t->zwt[c] = zb;
+    }
+    return CLMSOIL_OK;
+}

@@ -17,7 +17,7 @@ sys.path.insert(0, str(REPO / "clm.jl_amd"))
 
 from clmsoil import _abi  # noqa: E402
 from clmsoil.bind import (byref, fractions_args, ghf_args, heat_args, levels, mf_args,  # noqa: E402
-                          seb_args, water_table_args, zd_args)
+                          seb_args, theta_wt_args, water_table_args, zd_args)
 
 LIB_PATH = ORACLE_DIR / "liboracle_clmsoil.so"
 _lib = None
@@ -149,3 +149,26 @@ def num_threads() -> int:
 
 
 os.environ.setdefault("OMP_NUM_THREADS", str(os.cpu_count() or 1))
+
+
+def perched_water_table(d: dict) -> None:
+    L = lib()
+    check(L.orc_perched_water_table(byref(theta_wt_args(d)), byref(levels(d))), "perched_water_table")
+
+
+def theta_based_water_table(d: dict) -> None:
+    L = lib()
+    check(L.orc_theta_based_water_table(byref(theta_wt_args(d)), byref(levels(d))),
+          "theta_based_water_table")
+
+
+def clamp_zwt_to_bedrock(d: dict) -> None:
+    L = lib()
+    check(L.orc_clamp_zwt_to_bedrock(byref(theta_wt_args(d)), byref(levels(d))), "clamp_zwt_to_bedrock")
+
+
+def water_table_moisture_form(d: dict) -> None:
+    """The non-aquifer sequence of clm_driver.jl:2076-2092: perched, theta-based, clamp."""
+    perched_water_table(d)
+    theta_based_water_table(d)
+    clamp_zwt_to_bedrock(d)

@@ -23,6 +23,7 @@ STRUCTS = {
     "clmsoil_seb_args": _abi.SebArgs,
     "clmsoil_fractions_args": _abi.FractionsArgs,
     "clmsoil_water_table_args": _abi.WaterTableArgs,
+    "clmsoil_theta_wt_args": _abi.ThetaWtArgs,
 }
 
 

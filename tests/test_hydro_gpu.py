@@ -4,7 +4,9 @@ Bars: fractions bit-exact (divisions and min/max only). Water table: the
 specific yield takes one pow() per layer visited (the only operation whose last
 bit may differ between the GPU and the host library), so zwt and wa are compared
 at 1e-11 relative, the frost table, the drainage fluxes and the water contents
-written on the way bit-exact, and zwt_perched at 1e-11 relative."""
+written on the way bit-exact, and zwt_perched at 1e-11 relative. The water
+tables diagnosed from the moisture profile (perched, theta-based, bedrock clamp)
+are bit-exact."""
 import numpy as np
 import pytest
 
@@ -77,3 +79,39 @@ def test_zd09_then_water_table_conserves_water(gpu):
     expect = (d["qflx_infl"] - d["qflx_rootsoi"].sum(0)) * 1800.0
     scale = np.abs(d["h2osoi_liq"][ns:ns + n]).sum(0) + d["wa"]
     assert np.max(np.abs(soil + dwa - expect) / scale) < 1e-13
+
+
+TW = ("mask", "nbedrock", "dz", "z", "zi", "t_soisno", "watsat", "h2osoi_liq", "h2osoi_ice",
+      "zwt", "zwt_perched", "frost_table", "h2osoi_vol")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("entry", ["perched_water_table", "theta_based_water_table",
+                                   "clamp_zwt_to_bedrock", "water_table_moisture_form"])
+def test_moisture_water_tables_vs_oracle(gpu, entry):
+    """Bit-exact: divisions, products, sums and max only, no contraction."""
+    d = inputs.moisture_water_table_columns(nc=4101, seed=50)
+    d["mask"] = (np.arange(d["nc"]) % 7 != 3).astype(np.uint8)
+    got = run_gpu(d, getattr(api, entry), TW)
+    ref = clone(d)
+    getattr(oracle, entry)(ref)
+    for k in ("zwt", "zwt_perched", "frost_table", "h2osoi_vol"):
+        assert np.array_equal(got[k], ref[k]), (entry, k)
+    off = d["mask"] == 0
+    assert np.array_equal(got["zwt"][off], d["zwt"][off])
+
+
+@pytest.mark.gpu
+def test_moisture_water_table_bad_nbedrock_flags(gpu):
+    d = inputs.moisture_water_table_columns(nc=300, seed=51)
+    d["nbedrock"][17] = 0
+    api.device_errors()                                   # clear
+    got = run_gpu(d, api.water_table_moisture_form, TW)
+    assert api.device_errors() & 1                        # CLMSOIL_DEVERR_NBEDROCK
+    ref = clone(d)
+    ref["nbedrock"][17] = d["nlevsoi"]
+    ref["mask"] = (np.arange(d["nc"]) != 17).astype(np.uint8)
+    oracle.water_table_moisture_form(ref)
+    ok = np.arange(d["nc"]) != 17
+    assert np.array_equal(got["zwt"][ok], ref["zwt"][ok])
+    assert got["zwt"][17] == d["zwt"][17]                 # left unchanged
