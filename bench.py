@@ -295,6 +295,45 @@ def surface_energy(nc, steps):
                                                   "column), 0-12 snow layers + 25 ground layers")}
 
 
+# clmsoil_surface_hydrology with the driver's settings: bytes each column
+# reads (masks 3 x 1 B, 6 integer indices, 1 flag, 25 doubles: forcing,
+# fractions, tables, 3 layers of icefrac and hksat, slope, threshold, h2osfc,
+# gathered flood water) and writes (17 fluxes and states)
+SURFHYD_READ_BYTES = 3 * 1 + 6 * 8 + 1 + 25 * 8
+SURFHYD_WRITE_BYTES = 17 * 8
+
+
+def surface_hydrology_step(nc, steps):
+    """The surface hydrology before the Richards step (flood water, saturated
+    and infiltration excess runoff, surface water, infiltration, total runoff,
+    urban ponding) in one launch on `nc` columns of mixed landunits (55 % soil,
+    15 % crop, 15 % urban, 8 % lake, 7 % wetland); `steps` launches on the same
+    inputs, each bracketed by HIP events on its stream."""
+    import torch
+    from clmsoil import api, inputs
+    from clmsoil._abi import SH_FIELDS
+    d = inputs.surface_hydrology_columns(nc=nc, seed=53)
+    t = api.to_device(d)
+    st = {k: t[k] for k in SH_FIELDS if k in t} | {"nc": nc}
+    h0 = t["h2osfc"].clone()
+    for _ in range(2):
+        api.surface_hydrology(st, 1800.0)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
+    for e in ev:
+        t["h2osfc"].copy_(h0)
+        e[0].record()
+        api.surface_hydrology(st, 1800.0)
+        e[1].record()
+    torch.cuda.synchronize()
+    ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    by = (SURFHYD_READ_BYTES + SURFHYD_WRITE_BYTES) * nc
+    return {"workload": f"{nc} columns, mixed soil/crop/urban/lake/wetland landunits",
+            "ms_per_launch": ms, "columns_per_s": nc / (ms / 1e3),
+            "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                         "bytes_per_column": SURFHYD_READ_BYTES + SURFHYD_WRITE_BYTES}}
+
+
 def water_tables_moisture_form(nc, steps):
     """The water tables after a step without an aquifer layer (perched,
     theta-based, bedrock clamp in one launch) on `nc` columns of 20 soil layers
@@ -349,7 +388,7 @@ def main(argv=None):
                     help="run only the Zeng-Decker 2009 section (for profiling its kernel)")
     ap.add_argument("--hydro-only", action="store_true",
                     help="run only the soil hydrology sections (aquifer step, water tables "
-                         "without an aquifer)")
+                         "without an aquifer, surface hydrology)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -384,8 +423,8 @@ def main(argv=None):
     if args.hydro_only:
         print(json.dumps({"zengdecker_2009": zengdecker_2009(
             nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample),
-            "water_tables_moisture_form": water_tables_moisture_form(nc, args.steps)}),
-            flush=True)
+            "water_tables_moisture_form": water_tables_moisture_form(nc, args.steps),
+            "surface_hydrology": surface_hydrology_step(nc, args.steps)}), flush=True)
         return 0
     host, dev, gen_s = make_state(nc, args.seed)
     print(f"# state ready: {nc} columns in {gen_s:.1f} s", file=sys.stderr, flush=True)
@@ -474,6 +513,7 @@ def main(argv=None):
         out["zengdecker_2009"] = zengdecker_2009(
             nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample)
         out["water_tables_moisture_form"] = water_tables_moisture_form(nc, args.steps)
+        out["surface_hydrology"] = surface_hydrology_step(nc, args.steps)
     if not args.no_cpu_baseline:
         v, cores, secs = cpu_baseline(host, min(args.cpu_sample, nc), args.cpu_steps)
         out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": cores,

@@ -116,6 +116,25 @@ WT_FIELDS = ["mask", "dz", "z", "zi", "t_soisno", "watsat", "sucsat", "bsw", "qc
 THETA_WT_FIELDS = ["mask", "nbedrock", "dz", "z", "zi", "t_soisno", "watsat", "h2osoi_liq",
                    "h2osoi_ice", "zwt", "zwt_perched", "frost_table", "h2osoi_vol"]
 
+SH_INT_FIELDS = ["col_gridcell", "col_landunit", "lun_itype", "col_itype", "snl", "col_cold"]
+SH_U8_FIELDS = ["mask_nolakec", "mask_hydrologyc", "mask_urbanc", "col_urbpoi",
+                "is_hillslope_column", "col_active"]
+SH_FIELDS = ["mask_nolakec", "mask_hydrologyc", "mask_urbanc", "col_gridcell", "col_landunit",
+             "lun_itype", "col_itype", "snl", "col_cold", "col_urbpoi", "is_hillslope_column",
+             "col_active", "qflx_floodg", "frost_table", "zwt", "zwt_perched", "wtfact", "b_infil",
+             "top_max_moist", "top_moist_limited", "qflx_rain_plus_snomelt", "qflx_snow_h2osfc",
+             "qflx_liqevap_from_top_layer", "qflx_ev_soil", "qflx_ev_h2osfc", "frac_sno_eff",
+             "frac_h2osfc", "frac_h2osfc_nosnow", "icefrac", "hksat", "topo_slope",
+             "h2osfc_thresh", "h2osoi_liq", "h2osfc", "xs_urban", "qflx_floodc", "fsat", "fcov",
+             "qflx_sat_excess_surf", "qflx_top_soil", "qflx_in_soil", "qflx_top_soil_to_h2osfc",
+             "qinmax", "qflx_infl_excess", "qflx_in_soil_limited", "qflx_in_h2osfc",
+             "qflx_infl_excess_surf", "qflx_h2osfc_surf", "qflx_h2osfc_drain", "qflx_infl",
+             "qflx_surf"]
+SH_STEPS = {"floodc": 0x001, "sat_excess": 0x002, "qflx_inputs": 0x004, "infil_excess": 0x008,
+            "route": 0x010, "h2osfc": 0x020, "infiltration": 0x040, "surf_runoff": 0x080,
+            "urban_pond": 0x100}
+SH_ALL = 0x1ff
+
 
 class FractionsArgs(C.Structure):
     """clmsoil_fractions_args (include/clmsoil_hydro.h)."""
@@ -130,6 +149,20 @@ class WaterTableArgs(C.Structure):
 class ThetaWtArgs(C.Structure):
     """clmsoil_theta_wt_args (include/clmsoil_hydro.h)."""
     _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in THETA_WT_FIELDS]
+
+
+class SurfHydArgs(C.Structure):
+    """clmsoil_surfhyd_args (include/clmsoil_surfhyd.h)."""
+    _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in SH_FIELDS]
+
+
+class SurfHydParams(C.Structure):
+    """clmsoil_surfhyd_params (include/clmsoil_surfhyd.h)."""
+    _fields_ = [("steps", C.c_int32), ("fsat_method", C.c_int32), ("qinmax_method", C.c_int32),
+                ("h2osfcflag", C.c_int32), ("h2osfcflag_update", C.c_int32),
+                ("crop_fsat_equals_zero", C.c_int32), ("hillslope_fsat_equals_zero", C.c_int32),
+                ("reserved", C.c_int32), ("fff", C.c_double), ("e_ice", C.c_double),
+                ("pc", C.c_double), ("mu", C.c_double), ("pondmx_urban", C.c_double)]
 
 
 class HeatConfig(C.Structure):
@@ -165,6 +198,10 @@ def declare_common(lib: C.CDLL, prefix: str) -> C.CDLL:
         f = lib.clmsoil_water_table_moisture_form
         f.argtypes = [C.POINTER(ThetaWtArgs), C.POINTER(Levels), P]
         f.restype = C.c_int
+    f = getattr(lib, prefix + "surface_hydrology")
+    # last argument: the stream (product) or 2*nc doubles of scratch (oracle)
+    f.argtypes = [C.POINTER(SurfHydArgs), C.POINTER(Levels), dbl, C.POINTER(SurfHydParams), P]
+    f.restype = C.c_int
     f = getattr(lib, prefix + "soil_energy_balance")
     f.argtypes = [C.POINTER(SebArgs), C.POINTER(Levels), dbl] + ([P] if prefix == "clmsoil_" else [])
     f.restype = C.c_int

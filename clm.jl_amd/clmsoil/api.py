@@ -24,7 +24,8 @@ from dataclasses import dataclass, field
 
 from . import _abi
 from .bind import (byref, fractions_args, ghf_args, heat_args, heat_config, levels, mf_args,
-                   seb_args, swm_config, theta_wt_args, water_table_args, zd_args)
+                   seb_args, surfhyd_args, surfhyd_params, swm_config, theta_wt_args,
+                   water_table_args, zd_args)
 
 ZENGDECKER_2009, MOISTURE_FORM = _abi.ZENGDECKER_2009, _abi.MOISTURE_FORM
 BC_HEAD, BC_FLUX, BC_ZERO_FLUX, BC_WATERTABLE, BC_AQUIFER = (
@@ -240,7 +241,7 @@ def set_soil_water_fractions(state: dict, nlevsno: int = 12, nlevsoi: int = 20,
 def water_table(state: dict, dtime: float, nlevsno: int = 12, nlevsoi: int = 20,
                 nlevgrnd: int = 25, aq_sp_yield_min: float = 0.01,
                 recompute_frost_table: bool = True) -> None:
-    """water_table! (soil_hydrology.jl:849-893): the step's aquifer recharge moves
+    """water_table! (soil_hydrology.jl:849-902): the step's aquifer recharge moves
     the water table; frost and perched water tables. `state` maps the
     clmsoil_water_table_args fields to device tensors, plus `nc`."""
     d = dict(state)
@@ -252,6 +253,26 @@ def water_table(state: dict, dtime: float, nlevsno: int = 12, nlevsoi: int = 20,
     _abi.check(L, L.clmsoil_water_table(byref(water_table_args(d)), byref(levels(d)), float(dtime),
                                         float(aq_sp_yield_min), int(bool(recompute_frost_table)),
                                         _stream()), "water_table")
+
+
+def surface_hydrology(state: dict, dtime: float, nlevsno: int = 12, nlevsoi: int = 20,
+                      nlevgrnd: int = 25, **params) -> None:
+    """The surface hydrology before the Richards step (clm_driver.jl:1939-2012):
+    set_floodc!, saturated_excess_runoff!, set_qflx_inputs!,
+    infiltration_excess_runoff!, route_infiltration_excess!, update_h2osfc!,
+    infiltration!, total_surface_runoff!, update_urban_ponding!, in one launch.
+    `state` maps the clmsoil_surfhyd_args fields (include/clmsoil_surfhyd.h) to
+    device tensors, plus `nc`; `params` are the keywords of bind.surfhyd_params
+    (steps=bind.SH_ALL for the whole sequence)."""
+    d = dict(state)
+    d.setdefault("nc", d["h2osfc"].shape[0])
+    d.update(nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd)
+    if int(d["nc"]) == 0:
+        return
+    L = _lib()
+    _abi.check(L, L.clmsoil_surface_hydrology(byref(surfhyd_args(d)), byref(levels(d)),
+                                              float(dtime), byref(surfhyd_params(**params)),
+                                              _stream()), "surface_hydrology")
 
 
 def _theta_wt(entry: str, state: dict, nlevsno: int, nlevsoi: int, nlevgrnd: int) -> None:

@@ -11,12 +11,13 @@ import ctypes as C
 import numpy as np
 
 from ._abi import (FRAC_FIELDS, GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_FIELDS, MF_FIELDS,
-                   SEB_FIELDS, THETA_WT_FIELDS, WT_FIELDS, ZD_FIELDS, FractionsArgs, GhfArgs,
-                   HeatArgs, HeatConfig, Levels, MfArgs, SebArgs, SwmConfig, ThetaWtArgs,
+                   SEB_FIELDS, SH_ALL, SH_FIELDS, SH_INT_FIELDS, SH_U8_FIELDS, THETA_WT_FIELDS,
+                   WT_FIELDS, ZD_FIELDS, FractionsArgs, GhfArgs, HeatArgs, HeatConfig, Levels,
+                   MfArgs, SebArgs, SurfHydArgs, SurfHydParams, SwmConfig, ThetaWtArgs,
                    WaterTableArgs, ZdArgs)
 
-_INT_FIELDS = {"nbedrock", "imelt", "frac_veg_nosno"} | set(GHF_INT_FIELDS)
-_U8_FIELDS = {"mask", "mask_urbanp"} | set(GHF_U8_FIELDS)
+_INT_FIELDS = {"nbedrock", "imelt", "frac_veg_nosno"} | set(GHF_INT_FIELDS) | set(SH_INT_FIELDS)
+_U8_FIELDS = {"mask", "mask_urbanp"} | set(GHF_U8_FIELDS) | set(SH_U8_FIELDS)
 
 
 def _ptr(x):
@@ -136,6 +137,24 @@ def water_table_args(d: dict) -> WaterTableArgs:
 
 def theta_wt_args(d: dict) -> ThetaWtArgs:
     return _simple_args(ThetaWtArgs, THETA_WT_FIELDS, d)
+
+
+def surfhyd_args(d: dict) -> SurfHydArgs:
+    return _simple_args(SurfHydArgs, SH_FIELDS, d)
+
+
+def surfhyd_params(steps=SH_ALL, fsat_method=1, qinmax_method=1, h2osfcflag=1,
+                   h2osfcflag_update=1, crop_fsat_equals_zero=False,
+                   hillslope_fsat_equals_zero=False, fff=0.5, e_ice=6.0, pc=0.4, mu=0.13889,
+                   pondmx_urban=1.0) -> SurfHydParams:
+    """The driver's settings: TOPMODEL fsat, hksat qinmax, surface water on
+    (soil_hydrology.jl:20, surface_water.jl:19-20, infilt_excess_runoff.jl:25,
+    varcon.jl:68). The reference reads fff from its parameter file
+    (read_params.jl:22-23); the default here is 0.5 /m."""
+    return SurfHydParams(int(steps), int(fsat_method), int(qinmax_method), int(h2osfcflag),
+                         int(h2osfcflag_update), int(bool(crop_fsat_equals_zero)),
+                         int(bool(hillslope_fsat_equals_zero)), 0, float(fff), float(e_ice),
+                         float(pc), float(mu), float(pondmx_urban))
 
 
 def swm_config(lower_bc=2, upper_bc=1, method=1, flux_calculation=1, dtmin=60.0,

@@ -623,3 +623,70 @@ def moisture_water_table_columns(nc=4096, seed=40, nlevsno=NLEVSNO, nlevsoi=NLEV
     d["frost_table"] = np.full(nc, -1.0)
     d["h2osoi_vol"] = np.full((nlevgrnd, nc), -1.0)
     return d
+
+
+def surface_hydrology_columns(nc=4096, seed=60, nlevsno=NLEVSNO, nlevgrnd=NLEVGRND):
+    """Columns for the surface hydrology before the Richards step: one landunit
+    per column, 55 % soil, 15 % crop, 15 % urban (roof, sunwall, shadewall,
+    impervious and pervious road in turn), 8 % deep lake, 7 % wetland, on
+    gridcells of about 4 columns. Masks as CLM's filters: no-lake, hydrology
+    (soil, crop, pervious road), urban. Forcing: rain plus snowmelt ~ U(0, 5e-3)
+    mm/s with a fifth of the columns at U(0, 5e-2) (beyond hksat, so
+    infiltration excess occurs); flood water on a tenth of the gridcells; 30 %
+    snow-covered columns; surface water fractions with 40 % of the columns dry;
+    ponded h2osfc above and below its threshold (a few slightly negative);
+    frozen top layers on 30 % of the columns; water, perched and frost tables
+    that take both TOPMODEL branches; VIC inputs; hillslope flags."""
+    rng = np.random.default_rng(seed)
+    ng = max(1, nc // 4)
+    u = rng.uniform(size=nc)
+    lun = np.where(u < 0.55, 1, np.where(u < 0.70, 2, np.where(u < 0.85, 8,
+                   np.where(u < 0.93, 5, 6)))).astype(np.int64)
+    urban = lun == 8
+    itype = lun.copy()
+    itype[urban] = 71 + (np.cumsum(urban)[urban] - 1) % 5
+    hyd = (lun == 1) | (lun == 2) | (itype == 75)
+    nlev = nlevsno + nlevgrnd
+    wet = rng.uniform(size=nc) < 0.2
+    rain = np.where(wet, rng.uniform(0.0, 5e-2, nc), rng.uniform(0.0, 5e-3, nc))
+    snl = np.where(rng.uniform(size=nc) < 0.3, -rng.integers(1, 4, nc), 0).astype(np.int64)
+    fh = np.where(rng.uniform(size=nc) < 0.4, 0.0, rng.uniform(0.0, 0.3, nc))
+    frozen = rng.uniform(size=nc) < 0.3
+    icefrac = np.zeros((nlevgrnd, nc))
+    icefrac[:3, frozen] = rng.uniform(0.0, 0.8, (3, int(frozen.sum())))
+    zwt = rng.uniform(0.1, 5.0, nc)
+    h2osfc = np.where(rng.uniform(size=nc) < 0.3, 0.0, rng.uniform(0.0, 100.0, nc))
+    h2osfc[rng.uniform(size=nc) < 0.02] = -rng.uniform(0.0, 1e-3)
+    liq = np.zeros((nlev, nc))
+    liq[nlevsno] = rng.uniform(0.0, 1.5, nc)
+    top_max = rng.uniform(100.0, 500.0, nc)
+    sentinel = lambda: np.full(nc, 7.0)
+    d = dict(nc=nc, ng=ng, nl=nc, nlevsno=nlevsno, nlevsoi=20, nlevgrnd=nlevgrnd,
+             mask_nolakec=(lun != 5).astype(np.uint8), mask_hydrologyc=hyd.astype(np.uint8),
+             mask_urbanc=urban.astype(np.uint8),
+             col_gridcell=rng.integers(1, ng + 1, nc).astype(np.int64),
+             col_landunit=np.arange(1, nc + 1, dtype=np.int64), lun_itype=lun, col_itype=itype,
+             snl=snl, col_cold=np.where(rng.uniform(size=nc) < 0.5, -9999,
+                                        rng.integers(1, nc + 1, nc)).astype(np.int64),
+             col_urbpoi=urban.astype(np.uint8),
+             is_hillslope_column=(rng.uniform(size=nc) < 0.3).astype(np.uint8),
+             col_active=(rng.uniform(size=nc) < 0.9).astype(np.uint8),
+             qflx_floodg=np.where(rng.uniform(size=ng) < 0.1, rng.uniform(0.0, 1e-3, ng), 0.0),
+             frost_table=rng.uniform(0.0, 6.0, nc), zwt=zwt,
+             zwt_perched=zwt * rng.uniform(0.0, 1.0, nc), wtfact=rng.uniform(0.2, 0.5, nc),
+             b_infil=rng.uniform(0.01, 0.5, nc), top_max_moist=top_max,
+             top_moist_limited=top_max * rng.uniform(0.0, 1.0, nc),
+             qflx_rain_plus_snomelt=rain, qflx_snow_h2osfc=rng.uniform(0.0, 1e-4, nc),
+             qflx_liqevap_from_top_layer=rng.normal(1e-5, 2e-5, nc),
+             qflx_ev_soil=rng.normal(1e-5, 2e-5, nc), qflx_ev_h2osfc=rng.normal(1e-5, 2e-5, nc),
+             frac_sno_eff=np.where(snl < 0, rng.uniform(0.2, 1.0, nc) * (1.0 - fh), 0.0),
+             frac_h2osfc=fh, frac_h2osfc_nosnow=np.minimum(1.0, fh + rng.uniform(0.0, 0.7, nc)),
+             icefrac=icefrac, hksat=rng.uniform(0.001, 0.05, (nlevgrnd, nc)),
+             topo_slope=rng.uniform(0.0, 30.0, nc), h2osfc_thresh=rng.uniform(0.0, 50.0, nc),
+             h2osoi_liq=liq, h2osfc=h2osfc, xs_urban=np.zeros(nc))
+    for k in ("qflx_floodc", "fsat", "fcov", "qflx_sat_excess_surf", "qflx_top_soil",
+              "qflx_in_soil", "qflx_top_soil_to_h2osfc", "qinmax", "qflx_infl_excess",
+              "qflx_in_soil_limited", "qflx_in_h2osfc", "qflx_infl_excess_surf",
+              "qflx_h2osfc_surf", "qflx_h2osfc_drain", "qflx_infl", "qflx_surf"):
+        d[k] = sentinel()
+    return d

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <initializer_list>
 #include <cstdlib>
 #include <mutex>
 #include <vector>
@@ -18,6 +19,7 @@
 #include "../../include/clmsoil_zd09.h"
 #include "../../include/clmsoil_ghf.h"
 #include "../../include/clmsoil_hydro.h"
+#include "../../include/clmsoil_surfhyd.h"
 #include "common.hpp"
 #include "heat.hpp"
 #include "richards.hpp"
@@ -26,6 +28,7 @@
 #include "zd09.hpp"
 #include "ghf.hpp"
 #include "hydro.hpp"
+#include "surfhyd.hpp"
 
 using namespace clmsoil;
 
@@ -491,7 +494,75 @@ int clmsoil_water_table_moisture_form(const clmsoil_theta_wt_args *a,
     return theta_water_table(a, lev, 7, stream);
 }
 
-// water_table! (soil_hydrology.jl:849-893).
+// The surface hydrology of clm_driver.jl:1939-2012 (include/clmsoil_surfhyd.h).
+int clmsoil_surface_hydrology(const clmsoil_surfhyd_args *a, const clmsoil_levels *lev,
+                              double dtime, const clmsoil_surfhyd_params *p, void *stream) {
+    if (!a || !p) return CLMSOIL_ERR_NULL;
+    int st = validate_levels(lev);
+    if (st) return st;
+    if (a->nc < 0 || a->ld < a->nc) return CLMSOIL_ERR_SIZE;
+    if (!(dtime > 0.0) || !std::isfinite(dtime)) return CLMSOIL_ERR_ARG;
+    const int s = p->steps;
+    if ((s & ~CLMSOIL_SH_ALL) != 0) return CLMSOIL_ERR_ARG;
+    if (p->fsat_method != CLMSOIL_FSAT_TOPMODEL && p->fsat_method != CLMSOIL_FSAT_VIC)
+        return CLMSOIL_ERR_ARG;
+    if (p->qinmax_method != CLMSOIL_QINMAX_NONE && p->qinmax_method != CLMSOIL_QINMAX_HKSAT)
+        return CLMSOIL_ERR_ARG;
+    if (a->nc == 0 || s == 0) return CLMSOIL_OK;
+    auto need = [](std::initializer_list<const void *> l) {
+        for (const void *q : l)
+            if (!q) return false;
+        return true;
+    };
+    bool ok = true;
+    if (s & CLMSOIL_SH_FLOODC)
+        ok = ok && need({a->col_gridcell, a->col_itype, a->qflx_floodg, a->qflx_floodc});
+    if (s & CLMSOIL_SH_SAT_EXCESS) {
+        ok = ok && need({a->col_urbpoi, a->qflx_rain_plus_snomelt, a->qflx_floodc, a->fsat,
+                         a->fcov, a->qflx_sat_excess_surf});
+        ok = ok && (p->fsat_method == CLMSOIL_FSAT_TOPMODEL
+                        ? need({a->frost_table, a->zwt, a->zwt_perched, a->wtfact})
+                        : need({a->b_infil, a->top_max_moist, a->top_moist_limited}));
+        if (p->crop_fsat_equals_zero) ok = ok && need({a->col_landunit, a->lun_itype});
+        if (p->hillslope_fsat_equals_zero)
+            ok = ok && need({a->is_hillslope_column, a->col_active, a->col_cold});
+    }
+    if (s & CLMSOIL_SH_QFLX_INPUTS)
+        ok = ok && need({a->snl, a->qflx_rain_plus_snomelt, a->qflx_snow_h2osfc, a->qflx_floodc,
+                         a->qflx_liqevap_from_top_layer, a->qflx_ev_soil, a->qflx_ev_h2osfc,
+                         a->qflx_sat_excess_surf, a->frac_sno_eff, a->frac_h2osfc,
+                         a->qflx_top_soil, a->qflx_in_soil, a->qflx_top_soil_to_h2osfc});
+    if (s & CLMSOIL_SH_INFIL_EXCESS) {
+        ok = ok && need({a->fsat, a->qflx_in_soil, a->frac_h2osfc, a->qinmax,
+                         a->qflx_infl_excess});
+        if (p->qinmax_method == CLMSOIL_QINMAX_HKSAT) ok = ok && need({a->icefrac, a->hksat});
+    }
+    if (s & CLMSOIL_SH_ROUTE)
+        ok = ok && need({a->col_landunit, a->lun_itype, a->qflx_in_soil, a->qflx_infl_excess,
+                         a->qflx_top_soil_to_h2osfc, a->qflx_in_soil_limited,
+                         a->qflx_in_h2osfc, a->qflx_infl_excess_surf});
+    if (s & CLMSOIL_SH_H2OSFC)
+        ok = ok && need({a->h2osfc, a->qflx_in_h2osfc, a->frac_h2osfc, a->qinmax,
+                         a->qflx_h2osfc_surf, a->qflx_h2osfc_drain}) &&
+             (p->h2osfcflag_update != 1 ||
+              need({a->frac_h2osfc_nosnow, a->h2osfc_thresh, a->topo_slope}));
+    if (s & CLMSOIL_SH_INFILTRATION)
+        ok = ok && need({a->qflx_in_soil_limited, a->qflx_h2osfc_drain, a->qflx_infl});
+    if (s & CLMSOIL_SH_SURF_RUNOFF)
+        ok = ok && need({a->qflx_sat_excess_surf, a->qflx_infl_excess_surf, a->qflx_h2osfc_surf,
+                         a->qflx_surf});
+    if (s & (CLMSOIL_SH_SURF_RUNOFF | CLMSOIL_SH_URBAN_POND))
+        ok = ok && need({a->col_itype, a->snl, a->qflx_rain_plus_snomelt,
+                         a->qflx_liqevap_from_top_layer, a->qflx_floodc, a->h2osoi_liq,
+                         a->xs_urban});
+    if (!ok) return CLMSOIL_ERR_NULL;
+    return launch_surface_hydrology(*a, *p, lev->nlevsno, dtime, (hipStream_t)stream) ==
+                   hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
+}
+
+// water_table! (soil_hydrology.jl:849-902).
 int clmsoil_water_table(const clmsoil_water_table_args *a, const clmsoil_levels *lev,
                         double dtime, double aq_sp_yield_min, int recompute_frost_table,
                         void *stream) {

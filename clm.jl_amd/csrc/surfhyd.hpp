@@ -1,0 +1,13 @@
+// surfhyd.hpp — launch interface of the surface hydrology kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/clmsoil_surfhyd.h"
+
+namespace clmsoil {
+
+hipError_t launch_surface_hydrology(const clmsoil_surfhyd_args &a,
+                                    const clmsoil_surfhyd_params &p, int nlevsno, double dtime,
+                                    hipStream_t stream);
+
+}  // namespace clmsoil

@@ -6,7 +6,7 @@
  * the volumetric ice content (excess ice included), the effective porosity and
  * the ice fraction that the Richards step's impedance uses.
  *
- * clmsoil_water_table replaces water_table! (soil_hydrology.jl:849-893, kernel
+ * clmsoil_water_table replaces water_table! (soil_hydrology.jl:849-902, kernel
  * :676-826), which the reference runs after soil_water! when the lower boundary
  * uses an aquifer layer (clm_driver.jl:2066-2085): the aquifer recharge qcharge
  * of the step moves the water table through the layers' specific yield (or

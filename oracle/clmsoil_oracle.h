@@ -17,6 +17,7 @@
 #include "../include/clmsoil_zd09.h"
 #include "../include/clmsoil_ghf.h"
 #include "../include/clmsoil_hydro.h"
+#include "../include/clmsoil_surfhyd.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -95,6 +96,24 @@ int orc_water_table(const clmsoil_water_table_args *a, const clmsoil_levels *lev
 int orc_perched_water_table(const clmsoil_theta_wt_args *a, const clmsoil_levels *lev);
 int orc_theta_based_water_table(const clmsoil_theta_wt_args *a, const clmsoil_levels *lev);
 int orc_clamp_zwt_to_bedrock(const clmsoil_theta_wt_args *a, const clmsoil_levels *lev);
+
+/* Surface hydrology before the Richards step, clm_driver.jl:1939-2012
+ * (surfhyd_oracle.c). scratch2nc: 2*nc doubles for update_h2osfc!'s arrays. */
+void orc_sh_floodc(const clmsoil_surfhyd_args *a);
+void orc_sh_sat_excess(const clmsoil_surfhyd_args *a, const clmsoil_surfhyd_params *p);
+void orc_sh_qflx_inputs(const clmsoil_surfhyd_args *a);
+void orc_sh_infil_excess(const clmsoil_surfhyd_args *a, const clmsoil_surfhyd_params *p);
+void orc_sh_route(const clmsoil_surfhyd_args *a, const clmsoil_surfhyd_params *p);
+void orc_sh_update_h2osfc(const clmsoil_surfhyd_args *a, const clmsoil_surfhyd_params *p,
+                          double dtime, double *surf, double *drain);
+void orc_sh_infiltration(const clmsoil_surfhyd_args *a);
+void orc_sh_surf_runoff(const clmsoil_surfhyd_args *a, const clmsoil_surfhyd_params *p,
+                        int nlevsno, double dtime);
+void orc_sh_urban_pond(const clmsoil_surfhyd_args *a, const clmsoil_surfhyd_params *p,
+                       int nlevsno, double dtime);
+int orc_surface_hydrology(const clmsoil_surfhyd_args *a, const clmsoil_levels *lev, double dtime,
+                          const clmsoil_surfhyd_params *p, double *scratch2nc);
+void orc_set_surfhyd_jitter(uint64_t seed);
 /* Test hook: a non-zero seed moves every pow() of the Zeng-Decker step one ulp
  * (direction from a hash of the operands and the seed); 0 restores exact pow(). */
 void orc_set_zd_pow_jitter(uint64_t seed);

@@ -17,7 +17,8 @@ sys.path.insert(0, str(REPO / "clm.jl_amd"))
 
 from clmsoil import _abi  # noqa: E402
 from clmsoil.bind import (byref, fractions_args, ghf_args, heat_args, levels, mf_args,  # noqa: E402
-                          seb_args, theta_wt_args, water_table_args, zd_args)
+                          seb_args, surfhyd_args, surfhyd_params, theta_wt_args, water_table_args,
+                          zd_args)
 
 LIB_PATH = ORACLE_DIR / "liboracle_clmsoil.so"
 _lib = None
@@ -172,3 +173,22 @@ def water_table_moisture_form(d: dict) -> None:
     perched_water_table(d)
     theta_based_water_table(d)
     clamp_zwt_to_bedrock(d)
+
+
+def surface_hydrology(d: dict, dtime: float, **params) -> None:
+    """The surface hydrology steps of clm_driver.jl:1939-2012 on the host arrays
+    of `d`, each step over all columns in turn; `params` as bind.surfhyd_params."""
+    import numpy as np
+    L = lib()
+    scratch = np.zeros(2 * max(int(d["nc"]), 1))
+    check(L.orc_surface_hydrology(byref(surfhyd_args(d)), byref(levels(d)), float(dtime),
+                                  byref(surfhyd_params(**params)),
+                                  scratch.ctypes.data_as(C.c_void_p)), "surface_hydrology")
+
+
+def set_surfhyd_jitter(seed: int) -> None:
+    """Move every exp/pow/sin of the surface hydrology one ulp (0 = exact)."""
+    L = lib()
+    L.orc_set_surfhyd_jitter.argtypes = [C.c_uint64]
+    L.orc_set_surfhyd_jitter.restype = None
+    L.orc_set_surfhyd_jitter(int(seed))
