@@ -334,6 +334,37 @@ def surface_hydrology_step(nc, steps):
                          "bytes_per_column": SURFHYD_READ_BYTES + SURFHYD_WRITE_BYTES}}
 
 
+def root_sink_step(nc, steps):
+    """Root water uptake (default method) on `nc` columns of 0-12 patches each
+    (CLM's patch layout); `steps` launches, each bracketed by HIP events."""
+    import torch
+    from clmsoil import api, inputs
+    from clmsoil._abi import RS_FIELDS
+    d = inputs.root_sink_columns(nc=nc, seed=54)
+    t = api.to_device(d)
+    st = {k: t[k] for k in RS_FIELDS} | {"nc": nc, "np": d["np"]}
+    for _ in range(2):
+        api.compute_effec_rootfrac_and_vert_tran_sink(st)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
+    for e in ev:
+        e[0].record()
+        api.compute_effec_rootfrac_and_vert_tran_sink(st)
+        e[1].record()
+    torch.cuda.synchronize()
+    assert api.device_errors() == 0
+    ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    n = d["nlevsoi"]
+    # per column: mask, patchi, npatches, column transpiration in; rootr_col and
+    # qflx_rootsoi_col out over the soil layers. Per patch: active, wtcol,
+    # transpiration and the root profile over the soil layers in.
+    by = nc * (1 + 3 * 8 + 2 * 8 * n) + d["np"] * (1 + 2 * 8 + 8 * n)
+    return {"workload": f"{nc} columns, {d['np']} patches (0-12 per column)",
+            "ms_per_launch": ms, "columns_per_s": nc / (ms / 1e3),
+            "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                         "bytes_per_launch": by}}
+
+
 def water_tables_moisture_form(nc, steps):
     """The water tables after a step without an aquifer layer (perched,
     theta-based, bedrock clamp in one launch) on `nc` columns of 20 soil layers
@@ -388,7 +419,7 @@ def main(argv=None):
                     help="run only the Zeng-Decker 2009 section (for profiling its kernel)")
     ap.add_argument("--hydro-only", action="store_true",
                     help="run only the soil hydrology sections (aquifer step, water tables "
-                         "without an aquifer, surface hydrology)")
+                         "without an aquifer, surface hydrology, root uptake)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -424,7 +455,8 @@ def main(argv=None):
         print(json.dumps({"zengdecker_2009": zengdecker_2009(
             nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample),
             "water_tables_moisture_form": water_tables_moisture_form(nc, args.steps),
-            "surface_hydrology": surface_hydrology_step(nc, args.steps)}), flush=True)
+            "surface_hydrology": surface_hydrology_step(nc, args.steps),
+            "root_sink": root_sink_step(nc, args.steps)}), flush=True)
         return 0
     host, dev, gen_s = make_state(nc, args.seed)
     print(f"# state ready: {nc} columns in {gen_s:.1f} s", file=sys.stderr, flush=True)
@@ -514,6 +546,7 @@ def main(argv=None):
             nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample)
         out["water_tables_moisture_form"] = water_tables_moisture_form(nc, args.steps)
         out["surface_hydrology"] = surface_hydrology_step(nc, args.steps)
+        out["root_sink"] = root_sink_step(nc, args.steps)
     if not args.no_cpu_baseline:
         v, cores, secs = cpu_baseline(host, min(args.cpu_sample, nc), args.cpu_steps)
         out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": cores,

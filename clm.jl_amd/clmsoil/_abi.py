@@ -135,6 +135,11 @@ SH_STEPS = {"floodc": 0x001, "sat_excess": 0x002, "qflx_inputs": 0x004, "infil_e
             "urban_pond": 0x100}
 SH_ALL = 0x1ff
 
+RS_INT_FIELDS = ["patchi", "npatches"]
+RS_U8_FIELDS = ["mask_hydrologyc", "active"]
+RS_FIELDS = ["mask_hydrologyc", "patchi", "npatches", "active", "wtcol", "rootr_patch",
+             "qflx_tran_veg_patch", "qflx_tran_veg_col", "rootr_col", "qflx_rootsoi_col"]
+
 
 class FractionsArgs(C.Structure):
     """clmsoil_fractions_args (include/clmsoil_hydro.h)."""
@@ -163,6 +168,12 @@ class SurfHydParams(C.Structure):
                 ("crop_fsat_equals_zero", C.c_int32), ("hillslope_fsat_equals_zero", C.c_int32),
                 ("reserved", C.c_int32), ("fff", C.c_double), ("e_ice", C.c_double),
                 ("pc", C.c_double), ("mu", C.c_double), ("pondmx_urban", C.c_double)]
+
+
+class RootSinkArgs(C.Structure):
+    """clmsoil_rootsink_args (include/clmsoil_surfhyd.h)."""
+    _fields_ = [("nc", C.c_int64), ("ldc", C.c_int64), ("np", C.c_int64), ("ldp", C.c_int64)] + \
+        [(f, _p) for f in RS_FIELDS]
 
 
 class HeatConfig(C.Structure):
@@ -201,6 +212,9 @@ def declare_common(lib: C.CDLL, prefix: str) -> C.CDLL:
     f = getattr(lib, prefix + "surface_hydrology")
     # last argument: the stream (product) or 2*nc doubles of scratch (oracle)
     f.argtypes = [C.POINTER(SurfHydArgs), C.POINTER(Levels), dbl, C.POINTER(SurfHydParams), P]
+    f.restype = C.c_int
+    f = getattr(lib, prefix + "compute_effec_rootfrac_and_vert_tran_sink")
+    f.argtypes = [C.POINTER(RootSinkArgs), C.POINTER(Levels)] + ([P] if prefix == "clmsoil_" else [])
     f.restype = C.c_int
     f = getattr(lib, prefix + "soil_energy_balance")
     f.argtypes = [C.POINTER(SebArgs), C.POINTER(Levels), dbl] + ([P] if prefix == "clmsoil_" else [])

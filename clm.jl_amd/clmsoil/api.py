@@ -24,7 +24,7 @@ from dataclasses import dataclass, field
 
 from . import _abi
 from .bind import (byref, fractions_args, ghf_args, heat_args, heat_config, levels, mf_args,
-                   seb_args, surfhyd_args, surfhyd_params, swm_config, theta_wt_args,
+                   rootsink_args, seb_args, surfhyd_args, surfhyd_params, swm_config, theta_wt_args,
                    water_table_args, zd_args)
 
 ZENGDECKER_2009, MOISTURE_FORM = _abi.ZENGDECKER_2009, _abi.MOISTURE_FORM
@@ -273,6 +273,23 @@ def surface_hydrology(state: dict, dtime: float, nlevsno: int = 12, nlevsoi: int
     _abi.check(L, L.clmsoil_surface_hydrology(byref(surfhyd_args(d)), byref(levels(d)),
                                               float(dtime), byref(surfhyd_params(**params)),
                                               _stream()), "surface_hydrology")
+
+
+def compute_effec_rootfrac_and_vert_tran_sink(state: dict, nlevsno: int = 12, nlevsoi: int = 20,
+                                              nlevgrnd: int = 25) -> None:
+    """Root water uptake with use_hydrstress = false
+    (compute_effec_rootfrac_and_vert_tran_sink!, soil_water_plant_sink.jl:341-451):
+    the transpiration-weighted column root fraction and the Richards step's
+    root sink qflx_rootsoi_col. `state` maps the clmsoil_rootsink_args fields
+    (include/clmsoil_surfhyd.h) to device tensors, plus `nc` and `np`."""
+    d = dict(state)
+    d.update(nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd)
+    if int(d["nc"]) == 0:
+        return
+    L = _lib()
+    _abi.check(L, L.clmsoil_compute_effec_rootfrac_and_vert_tran_sink(
+        byref(rootsink_args(d)), byref(levels(d)), _stream()),
+        "compute_effec_rootfrac_and_vert_tran_sink")
 
 
 def _theta_wt(entry: str, state: dict, nlevsno: int, nlevsoi: int, nlevgrnd: int) -> None:

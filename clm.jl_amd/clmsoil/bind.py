@@ -11,13 +11,15 @@ import ctypes as C
 import numpy as np
 
 from ._abi import (FRAC_FIELDS, GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_FIELDS, MF_FIELDS,
-                   SEB_FIELDS, SH_ALL, SH_FIELDS, SH_INT_FIELDS, SH_U8_FIELDS, THETA_WT_FIELDS,
+                   RS_FIELDS, RS_INT_FIELDS, RS_U8_FIELDS, SEB_FIELDS, SH_ALL, SH_FIELDS,
+                   SH_INT_FIELDS, SH_U8_FIELDS, THETA_WT_FIELDS, RootSinkArgs,
                    WT_FIELDS, ZD_FIELDS, FractionsArgs, GhfArgs, HeatArgs, HeatConfig, Levels,
                    MfArgs, SebArgs, SurfHydArgs, SurfHydParams, SwmConfig, ThetaWtArgs,
                    WaterTableArgs, ZdArgs)
 
-_INT_FIELDS = {"nbedrock", "imelt", "frac_veg_nosno"} | set(GHF_INT_FIELDS) | set(SH_INT_FIELDS)
-_U8_FIELDS = {"mask", "mask_urbanp"} | set(GHF_U8_FIELDS) | set(SH_U8_FIELDS)
+_INT_FIELDS = ({"nbedrock", "imelt", "frac_veg_nosno"} | set(GHF_INT_FIELDS) | set(SH_INT_FIELDS)
+               | set(RS_INT_FIELDS))
+_U8_FIELDS = {"mask", "mask_urbanp"} | set(GHF_U8_FIELDS) | set(SH_U8_FIELDS) | set(RS_U8_FIELDS)
 
 
 def _ptr(x):
@@ -137,6 +139,19 @@ def water_table_args(d: dict) -> WaterTableArgs:
 
 def theta_wt_args(d: dict) -> ThetaWtArgs:
     return _simple_args(ThetaWtArgs, THETA_WT_FIELDS, d)
+
+
+def rootsink_args(d: dict) -> RootSinkArgs:
+    """clmsoil_rootsink_args from a dict of arrays (`nc`, `np` integers)."""
+    a = RootSinkArgs()
+    a.nc, a.np = int(d["nc"]), int(d["np"])
+    a.ldc = int(d.get("ldc", a.nc))
+    a.ldp = int(d.get("ldp", a.np))
+    for f in RS_FIELDS:
+        x = d.get(f)
+        _check_dtype(f, x)
+        setattr(a, f, _ptr(x))
+    return a
 
 
 def surfhyd_args(d: dict) -> SurfHydArgs:

@@ -690,3 +690,39 @@ def surface_hydrology_columns(nc=4096, seed=60, nlevsno=NLEVSNO, nlevgrnd=NLEVGR
               "qflx_h2osfc_surf", "qflx_h2osfc_drain", "qflx_infl", "qflx_surf"):
         d[k] = sentinel()
     return d
+
+
+def root_sink_columns(nc=4096, seed=80, nlevsoi=NLEVSOI, nlevgrnd=NLEVGRND, max_patches=12,
+                      scatter=False):
+    """Columns and patches for the root water uptake: 0..max_patches patches per
+    column (CLM layout: a column's patches contiguous, columns in order; with
+    `scatter` the columns' patch blocks are shuffled over the patch arrays),
+    90 % of the patches active, weights summing to 1 over each column, root
+    profiles rootr ~ exp(-z/d) with d ~ U(0.2, 1.5) m normalised over the soil
+    layers, transpiration ~ U(0, 5e-4) mm/s with a fifth of the patches at 0,
+    and the column transpiration as the weighted sum over active patches. 85 %
+    of the columns are in the hydrology filter."""
+    rng = np.random.default_rng(seed)
+    npc = rng.integers(0, max_patches + 1, nc).astype(np.int64)
+    np_ = int(npc.sum())
+    order = rng.permutation(nc) if scatter else np.arange(nc)
+    start = np.zeros(nc, dtype=np.int64)
+    start[order] = np.concatenate([[0], np.cumsum(npc[order])[:-1]])
+    patchi = start + 1
+    owner = np.repeat(order, npc[order]).astype(np.int64)      # column of each patch
+    active = (rng.uniform(size=np_) < 0.9).astype(np.uint8)
+    w = rng.uniform(0.05, 1.0, np_)
+    tot = np.bincount(owner, weights=w, minlength=nc)
+    wtcol = w / tot[owner]
+    _, zs, _ = clm5_grid(nlevsoi, nlevgrnd)
+    depth = rng.uniform(0.2, 1.5, np_)
+    prof = np.exp(-zs[:nlevsoi, None] / depth[None, :])
+    rootr = np.zeros((nlevgrnd, np_))
+    rootr[:nlevsoi] = prof / prof.sum(0)
+    qt = np.where(rng.uniform(size=np_) < 0.2, 0.0, rng.uniform(0.0, 5e-4, np_))
+    qcol = np.bincount(owner, weights=qt * wtcol * active, minlength=nc)
+    return dict(nc=nc, np=np_, nlevsno=NLEVSNO, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd,
+                mask_hydrologyc=(rng.uniform(size=nc) < 0.85).astype(np.uint8), patchi=patchi,
+                npatches=npc, active=active, wtcol=wtcol, rootr_patch=rootr,
+                qflx_tran_veg_patch=qt, qflx_tran_veg_col=qcol,
+                rootr_col=np.full((nlevgrnd, nc), 7.0), qflx_rootsoi_col=np.full((nlevsoi, nc), 7.0))

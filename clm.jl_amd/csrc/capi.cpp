@@ -562,6 +562,29 @@ int clmsoil_surface_hydrology(const clmsoil_surfhyd_args *a, const clmsoil_level
                : CLMSOIL_ERR_HIP;
 }
 
+// compute_effec_rootfrac_and_vert_tran_sink! with use_hydrstress = false
+// (soil_water_plant_sink.jl:341-451).
+int clmsoil_compute_effec_rootfrac_and_vert_tran_sink(const clmsoil_rootsink_args *a,
+                                                      const clmsoil_levels *lev, void *stream) {
+    if (!a) return CLMSOIL_ERR_NULL;
+    int st = validate_levels(lev);
+    if (st) return st;
+    if (a->nc < 0 || a->np < 0 || a->ldc < a->nc || a->ldp < a->np) return CLMSOIL_ERR_SIZE;
+    if (a->nc == 0) return CLMSOIL_OK;
+    const void *req[] = {a->patchi, a->npatches, a->qflx_tran_veg_col, a->rootr_col,
+                         a->qflx_rootsoi_col};
+    for (const void *q : req)
+        if (!q) return CLMSOIL_ERR_NULL;
+    if (a->np > 0 && (!a->active || !a->wtcol || !a->rootr_patch || !a->qflx_tran_veg_patch))
+        return CLMSOIL_ERR_NULL;
+    DeviceState *s = nullptr;
+    st = current_device_state(&s);
+    if (st) return st;
+    return launch_root_sink(*a, lev->nlevsoi, s->err, (hipStream_t)stream) == hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
+}
+
 // water_table! (soil_hydrology.jl:849-902).
 int clmsoil_water_table(const clmsoil_water_table_args *a, const clmsoil_levels *lev,
                         double dtime, double aq_sp_yield_min, int recompute_frost_table,

@@ -174,7 +174,140 @@ __global__ void __launch_bounds__(256) surfhyd_kernel(clmsoil_surfhyd_args a,
     }
 }
 
+// Root water uptake, default method (soil_water_plant_sink.jl:48-81, 341-451).
+//
+// A lane owns a column and sums its patches in ascending order with the
+// reference's operation order, so the result is bit-identical to the
+// reference's loop. The patch transpiration, weight and activity of up to
+// RS_KMAX patches sit in registers for all layers. CLM stores a column's
+// patches contiguously and neighbouring columns' ranges next to each other, so
+// a wavefront's 64 columns cover one short window of the patch arrays: when it
+// holds at most RS_WCAP patches, each layer's row of rootr_patch over that
+// window is read with coalesced loads into the wavefront's LDS slice (no
+// workgroup barrier, the wavefront only waits for itself) and the next
+// layer's row is already in flight while the lanes sum the current one. A wider
+// window (patch ranges scattered over the arrays) or a column with more than
+// RS_KMAX patches reads the same values straight from global memory.
+constexpr int RS_BLOCK = 256;
+constexpr int RS_KMAX = 16;
+constexpr int RS_WCAP = 1024;                 // patches a wavefront stages per layer
+
+__device__ __forceinline__ int64_t wave_min(int64_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const int64_t w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ int64_t wave_max(int64_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const int64_t w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(RS_BLOCK) rootsink_kernel(clmsoil_rootsink_args a, int nlevsoi,
+                                                            unsigned *err) {
+    __shared__ double s_row[RS_BLOCK / 64][RS_WCAP];
+    const int lane = threadIdx.x & 63;
+    double *row = s_row[threadIdx.x >> 6];
+    const int64_t c = (int64_t)blockIdx.x * RS_BLOCK + threadIdx.x;
+    bool run = c < a.nc && on(a.mask_hydrologyc, c);
+    int64_t lo = 0, n = 0;
+    if (run) {
+        lo = a.patchi[c] - 1;
+        n = a.npatches[c];
+        if (n < 0 || lo < 0 || lo > a.np - n) {
+            atomicOr(err, (unsigned)DEV_ERR_SUBGRID);
+            run = false;
+        }
+    }
+    const bool has = run && n > 0;
+    const int64_t p0 = wave_min(has ? lo : INT64_MAX);
+    const int64_t p1 = wave_max(has ? lo + n : INT64_MIN);
+    const int64_t w = p0 == INT64_MAX ? 0 : p1 - p0;
+    const bool staged = w <= RS_WCAP;            // the same in every lane of the wavefront
+    const bool small = n <= RS_KMAX;
+    double qt[RS_KMAX], wt[RS_KMAX];
+    unsigned actm = 0;
+    double temp = 0.0;
+    if (run) {
+        if (small) {
+#pragma unroll
+            for (int k = 0; k < RS_KMAX; ++k) {
+                qt[k] = 0.0;
+                wt[k] = 0.0;
+                if (k < n) {
+                    qt[k] = a.qflx_tran_veg_patch[lo + k];
+                    wt[k] = a.wtcol[lo + k];
+                    if (a.active[lo + k]) actm |= 1u << k;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < RS_KMAX; ++k)
+                if ((actm >> k) & 1u) temp += qt[k] * wt[k];
+        } else {
+            for (int64_t k = 0; k < n; ++k)
+                if (a.active[lo + k]) temp += a.qflx_tran_veg_patch[lo + k] * a.wtcol[lo + k];
+        }
+    }
+    const double qcol = run ? a.qflx_tran_veg_col[c] : 0.0;
+    const int64_t o = lo - p0;
+    // staged rows: layer j+1 is loaded into registers while layer j is summed
+    constexpr int NPRE = RS_WCAP / 64;
+    double pre[NPRE];
+    if (staged) {
+#pragma unroll
+        for (int t = 0; t < NPRE; ++t)
+            if (lane + 64 * t < w) pre[t] = a.rootr_patch[p0 + lane + 64 * t];
+    }
+    for (int j = 0; j < nlevsoi; ++j) {
+        const double *root = a.rootr_patch + (int64_t)j * a.ldp;
+        if (staged) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int t = 0; t < NPRE; ++t)
+                if (lane + 64 * t < w) row[lane + 64 * t] = pre[t];
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (j + 1 < nlevsoi) {
+                const double *next = root + a.ldp;
+#pragma unroll
+                for (int t = 0; t < NPRE; ++t)
+                    if (lane + 64 * t < w) pre[t] = next[p0 + lane + 64 * t];
+            }
+        }
+        if (!run) continue;
+        double acc = 0.0;
+        if (small && staged) {
+#pragma unroll
+            for (int k = 0; k < RS_KMAX; ++k)
+                if ((actm >> k) & 1u) acc += row[o + k] * qt[k] * wt[k];
+        } else if (small) {
+#pragma unroll
+            for (int k = 0; k < RS_KMAX; ++k)
+                if ((actm >> k) & 1u) acc += root[lo + k] * qt[k] * wt[k];
+        } else {
+            for (int64_t k = 0; k < n; ++k)
+                if (a.active[lo + k])
+                    acc += root[lo + k] * a.qflx_tran_veg_patch[lo + k] * a.wtcol[lo + k];
+        }
+        if (temp != 0.0) acc /= temp;
+        a.rootr_col[c + (int64_t)j * a.ldc] = acc;
+        a.qflx_rootsoi_col[c + (int64_t)j * a.ldc] = acc * qcol;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_root_sink(const clmsoil_rootsink_args &a, int nlevsoi, unsigned *err,
+                            hipStream_t stream) {
+    const dim3 grid((unsigned)((a.nc + RS_BLOCK - 1) / RS_BLOCK));
+    hipLaunchKernelGGL(rootsink_kernel, grid, dim3(RS_BLOCK), 0, stream, a, nlevsoi, err);
+    return hipGetLastError();
+}
 
 hipError_t launch_surface_hydrology(const clmsoil_surfhyd_args &a,
                                     const clmsoil_surfhyd_params &p, int nlevsno, double dtime,

@@ -106,6 +106,37 @@ typedef struct clmsoil_surfhyd_args {
 int clmsoil_surface_hydrology(const clmsoil_surfhyd_args *a, const clmsoil_levels *lev,
                               double dtime, const clmsoil_surfhyd_params *p, void *stream);
 
+/* Root water uptake: compute_effec_rootfrac_and_vert_tran_sink!
+ * (soil_water_plant_sink.jl:341-451) with use_hydrstress = false, which runs
+ * compute_effec_rootfrac_and_vert_tran_sink_default! (:115-141, kernel :48-81)
+ * on the pervious-road, other and natural-vegetation columns of the hydrology
+ * filter; the three groups partition that filter, so one pass over it is the
+ * same. Per hydrology column c with patches patchi[c] .. patchi[c]+npatches[c]-1:
+ *   temp      = sum over active patches of qflx_tran_veg_patch * wtcol
+ *   rootr_col = sum over active patches of rootr_patch * qflx_tran_veg_patch * wtcol,
+ *               divided by temp when temp != 0
+ *   qflx_rootsoi_col = rootr_col * qflx_tran_veg_col
+ * for soil layers 1..nlevsoi, summed in ascending patch order. This produces the
+ * Richards step's root sink (clm_driver.jl:2016-2022). A column whose patch range
+ * leaves 1..np is left unchanged and sets CLMSOIL_DEVERR_SUBGRID. */
+typedef struct clmsoil_rootsink_args {
+    int64_t nc, ldc;                  /* columns; leading dimension of the column arrays */
+    int64_t np, ldp;                  /* patches; leading dimension of rootr_patch     */
+    const uint8_t *mask_hydrologyc;   /* [nc] NULL = all columns                       */
+    const int64_t *patchi;            /* [nc] first patch (1-based)                    */
+    const int64_t *npatches;          /* [nc] number of patches                        */
+    const uint8_t *active;            /* [np] patch active                             */
+    const double *wtcol;              /* [np] patch weight in its column               */
+    const double *rootr_patch;        /* [ldp, nlevgrnd] effective root fraction       */
+    const double *qflx_tran_veg_patch;/* [np] transpiration (mm/s)                     */
+    const double *qflx_tran_veg_col;  /* [nc] column transpiration (mm/s)              */
+    double *rootr_col;                /* [ldc, nlevgrnd] (layers 1..nlevsoi written)   */
+    double *qflx_rootsoi_col;         /* [ldc, nlevsoi] (mm/s)                         */
+} clmsoil_rootsink_args;
+
+int clmsoil_compute_effec_rootfrac_and_vert_tran_sink(const clmsoil_rootsink_args *a,
+                                                      const clmsoil_levels *lev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

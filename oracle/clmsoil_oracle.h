@@ -114,6 +114,8 @@ void orc_sh_urban_pond(const clmsoil_surfhyd_args *a, const clmsoil_surfhyd_para
 int orc_surface_hydrology(const clmsoil_surfhyd_args *a, const clmsoil_levels *lev, double dtime,
                           const clmsoil_surfhyd_params *p, double *scratch2nc);
 void orc_set_surfhyd_jitter(uint64_t seed);
+int orc_compute_effec_rootfrac_and_vert_tran_sink(const clmsoil_rootsink_args *a,
+                                                  const clmsoil_levels *lev);
 /* Test hook: a non-zero seed moves every pow() of the Zeng-Decker step one ulp
  * (direction from a hash of the operands and the seed); 0 restores exact pow(). */
 void orc_set_zd_pow_jitter(uint64_t seed);

@@ -17,8 +17,8 @@ sys.path.insert(0, str(REPO / "clm.jl_amd"))
 
 from clmsoil import _abi  # noqa: E402
 from clmsoil.bind import (byref, fractions_args, ghf_args, heat_args, levels, mf_args,  # noqa: E402
-                          seb_args, surfhyd_args, surfhyd_params, theta_wt_args, water_table_args,
-                          zd_args)
+                          rootsink_args, seb_args, surfhyd_args, surfhyd_params, theta_wt_args,
+                          water_table_args, zd_args)
 
 LIB_PATH = ORACLE_DIR / "liboracle_clmsoil.so"
 _lib = None
@@ -192,3 +192,9 @@ def set_surfhyd_jitter(seed: int) -> None:
     L.orc_set_surfhyd_jitter.argtypes = [C.c_uint64]
     L.orc_set_surfhyd_jitter.restype = None
     L.orc_set_surfhyd_jitter(int(seed))
+
+
+def compute_effec_rootfrac_and_vert_tran_sink(d: dict) -> None:
+    L = lib()
+    check(L.orc_compute_effec_rootfrac_and_vert_tran_sink(byref(rootsink_args(d)), byref(levels(d))),
+          "compute_effec_rootfrac_and_vert_tran_sink")

@@ -287,3 +287,35 @@ int orc_surface_hydrology(const clmsoil_surfhyd_args *a, const clmsoil_levels *l
     if (s & CLMSOIL_SH_URBAN_POND) orc_sh_urban_pond(a, p, lev->nlevsno, dtime);
     return CLMSOIL_OK;
 }
+
+/* _plantsink_default_kernel! (soil_water_plant_sink.jl:48-81) over the hydrology
+ * columns, as compute_effec_rootfrac_and_vert_tran_sink! (:341-451) runs it with
+ * use_hydrstress = false. A patch range outside 1..np is an error here. */
+int orc_compute_effec_rootfrac_and_vert_tran_sink(const clmsoil_rootsink_args *a,
+                                                  const clmsoil_levels *lev) {
+    if (!a || !lev) return CLMSOIL_ERR_NULL;
+    for (int64_t c = 0; c < a->nc; ++c) {
+        if (!on(a->mask_hydrologyc, c)) continue;
+        if (a->npatches[c] < 0 || a->patchi[c] < 1 || a->patchi[c] - 1 + a->npatches[c] > a->np)
+            return CLMSOIL_ERR_SIZE;
+    }
+    for (int64_t c = 0; c < a->nc; ++c) {
+        if (!on(a->mask_hydrologyc, c)) continue;
+        const int64_t p_lo = a->patchi[c] - 1, p_hi = a->patchi[c] - 1 + a->npatches[c] - 1;
+        double temp_c = 0.0;
+        for (int64_t p = p_lo; p <= p_hi; ++p)
+            if (a->active[p]) temp_c += a->qflx_tran_veg_patch[p] * a->wtcol[p];
+        for (int j = 1; j <= lev->nlevsoi; ++j) {
+            double acc = 0.0;
+            for (int64_t p = p_lo; p <= p_hi; ++p)
+                if (a->active[p])
+                    acc += a->rootr_patch[(size_t)p + (size_t)(j - 1) * (size_t)a->ldp] *
+                           a->qflx_tran_veg_patch[p] * a->wtcol[p];
+            if (temp_c != 0.0) acc /= temp_c;
+            const size_t o = (size_t)c + (size_t)(j - 1) * (size_t)a->ldc;
+            a->rootr_col[o] = acc;
+            a->qflx_rootsoi_col[o] = acc * a->qflx_tran_veg_col[c];
+        }
+    }
+    return CLMSOIL_OK;
+}

@@ -24,6 +24,9 @@ STRUCTS = {
     "clmsoil_fractions_args": _abi.FractionsArgs,
     "clmsoil_water_table_args": _abi.WaterTableArgs,
     "clmsoil_theta_wt_args": _abi.ThetaWtArgs,
+    "clmsoil_surfhyd_args": _abi.SurfHydArgs,
+    "clmsoil_surfhyd_params": _abi.SurfHydParams,
+    "clmsoil_rootsink_args": _abi.RootSinkArgs,
 }
 
 
@@ -34,7 +37,7 @@ def c_layout(tmp_path_factory):
         pytest.skip("no C compiler")
     lines = ["#include <stdio.h>", "#include <stddef.h>",
              '#include "clmsoil.h"', '#include "clmsoil_zd09.h"', '#include "clmsoil_ghf.h"',
-             '#include "clmsoil_hydro.h"',
+             '#include "clmsoil_hydro.h"', '#include "clmsoil_surfhyd.h"',
              "int main(void) {"]
     for cname, py in STRUCTS.items():
         lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')

@@ -7,7 +7,8 @@ every one of those calls is moved one ulp (8 jitter seeds), and at least 99.9 %
 of the written elements must meet plain 1e-12 relative. Everything else in the
 sequence (sums, products, min/max, the truncation rule) carries no such freedom:
 with the transcendentals removed (steps that use none) the results are
-bit-identical."""
+bit-identical. The root water uptake has no transcendental and is bit-exact.
+"""
 import numpy as np
 import pytest
 
@@ -118,3 +119,50 @@ def test_surface_hydrology_then_richards_infiltration(gpu):
     infl = api.to_host(t)["qflx_infl"]
     dliq = (h["h2osoi_liq"][12:32] - r["h2osoi_liq"][12:32]).sum(0)
     np.testing.assert_allclose(dliq, infl * 1800.0, rtol=1e-10, atol=1e-9)
+
+
+RS = ("mask_hydrologyc", "patchi", "npatches", "active", "wtcol", "rootr_patch",
+      "qflx_tran_veg_patch", "qflx_tran_veg_col", "rootr_col", "qflx_rootsoi_col")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed,scatter,max_patches", [(84, False, 12), (85, True, 12),
+                                                      (86, False, 40)])
+def test_root_sink_vs_oracle(gpu, seed, scatter, max_patches):
+    """Bit-exact: the same products and sums in ascending patch order. Ordered
+    columns take the LDS-staged path; shuffled patch blocks and 40-patch
+    columns (windows wider than the LDS stage) take the direct one."""
+    import torch
+    d = inputs.root_sink_columns(nc=3001, seed=seed, scatter=scatter, max_patches=max_patches)
+    t = api.to_device(d)
+    api.device_errors()
+    api.compute_effec_rootfrac_and_vert_tran_sink({k: t[k] for k in RS} | {"nc": d["nc"],
+                                                                          "np": d["np"]})
+    torch.cuda.synchronize()
+    assert api.device_errors() == 0
+    got = api.to_host(t)
+    ref = clone(d)
+    oracle.compute_effec_rootfrac_and_vert_tran_sink(ref)
+    assert np.array_equal(got["rootr_col"], ref["rootr_col"])
+    assert np.array_equal(got["qflx_rootsoi_col"], ref["qflx_rootsoi_col"])
+
+
+@pytest.mark.gpu
+def test_root_sink_bad_range_flags(gpu):
+    import torch
+    d = inputs.root_sink_columns(nc=500, seed=87)
+    d["mask_hydrologyc"][:] = 1
+    d["npatches"][9] = d["np"] + 3
+    t = api.to_device(d)
+    api.device_errors()
+    api.compute_effec_rootfrac_and_vert_tran_sink({k: t[k] for k in RS} | {"nc": d["nc"],
+                                                                          "np": d["np"]})
+    torch.cuda.synchronize()
+    assert api.device_errors() & 16                      # CLMSOIL_DEVERR_SUBGRID
+    got = api.to_host(t)
+    assert np.all(got["rootr_col"][:, 9] == 7.0)
+    ref = clone(d)
+    ref["mask_hydrologyc"][9] = 0
+    oracle.compute_effec_rootfrac_and_vert_tran_sink(ref)
+    ok = np.arange(d["nc"]) != 9
+    assert np.array_equal(got["rootr_col"][:, ok], ref["rootr_col"][:, ok])

@@ -7,11 +7,15 @@
   (test_infilt_excess_runoff.jl:72-150, 150-240: qinmax from hksat and ice, the
   HKSAT and NONE methods, masking), set_floodc! and infiltration!
   (test_soil_hydrology_mod.jl:117-150), the relative truncation of the surface
-  water store (:152-172) and urban ponding with its water balance (:495-590);
+  water store (:152-172), urban ponding with its water balance (:495-590) and
+  the default root water uptake (test_soil_water_plant_sink.jl:118-276, 479-530);
 * the water balance of the whole sequence on a soil or crop column, which the
   mathematics fixes: infiltration + surface runoff + d(h2osfc)/dt equals the
   water arriving at the surface minus the evaporation drawn from soil and
-  ponded water (a dropped term, a wrong sign or a wrong fraction fails it).
+  ponded water (a dropped term, a wrong sign or a wrong fraction fails it);
+* the root uptake distributes the column transpiration: with every patch's
+  root profile summing to 1, the column profile sums to 1 and the root sink to
+  the column transpiration.
 """
 import numpy as np
 import pytest
@@ -233,3 +237,79 @@ def test_unknown_method_rejected():
         oracle.surface_hydrology(d, 1800.0, fsat_method=3)
     with pytest.raises(RuntimeError):
         oracle.surface_hydrology(d, 1800.0, steps=0x200)
+
+
+# ---- root water uptake (test_soil_water_plant_sink.jl) ----
+
+def sink(rootr, tran_patch, tran_col, wtcol, active, npatches, nlevsoi=4, mask=None):
+    """Columns in order, their patches contiguous (the reference test's layout)."""
+    rootr = np.asarray(rootr, dtype=float)
+    np_, nc = rootr.shape[0], len(npatches)
+    rp = np.zeros((10, np_))
+    rp[:nlevsoi] = rootr.T
+    return dict(nc=nc, np=np_, nlevsno=12, nlevsoi=nlevsoi, nlevgrnd=10,
+                mask_hydrologyc=np.ones(nc, np.uint8) if mask is None else np.asarray(mask, np.uint8),
+                patchi=(np.concatenate([[0], np.cumsum(npatches)[:-1]]) + 1).astype(np.int64),
+                npatches=np.asarray(npatches, np.int64), active=np.asarray(active, np.uint8),
+                wtcol=np.asarray(wtcol, float), rootr_patch=rp,
+                qflx_tran_veg_patch=np.asarray(tran_patch, float),
+                qflx_tran_veg_col=np.asarray(tran_col, float),
+                rootr_col=np.full((10, nc), np.nan), qflx_rootsoi_col=np.full((nlevsoi, nc), np.nan))
+
+
+def test_root_sink_reference_kats():
+    r = [[0.4, 0.3, 0.2, 0.1]]
+    d = sink(r, [0.002], [0.002], [1.0], [1], [1])
+    oracle.compute_effec_rootfrac_and_vert_tran_sink(d)
+    np.testing.assert_allclose(d["rootr_col"][:4, 0], r[0], atol=1e-12)
+    np.testing.assert_allclose(d["qflx_rootsoi_col"][:, 0], np.array(r[0]) * 0.002, atol=1e-14)
+    r2 = [[0.1, 0.2, 0.3, 0.4], [0.5, 0.3, 0.15, 0.05]]
+    tp, wt = [0.001, 0.003], [0.4, 0.6]
+    tc = tp[0] * wt[0] + tp[1] * wt[1]
+    d = sink(r2, tp, [tc], wt, [1, 1], [2])
+    oracle.compute_effec_rootfrac_and_vert_tran_sink(d)
+    exp = [(r2[0][j] * tp[0] * wt[0] + r2[1][j] * tp[1] * wt[1]) / tc for j in range(4)]
+    np.testing.assert_allclose(d["rootr_col"][:4, 0], exp, atol=1e-12)
+    np.testing.assert_allclose(d["qflx_rootsoi_col"][:, 0], np.array(exp) * tc, atol=1e-14)
+    d = sink([[0.25] * 4], [0.0], [0.0], [1.0], [1], [1])          # zero transpiration
+    oracle.compute_effec_rootfrac_and_vert_tran_sink(d)
+    assert np.all(d["rootr_col"][:4, 0] == 0.0) and np.all(d["qflx_rootsoi_col"][:, 0] == 0.0)
+    d = sink([[0.4, 0.3, 0.2, 0.1], [0.1, 0.1, 0.1, 0.7]], [0.002, 0.005], [0.001], [0.5, 0.5],
+             [1, 0], [2])                                          # inactive patch skipped
+    oracle.compute_effec_rootfrac_and_vert_tran_sink(d)
+    np.testing.assert_allclose(d["rootr_col"][:4, 0], [0.4, 0.3, 0.2, 0.1], atol=1e-12)
+    r3 = [[0.25] * 4, [0.4, 0.3, 0.2, 0.1], [0.1, 0.2, 0.3, 0.4]]  # road, crop, natural veg
+    d = sink(r3, [0.001, 0.002, 0.003], [0.001, 0.002, 0.003], [1.0] * 3, [1] * 3, [1, 1, 1])
+    oracle.compute_effec_rootfrac_and_vert_tran_sink(d)
+    np.testing.assert_allclose(d["rootr_col"][:4], np.array(r3).T, atol=1e-12)
+    np.testing.assert_allclose(d["qflx_rootsoi_col"], np.array(r3).T * [0.001, 0.002, 0.003],
+                               atol=1e-14)
+
+
+def test_root_sink_distributes_the_transpiration():
+    """Each patch's root profile sums to 1 over the soil layers, so the
+    transpiration-weighted column profile does too, and the root sink sums to
+    the column transpiration; masked-off columns stay untouched."""
+    for scatter in (False, True):
+        d = inputs.root_sink_columns(nc=4000, seed=82, scatter=scatter)
+        oracle.compute_effec_rootfrac_and_vert_tran_sink(d)
+        m = d["mask_hydrologyc"] == 1
+        live = np.zeros(d["nc"], bool)
+        for c in np.flatnonzero(m):
+            p = slice(d["patchi"][c] - 1, d["patchi"][c] - 1 + d["npatches"][c])
+            live[c] = np.sum(d["qflx_tran_veg_patch"][p] * d["wtcol"][p] * d["active"][p]) != 0
+        n = d["nlevsoi"]
+        np.testing.assert_allclose(d["rootr_col"][:n, live].sum(0), 1.0, rtol=1e-14)
+        assert np.all(d["rootr_col"][:n, m & ~live] == 0.0)
+        np.testing.assert_allclose(d["qflx_rootsoi_col"][:, m].sum(0), d["qflx_tran_veg_col"][m],
+                                   rtol=1e-13, atol=1e-19)
+        assert np.all(d["rootr_col"][:, ~m] == 7.0) and np.all(d["rootr_col"][n:] == 7.0)
+        assert live.sum() > 2000 and (m & ~live).sum() > 50
+
+
+def test_root_sink_bad_patch_range():
+    d = inputs.root_sink_columns(nc=64, seed=83)
+    d["mask_hydrologyc"][:] = 1
+    d["npatches"][5] = d["np"] + 1
+    with pytest.raises(RuntimeError):
+        oracle.compute_effec_rootfrac_and_vert_tran_sink(d)
