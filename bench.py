@@ -365,6 +365,41 @@ def root_sink_step(nc, steps):
                          "bytes_per_launch": by}}
 
 
+def lake_solve_step(nc, steps):
+    """The lake temperature solve (assembly, tridiagonal solve and unpack in one
+    launch) on `nc` lake columns: 0-6 snow layers, 10 lake levels, 25 soil
+    levels; `steps` launches on the same inputs, each bracketed by HIP events."""
+    import numpy as np
+    import torch
+    from clmsoil import api, inputs
+    from clmsoil._abi import LAKE_FIELDS
+    d = inputs.lake_columns(nc=nc, seed=55)
+    t = api.to_device(d)
+    st = {k: t[k] for k in LAKE_FIELDS} | {"nc": nc, "nlevlak": d["nlevlak"]}
+    t0s, t0l = t["t_soisno"].clone(), t["t_lake"].clone()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps + 2)]
+    for e in ev:
+        t["t_soisno"].copy_(t0s)
+        t["t_lake"].copy_(t0l)
+        e[0].record()
+        api.lake_temperature_solve(st, 1800.0)
+        e[1].record()
+    torch.cuda.synchronize()
+    assert api.device_errors() == 0
+    ms = sum(e[0].elapsed_time(e[1]) for e in ev[2:]) / steps
+    on = d["mask_lakec"] == 1
+    ns = -d["snl"][on]
+    # per lake column: snl, fin, phi_soil, tktopsoillay; per snow level z, cv, tk,
+    # sabg_lyr_col and T in, T out; per soil level z, cv, tk and T in, T out; per
+    # lake level z, dz, cv, tk, phi and T in, T out; plus the mask of every column
+    by = nc + int(on.sum()) * (4 * 8 + 25 * 5 * 8 + 10 * 7 * 8) + int(ns.sum()) * 6 * 8
+    return {"workload": f"{nc} lake columns: 0-6 snow, 10 lake, 25 soil levels",
+            "ms_per_launch": ms, "columns_per_s": nc / (ms / 1e3),
+            "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                         "bytes_per_launch": by}}
+
+
 def water_tables_moisture_form(nc, steps):
     """The water tables after a step without an aquifer layer (perched,
     theta-based, bedrock clamp in one launch) on `nc` columns of 20 soil layers
@@ -419,7 +454,7 @@ def main(argv=None):
                     help="run only the Zeng-Decker 2009 section (for profiling its kernel)")
     ap.add_argument("--hydro-only", action="store_true",
                     help="run only the soil hydrology sections (aquifer step, water tables "
-                         "without an aquifer, surface hydrology, root uptake)")
+                         "without an aquifer, surface hydrology, root uptake, lake solve)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -456,7 +491,8 @@ def main(argv=None):
             nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample),
             "water_tables_moisture_form": water_tables_moisture_form(nc, args.steps),
             "surface_hydrology": surface_hydrology_step(nc, args.steps),
-            "root_sink": root_sink_step(nc, args.steps)}), flush=True)
+            "root_sink": root_sink_step(nc, args.steps),
+            "lake_solve": lake_solve_step(nc, args.steps)}), flush=True)
         return 0
     host, dev, gen_s = make_state(nc, args.seed)
     print(f"# state ready: {nc} columns in {gen_s:.1f} s", file=sys.stderr, flush=True)
@@ -547,6 +583,7 @@ def main(argv=None):
         out["water_tables_moisture_form"] = water_tables_moisture_form(nc, args.steps)
         out["surface_hydrology"] = surface_hydrology_step(nc, args.steps)
         out["root_sink"] = root_sink_step(nc, args.steps)
+        out["lake_solve"] = lake_solve_step(nc, args.steps)
     if not args.no_cpu_baseline:
         v, cores, secs = cpu_baseline(host, min(args.cpu_sample, nc), args.cpu_steps)
         out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": cores,

@@ -140,6 +140,10 @@ RS_U8_FIELDS = ["mask_hydrologyc", "active"]
 RS_FIELDS = ["mask_hydrologyc", "patchi", "npatches", "active", "wtcol", "rootr_patch",
              "qflx_tran_veg_patch", "qflx_tran_veg_col", "rootr_col", "qflx_rootsoi_col"]
 
+LAKE_FIELDS = ["mask_lakec", "snl", "z", "cv", "tk", "sabg_lyr_col", "z_lake", "dz_lake",
+               "cv_lake", "tk_lake", "phi", "phi_soil", "fin", "tktopsoillay", "t_soisno",
+               "t_lake"]
+
 
 class FractionsArgs(C.Structure):
     """clmsoil_fractions_args (include/clmsoil_hydro.h)."""
@@ -174,6 +178,12 @@ class RootSinkArgs(C.Structure):
     """clmsoil_rootsink_args (include/clmsoil_surfhyd.h)."""
     _fields_ = [("nc", C.c_int64), ("ldc", C.c_int64), ("np", C.c_int64), ("ldp", C.c_int64)] + \
         [(f, _p) for f in RS_FIELDS]
+
+
+class LakeSolveArgs(C.Structure):
+    """clmsoil_lake_solve_args (include/clmsoil_lake.h)."""
+    _fields_ = [("nc", C.c_int64), ("ld", C.c_int64), ("nlevlak", C.c_int64)] + \
+        [(f, _p) for f in LAKE_FIELDS]
 
 
 class HeatConfig(C.Structure):
@@ -215,6 +225,10 @@ def declare_common(lib: C.CDLL, prefix: str) -> C.CDLL:
     f.restype = C.c_int
     f = getattr(lib, prefix + "compute_effec_rootfrac_and_vert_tran_sink")
     f.argtypes = [C.POINTER(RootSinkArgs), C.POINTER(Levels)] + ([P] if prefix == "clmsoil_" else [])
+    f.restype = C.c_int
+    f = getattr(lib, prefix + "lake_temperature_solve")
+    # last argument: the stream (product) or the scratch copy-out, may be NULL (oracle)
+    f.argtypes = [C.POINTER(LakeSolveArgs), C.POINTER(Levels), dbl, P]
     f.restype = C.c_int
     f = getattr(lib, prefix + "soil_energy_balance")
     f.argtypes = [C.POINTER(SebArgs), C.POINTER(Levels), dbl] + ([P] if prefix == "clmsoil_" else [])

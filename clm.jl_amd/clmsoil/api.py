@@ -24,7 +24,7 @@ from dataclasses import dataclass, field
 
 from . import _abi
 from .bind import (byref, fractions_args, ghf_args, heat_args, heat_config, levels, mf_args,
-                   rootsink_args, seb_args, surfhyd_args, surfhyd_params, swm_config, theta_wt_args,
+                   lake_args, rootsink_args, seb_args, surfhyd_args, surfhyd_params, swm_config, theta_wt_args,
                    water_table_args, zd_args)
 
 ZENGDECKER_2009, MOISTURE_FORM = _abi.ZENGDECKER_2009, _abi.MOISTURE_FORM
@@ -273,6 +273,22 @@ def surface_hydrology(state: dict, dtime: float, nlevsno: int = 12, nlevsoi: int
     _abi.check(L, L.clmsoil_surface_hydrology(byref(surfhyd_args(d)), byref(levels(d)),
                                               float(dtime), byref(surfhyd_params(**params)),
                                               _stream()), "surface_hydrology")
+
+
+def lake_temperature_solve(state: dict, dtime: float, nlevsno: int = 12, nlevsoi: int = 20,
+                           nlevgrnd: int = 25) -> None:
+    """Steps 6-7 of lake_temperature! and the unpack (lake_temperature.jl:1205-1225):
+    assemble the Crank-Nicolson system of the snow + lake + soil column and solve
+    it in place into t_soisno and t_lake. `state` maps the clmsoil_lake_solve_args
+    fields (include/clmsoil_lake.h) to device tensors, plus `nc` and `nlevlak`."""
+    d = dict(state)
+    d.update(nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd)
+    if int(d["nc"]) == 0:
+        return
+    L = _lib()
+    _abi.check(L, L.clmsoil_lake_temperature_solve(byref(lake_args(d)), byref(levels(d)),
+                                                   float(dtime), _stream()),
+               "lake_temperature_solve")
 
 
 def compute_effec_rootfrac_and_vert_tran_sink(state: dict, nlevsno: int = 12, nlevsoi: int = 20,

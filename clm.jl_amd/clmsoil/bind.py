@@ -11,7 +11,8 @@ import ctypes as C
 import numpy as np
 
 from ._abi import (FRAC_FIELDS, GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_FIELDS, MF_FIELDS,
-                   RS_FIELDS, RS_INT_FIELDS, RS_U8_FIELDS, SEB_FIELDS, SH_ALL, SH_FIELDS,
+                   LAKE_FIELDS, RS_FIELDS, RS_INT_FIELDS, RS_U8_FIELDS, SEB_FIELDS, SH_ALL,
+                   SH_FIELDS, LakeSolveArgs,
                    SH_INT_FIELDS, SH_U8_FIELDS, THETA_WT_FIELDS, RootSinkArgs,
                    WT_FIELDS, ZD_FIELDS, FractionsArgs, GhfArgs, HeatArgs, HeatConfig, Levels,
                    MfArgs, SebArgs, SurfHydArgs, SurfHydParams, SwmConfig, ThetaWtArgs,
@@ -19,7 +20,8 @@ from ._abi import (FRAC_FIELDS, GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_
 
 _INT_FIELDS = ({"nbedrock", "imelt", "frac_veg_nosno"} | set(GHF_INT_FIELDS) | set(SH_INT_FIELDS)
                | set(RS_INT_FIELDS))
-_U8_FIELDS = {"mask", "mask_urbanp"} | set(GHF_U8_FIELDS) | set(SH_U8_FIELDS) | set(RS_U8_FIELDS)
+_U8_FIELDS = ({"mask", "mask_urbanp", "mask_lakec"} | set(GHF_U8_FIELDS) | set(SH_U8_FIELDS)
+              | set(RS_U8_FIELDS))
 
 
 def _ptr(x):
@@ -139,6 +141,19 @@ def water_table_args(d: dict) -> WaterTableArgs:
 
 def theta_wt_args(d: dict) -> ThetaWtArgs:
     return _simple_args(ThetaWtArgs, THETA_WT_FIELDS, d)
+
+
+def lake_args(d: dict) -> LakeSolveArgs:
+    """clmsoil_lake_solve_args from a dict of arrays (`nc`, `nlevlak` integers)."""
+    a = LakeSolveArgs()
+    a.nc = int(d["nc"])
+    a.ld = int(d.get("ld", a.nc))
+    a.nlevlak = int(d["nlevlak"])
+    for f in LAKE_FIELDS:
+        x = d.get(f)
+        _check_dtype(f, x)
+        setattr(a, f, _ptr(x))
+    return a
 
 
 def rootsink_args(d: dict) -> RootSinkArgs:

@@ -726,3 +726,48 @@ def root_sink_columns(nc=4096, seed=80, nlevsoi=NLEVSOI, nlevgrnd=NLEVGRND, max_
                 npatches=npc, active=active, wtcol=wtcol, rootr_patch=rootr,
                 qflx_tran_veg_patch=qt, qflx_tran_veg_col=qcol,
                 rootr_col=np.full((nlevgrnd, nc), 7.0), qflx_rootsoi_col=np.full((nlevsoi, nc), 7.0))
+
+
+# varcon.jl:291 (the default 10 lake levels of a 50 m lake)
+DZLAK_10 = np.array([0.1, 1.0, 2.0, 3.0, 4.0, 5.0, 7.0, 7.0, 10.45, 10.45])
+
+
+def lake_columns(nc=4096, seed=90, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrnd=NLEVGRND,
+                 max_snow=6, uniform_tk=None):
+    """Lake columns for the lake temperature solve: 10 lake levels of the
+    default grid scaled to a depth ~ U(2, 60) m, 0..max_snow snow layers (dz ~
+    U(0.02, 0.2) m stacked above the surface), the CLM5 soil grid below the lake;
+    heat capacities ~ 2-4e6 J/m3/K times thickness, conductivities snow ~
+    U(0.1, 0.5), soil ~ U(0.5, 3) with 0 at the bottom interface, lake ~ U(0.5, 50)
+    (molecular plus eddy); temperatures snow 250-273 K, lake 274-295 K, soil
+    275-290 K; solar sources in snow and lake, flux into the top ~ N(0, 100) W/m2.
+    `uniform_tk` sets every conductivity to one value. 90 % of the columns are
+    in the lake filter."""
+    rng = np.random.default_rng(seed)
+    nlev = nlevsno + nlevgrnd
+    ns = rng.integers(0, max_snow + 1, nc)
+    snl = (-ns).astype(np.int64)
+    snow_dz = rng.uniform(0.02, 0.2, (nlevsno, nc))
+    z, dz, zi = _geometry(nc, snl, snow_dz, nlevsno, nlevsoi, nlevgrnd, fill=0.0)
+    depth = rng.uniform(2.0, 60.0, nc)
+    dzl = DZLAK_10[:, None] * (depth / DZLAK_10.sum())[None, :]
+    zl = np.cumsum(dzl, 0) - 0.5 * dzl
+    cv = rng.uniform(2e6, 4e6, (nlev, nc)) * dz
+    cv[:nlevsno] = rng.uniform(1e5, 2e6, (nlevsno, nc)) * dz[:nlevsno]
+    tk = np.concatenate([rng.uniform(0.1, 0.5, (nlevsno, nc)), rng.uniform(0.5, 3.0, (nlevgrnd, nc))])
+    tk[nlev - 1] = 0.0
+    tkl = rng.uniform(0.5, 50.0, (10, nc))
+    tts = rng.uniform(0.5, 3.0, nc)
+    if uniform_tk is not None:
+        tk[:] = uniform_tk
+        tkl[:] = uniform_tk
+        tts[:] = uniform_tk
+    t = np.concatenate([rng.uniform(250.0, 273.0, (nlevsno, nc)), rng.uniform(275.0, 290.0, (nlevgrnd, nc))])
+    phi = np.sort(rng.uniform(0.0, 100.0, (10, nc)), axis=0)[::-1].copy()
+    return dict(nc=nc, nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd, nlevlak=10,
+                mask_lakec=(rng.uniform(size=nc) < 0.9).astype(np.uint8), snl=snl, z=z, dz=dz,
+                cv=cv, tk=tk, sabg_lyr_col=rng.uniform(0.0, 50.0, (nlevsno + 1, nc)),
+                z_lake=zl, dz_lake=dzl, cv_lake=rng.uniform(3.5e6, 4.2e6, (10, nc)) * dzl,
+                tk_lake=tkl, phi=phi, phi_soil=rng.uniform(0.0, 10.0, nc),
+                fin=rng.normal(0.0, 100.0, nc), tktopsoillay=tts, t_soisno=t,
+                t_lake=rng.uniform(274.0, 295.0, (10, nc)))

@@ -20,6 +20,7 @@
 #include "../../include/clmsoil_ghf.h"
 #include "../../include/clmsoil_hydro.h"
 #include "../../include/clmsoil_surfhyd.h"
+#include "../../include/clmsoil_lake.h"
 #include "common.hpp"
 #include "heat.hpp"
 #include "richards.hpp"
@@ -29,6 +30,7 @@
 #include "ghf.hpp"
 #include "hydro.hpp"
 #include "surfhyd.hpp"
+#include "lake.hpp"
 
 using namespace clmsoil;
 
@@ -581,6 +583,30 @@ int clmsoil_compute_effec_rootfrac_and_vert_tran_sink(const clmsoil_rootsink_arg
     st = current_device_state(&s);
     if (st) return st;
     return launch_root_sink(*a, lev->nlevsoi, s->err, (hipStream_t)stream) == hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
+}
+
+// Steps 6-7 of lake_temperature! and the unpack (lake_temperature.jl:1205-1225).
+int clmsoil_lake_temperature_solve(const clmsoil_lake_solve_args *a, const clmsoil_levels *lev,
+                                   double dtime, void *stream) {
+    if (!a || !lev) return CLMSOIL_ERR_NULL;
+    if (lev->nlevsno < 1 || lev->nlevsno > MAX_NLEVSNO || lev->nlevgrnd < 1) return CLMSOIL_ERR_SIZE;
+    if (a->nlevlak < 1 || lev->nlevsno + a->nlevlak + lev->nlevgrnd > LK_NMAX)
+        return CLMSOIL_ERR_SIZE;
+    if (a->nc < 0 || a->ld < a->nc) return CLMSOIL_ERR_SIZE;
+    if (!(dtime > 0.0) || !std::isfinite(dtime)) return CLMSOIL_ERR_ARG;
+    if (a->nc == 0) return CLMSOIL_OK;
+    const void *req[] = {a->snl, a->z, a->cv, a->tk, a->sabg_lyr_col, a->z_lake, a->dz_lake,
+                         a->cv_lake, a->tk_lake, a->phi, a->phi_soil, a->fin, a->tktopsoillay,
+                         a->t_soisno, a->t_lake};
+    for (const void *q : req)
+        if (!q) return CLMSOIL_ERR_NULL;
+    DeviceState *s = nullptr;
+    int st = current_device_state(&s);
+    if (st) return st;
+    return launch_lake_temperature_solve(*a, lev->nlevsno, lev->nlevgrnd, dtime, s->err,
+                                         (hipStream_t)stream) == hipSuccess
                ? CLMSOIL_OK
                : CLMSOIL_ERR_HIP;
 }

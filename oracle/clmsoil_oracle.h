@@ -18,6 +18,7 @@
 #include "../include/clmsoil_ghf.h"
 #include "../include/clmsoil_hydro.h"
 #include "../include/clmsoil_surfhyd.h"
+#include "../include/clmsoil_lake.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -116,6 +117,11 @@ int orc_surface_hydrology(const clmsoil_surfhyd_args *a, const clmsoil_levels *l
 void orc_set_surfhyd_jitter(uint64_t seed);
 int orc_compute_effec_rootfrac_and_vert_tran_sink(const clmsoil_rootsink_args *a,
                                                   const clmsoil_levels *lev);
+
+/* Lake temperature solve, lake_temperature.jl:632-785, 1205-1225 (lake_oracle.c);
+ * dbg NULL or 11*N*nc doubles of the assembly's scratch arrays. */
+int orc_lake_temperature_solve(const clmsoil_lake_solve_args *a, const clmsoil_levels *lev,
+                               double dtime, double *dbg);
 /* Test hook: a non-zero seed moves every pow() of the Zeng-Decker step one ulp
  * (direction from a hash of the operands and the seed); 0 restores exact pow(). */
 void orc_set_zd_pow_jitter(uint64_t seed);

@@ -17,7 +17,7 @@ sys.path.insert(0, str(REPO / "clm.jl_amd"))
 
 from clmsoil import _abi  # noqa: E402
 from clmsoil.bind import (byref, fractions_args, ghf_args, heat_args, levels, mf_args,  # noqa: E402
-                          rootsink_args, seb_args, surfhyd_args, surfhyd_params, theta_wt_args,
+                          lake_args, rootsink_args, seb_args, surfhyd_args, surfhyd_params, theta_wt_args,
                           water_table_args, zd_args)
 
 LIB_PATH = ORACLE_DIR / "liboracle_clmsoil.so"
@@ -198,3 +198,20 @@ def compute_effec_rootfrac_and_vert_tran_sink(d: dict) -> None:
     L = lib()
     check(L.orc_compute_effec_rootfrac_and_vert_tran_sink(byref(rootsink_args(d)), byref(levels(d))),
           "compute_effec_rootfrac_and_vert_tran_sink")
+
+
+LAKE_DBG = ("zx", "cvx", "phix", "tx", "tkix", "factx", "fnx", "a", "b", "c1", "r")
+
+
+def lake_temperature_solve(d: dict, dtime: float, dbg: bool = False):
+    """The lake temperature solve on the host arrays of `d`; with `dbg` also the
+    assembly's scratch arrays, each (N, nc) over the extended levels."""
+    import numpy as np
+    L = lib()
+    nc = int(d["nc"])
+    n = int(d["nlevsno"]) + int(d["nlevlak"]) + int(d["nlevgrnd"])
+    out = np.zeros((len(LAKE_DBG), n, max(nc, 1))) if dbg else None
+    check(L.orc_lake_temperature_solve(byref(lake_args(d)), byref(levels(d)), float(dtime),
+                                       None if out is None else out.ctypes.data_as(C.c_void_p)),
+          "lake_temperature_solve")
+    return dict(zip(LAKE_DBG, out)) if dbg else None
