@@ -400,6 +400,45 @@ def lake_solve_step(nc, steps):
                          "bytes_per_launch": by}}
 
 
+def litter_transport_step(nc, steps):
+    """Vertical transport of the soil organic matter pools (the CENTURY cascade's
+    7 pools of carbon and nitrogen, 20 decomposition levels) on `nc` columns;
+    `steps` launches on the same inputs, each bracketed by HIP events."""
+    import torch
+    from clmsoil import api, inputs
+    from clmsoil._abi import LVT_FIELDS
+    d = inputs.lvt_columns(nc=nc, seed=56)
+    t = api.to_device({k: d[k] for k in LVT_FIELDS})
+    conc = [torch.from_numpy(x).cuda() for x in d["conc"]]
+    conc0 = [x.clone() for x in conc]
+    src = [torch.from_numpy(x).cuda() for x in d["source"]]
+    tend = [torch.from_numpy(x).cuda() for x in d["tendency"]]
+    st = t | {"nc": nc, "conc": conc, "source": src, "tendency": tend}
+    kw = dict(nlevdecomp=d["nlevdecomp"], nlevdecomp_full=d["nlevdecomp_full"],
+              npools=d["npools"], ntypes=d["ntypes"], is_cwd=d["is_cwd"])
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps + 2)]
+    for e in ev:
+        for x, x0 in zip(conc, conc0):
+            x.copy_(x0)
+        e[0].record()
+        api.litter_vert_transp(st, 1800.0, **kw)
+        e[1].record()
+    torch.cuda.synchronize()
+    assert api.device_errors() == 0
+    ms = sum(e[0].elapsed_time(e[1]) for e in ev[2:]) / steps
+    on = int(d["mask_bgc_soilc"].sum())
+    nd, npools, nt = d["nlevdecomp"], d["npools"], d["ntypes"]
+    # per filtered column: nbedrock, altmax twice, the two coefficient rows written
+    # and read back once; per (tracer, pool, level): concentration and source in,
+    # concentration and tendency out; plus the mask
+    by = nc + on * (3 * 8 + 2 * 2 * 8 * (nd + 1)) + on * nt * npools * nd * 4 * 8
+    return {"workload": f"{nc} columns x {npools} pools x {nt} tracers x {nd} levels",
+            "ms_per_launch": ms, "column_steps_per_s": nc / (ms / 1e3),
+            "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                         "bytes_per_launch": by}}
+
+
 def water_tables_moisture_form(nc, steps):
     """The water tables after a step without an aquifer layer (perched,
     theta-based, bedrock clamp in one launch) on `nc` columns of 20 soil layers
@@ -454,7 +493,8 @@ def main(argv=None):
                     help="run only the Zeng-Decker 2009 section (for profiling its kernel)")
     ap.add_argument("--hydro-only", action="store_true",
                     help="run only the soil hydrology sections (aquifer step, water tables "
-                         "without an aquifer, surface hydrology, root uptake, lake solve)")
+                         "without an aquifer, surface hydrology, root uptake, lake solve, "
+                         "litter transport)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -492,7 +532,8 @@ def main(argv=None):
             "water_tables_moisture_form": water_tables_moisture_form(nc, args.steps),
             "surface_hydrology": surface_hydrology_step(nc, args.steps),
             "root_sink": root_sink_step(nc, args.steps),
-            "lake_solve": lake_solve_step(nc, args.steps)}), flush=True)
+            "lake_solve": lake_solve_step(nc, args.steps),
+            "litter_transport": litter_transport_step(nc, args.steps)}), flush=True)
         return 0
     host, dev, gen_s = make_state(nc, args.seed)
     print(f"# state ready: {nc} columns in {gen_s:.1f} s", file=sys.stderr, flush=True)
@@ -584,6 +625,7 @@ def main(argv=None):
         out["surface_hydrology"] = surface_hydrology_step(nc, args.steps)
         out["root_sink"] = root_sink_step(nc, args.steps)
         out["lake_solve"] = lake_solve_step(nc, args.steps)
+        out["litter_transport"] = litter_transport_step(nc, args.steps)
     if not args.no_cpu_baseline:
         v, cores, secs = cpu_baseline(host, min(args.cpu_sample, nc), args.cpu_steps)
         out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": cores,

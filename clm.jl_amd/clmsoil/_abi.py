@@ -144,6 +144,10 @@ LAKE_FIELDS = ["mask_lakec", "snl", "z", "cv", "tk", "sabg_lyr_col", "z_lake", "
                "cv_lake", "tk_lake", "phi", "phi_soil", "fin", "tktopsoillay", "t_soisno",
                "t_lake"]
 
+LVT_FIELDS = ["mask_bgc_soilc", "nbedrock", "col_gridcell", "latdeg", "altmax", "altmax_lastyear",
+              "zsoi", "dzsoi_decomp", "zisoi", "som_diffus_coef", "som_adv_coef"]
+LVT_MAXPOOLS, LVT_MAXTYPES = 16, 4
+
 
 class FractionsArgs(C.Structure):
     """clmsoil_fractions_args (include/clmsoil_hydro.h)."""
@@ -184,6 +188,23 @@ class LakeSolveArgs(C.Structure):
     """clmsoil_lake_solve_args (include/clmsoil_lake.h)."""
     _fields_ = [("nc", C.c_int64), ("ld", C.c_int64), ("nlevlak", C.c_int64)] + \
         [(f, _p) for f in LAKE_FIELDS]
+
+
+class LvtArgs(C.Structure):
+    """clmsoil_lvt_args (include/clmsoil_lvt.h)."""
+    _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in LVT_FIELDS] + \
+        [("conc", _p * LVT_MAXTYPES), ("source", _p * LVT_MAXTYPES),
+         ("tendency", _p * LVT_MAXTYPES)]
+
+
+class LvtParams(C.Structure):
+    """clmsoil_lvt_params (include/clmsoil_lvt.h)."""
+    _fields_ = [("nlevdecomp", C.c_int32), ("nlevdecomp_full", C.c_int32), ("npools", C.c_int32),
+                ("ntypes", C.c_int32), ("spinup_state", C.c_int32),
+                ("use_soil_matrixcn", C.c_int32), ("som_diffus", C.c_double),
+                ("cryoturb_diffusion_k", C.c_double), ("max_altdepth_cryoturbation", C.c_double),
+                ("som_adv_flux", C.c_double), ("max_depth_cryoturb", C.c_double),
+                ("spinup_factor", C.c_double * LVT_MAXPOOLS), ("is_cwd", C.c_uint8 * LVT_MAXPOOLS)]
 
 
 class HeatConfig(C.Structure):
@@ -229,6 +250,9 @@ def declare_common(lib: C.CDLL, prefix: str) -> C.CDLL:
     f = getattr(lib, prefix + "lake_temperature_solve")
     # last argument: the stream (product) or the scratch copy-out, may be NULL (oracle)
     f.argtypes = [C.POINTER(LakeSolveArgs), C.POINTER(Levels), dbl, P]
+    f.restype = C.c_int
+    f = getattr(lib, prefix + "litter_vert_transp")
+    f.argtypes = [C.POINTER(LvtArgs), dbl, C.POINTER(LvtParams)] + ([P] if prefix == "clmsoil_" else [])
     f.restype = C.c_int
     f = getattr(lib, prefix + "soil_energy_balance")
     f.argtypes = [C.POINTER(SebArgs), C.POINTER(Levels), dbl] + ([P] if prefix == "clmsoil_" else [])

@@ -24,7 +24,7 @@ from dataclasses import dataclass, field
 
 from . import _abi
 from .bind import (byref, fractions_args, ghf_args, heat_args, heat_config, levels, mf_args,
-                   lake_args, rootsink_args, seb_args, surfhyd_args, surfhyd_params, swm_config, theta_wt_args,
+                   lake_args, lvt_args, lvt_params, rootsink_args, seb_args, surfhyd_args, surfhyd_params, swm_config, theta_wt_args,
                    water_table_args, zd_args)
 
 ZENGDECKER_2009, MOISTURE_FORM = _abi.ZENGDECKER_2009, _abi.MOISTURE_FORM
@@ -273,6 +273,21 @@ def surface_hydrology(state: dict, dtime: float, nlevsno: int = 12, nlevsoi: int
     _abi.check(L, L.clmsoil_surface_hydrology(byref(surfhyd_args(d)), byref(levels(d)),
                                               float(dtime), byref(surfhyd_params(**params)),
                                               _stream()), "surface_hydrology")
+
+
+def litter_vert_transp(state: dict, dtime: float, **params) -> None:
+    """litter_vert_transp! without the matrix-CN path (litter_vert_transp.jl:479-639):
+    the vertical advection-diffusion of every pool of every tracer type given.
+    `state` maps the clmsoil_lvt_args fields (include/clmsoil_lvt.h) to device
+    tensors, `conc`/`source`/`tendency` to lists of [npools, nlevdecomp_full, nc]
+    tensors, plus `nc`; `params` are the keywords of bind.lvt_params."""
+    d = dict(state)
+    if int(d["nc"]) == 0:
+        return
+    L = _lib()
+    _abi.check(L, L.clmsoil_litter_vert_transp(byref(lvt_args(d)), float(dtime),
+                                               byref(lvt_params(**params)), _stream()),
+               "litter_vert_transp")
 
 
 def lake_temperature_solve(state: dict, dtime: float, nlevsno: int = 12, nlevsoi: int = 20,

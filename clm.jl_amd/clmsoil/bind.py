@@ -11,17 +11,18 @@ import ctypes as C
 import numpy as np
 
 from ._abi import (FRAC_FIELDS, GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_FIELDS, MF_FIELDS,
-                   LAKE_FIELDS, RS_FIELDS, RS_INT_FIELDS, RS_U8_FIELDS, SEB_FIELDS, SH_ALL,
+                   LAKE_FIELDS, LVT_FIELDS, LVT_MAXPOOLS, LVT_MAXTYPES, LvtArgs, LvtParams,
+                   RS_FIELDS, RS_INT_FIELDS, RS_U8_FIELDS, SEB_FIELDS, SH_ALL,
                    SH_FIELDS, LakeSolveArgs,
                    SH_INT_FIELDS, SH_U8_FIELDS, THETA_WT_FIELDS, RootSinkArgs,
                    WT_FIELDS, ZD_FIELDS, FractionsArgs, GhfArgs, HeatArgs, HeatConfig, Levels,
                    MfArgs, SebArgs, SurfHydArgs, SurfHydParams, SwmConfig, ThetaWtArgs,
                    WaterTableArgs, ZdArgs)
 
-_INT_FIELDS = ({"nbedrock", "imelt", "frac_veg_nosno"} | set(GHF_INT_FIELDS) | set(SH_INT_FIELDS)
-               | set(RS_INT_FIELDS))
-_U8_FIELDS = ({"mask", "mask_urbanp", "mask_lakec"} | set(GHF_U8_FIELDS) | set(SH_U8_FIELDS)
-              | set(RS_U8_FIELDS))
+_INT_FIELDS = ({"nbedrock", "imelt", "frac_veg_nosno", "col_gridcell"} | set(GHF_INT_FIELDS)
+               | set(SH_INT_FIELDS) | set(RS_INT_FIELDS))
+_U8_FIELDS = ({"mask", "mask_urbanp", "mask_lakec", "mask_bgc_soilc"} | set(GHF_U8_FIELDS)
+              | set(SH_U8_FIELDS) | set(RS_U8_FIELDS))
 
 
 def _ptr(x):
@@ -141,6 +142,49 @@ def water_table_args(d: dict) -> WaterTableArgs:
 
 def theta_wt_args(d: dict) -> ThetaWtArgs:
     return _simple_args(ThetaWtArgs, THETA_WT_FIELDS, d)
+
+
+def lvt_args(d: dict) -> LvtArgs:
+    """clmsoil_lvt_args from a dict of arrays: the LVT_FIELDS plus lists `conc`,
+    `source`, `tendency` (one array per tracer type) and `nc`."""
+    a = LvtArgs()
+    a.nc = int(d["nc"])
+    a.ld = int(d.get("ld", a.nc))
+    for f in LVT_FIELDS:
+        x = d.get(f)
+        _check_dtype(f, x)
+        setattr(a, f, _ptr(x))
+    for key in ("conc", "source", "tendency"):
+        xs = list(d[key])
+        if len(xs) > LVT_MAXTYPES:
+            raise ValueError(f"at most {LVT_MAXTYPES} tracer types")
+        arr = getattr(a, key)
+        for i, x in enumerate(xs):
+            _check_dtype(key, x)
+            arr[i] = _ptr(x)
+    return a
+
+
+def lvt_params(nlevdecomp, nlevdecomp_full, npools, ntypes, is_cwd, spinup_factor=None,
+               spinup_state=0, som_diffus=1.0e-4 / (365.0 * 86400.0),
+               cryoturb_diffusion_k=5.0e-4 / (365.0 * 86400.0), max_altdepth_cryoturbation=2.0,
+               som_adv_flux=0.0, max_depth_cryoturb=3.0, use_soil_matrixcn=False) -> LvtParams:
+    """clmsoil_lvt_params. The reference reads som_diffus, cryoturb_diffusion_k and
+    max_altdepth_cryoturbation from its parameter file (litter_vert_transp.jl:29-33);
+    the defaults here are 1e-4 and 5e-4 m2/yr and 2 m. som_adv_flux 0 and
+    max_depth_cryoturb 3 m are the reference's (:36-37)."""
+    p = LvtParams()
+    p.nlevdecomp, p.nlevdecomp_full, p.npools, p.ntypes = (int(nlevdecomp), int(nlevdecomp_full),
+                                                           int(npools), int(ntypes))
+    p.spinup_state, p.use_soil_matrixcn = int(spinup_state), int(bool(use_soil_matrixcn))
+    p.som_diffus, p.cryoturb_diffusion_k = float(som_diffus), float(cryoturb_diffusion_k)
+    p.max_altdepth_cryoturbation, p.som_adv_flux = float(max_altdepth_cryoturbation), float(som_adv_flux)
+    p.max_depth_cryoturb = float(max_depth_cryoturb)
+    sf = [1.0] * npools if spinup_factor is None else list(spinup_factor)
+    for i in range(npools):
+        p.spinup_factor[i] = float(sf[i])
+        p.is_cwd[i] = int(bool(is_cwd[i]))
+    return p
 
 
 def lake_args(d: dict) -> LakeSolveArgs:

@@ -771,3 +771,40 @@ def lake_columns(nc=4096, seed=90, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrnd=NL
                 tk_lake=tkl, phi=phi, phi_soil=rng.uniform(0.0, 10.0, nc),
                 fin=rng.normal(0.0, 100.0, nc), tktopsoillay=tts, t_soisno=t,
                 t_lake=rng.uniform(274.0, 295.0, (10, nc)))
+
+
+def lvt_columns(nc=4096, seed=110, nlevdecomp=NLEVSOI, nlevdecomp_full=NLEVGRND, npools=7,
+                ntypes=2, cwd_pools=(3,), ng=None):
+    """Columns for the vertical transport of the soil organic matter pools: the
+    CLM5 grid's first nlevdecomp levels, bedrock at layer U{3..nlevdecomp}, active
+    layer thickness ~ U(0, 4) m this year and last year (a fifth of the columns
+    0: frozen, no mixing; those below 2 m mix by cryoturbation), pools
+    [npools, nlevdecomp_full, nc] (the reference's [c, j, s]) with carbon
+    ~ U(0.5, 1.5) * 1000 * exp(-z) g/m3 and nitrogen at C/N ~ U(10, 30), sources
+    ~ U(-0.01, 0.05) g/m3 per step; pool indices in cwd_pools are coarse woody
+    debris (CENTURY's fourth pool). 90 % of the columns in the filter."""
+    rng = np.random.default_rng(seed)
+    dzs, zs, zis = clm5_grid(NLEVSOI, NLEVGRND)
+    nd = nlevdecomp
+    ng = ng or max(1, nc // 4)
+    alt = np.where(rng.uniform(size=nc) < 0.2, 0.0, rng.uniform(0.0, 4.0, nc))
+    shape = (npools, nlevdecomp_full, nc)
+    prof = np.exp(-zs[:nlevdecomp_full])[None, :, None]
+    cpool = rng.uniform(0.5, 1.5, shape) * 1000.0 * prof
+    conc = [cpool]
+    if ntypes >= 2:
+        conc.append(cpool / rng.uniform(10.0, 30.0, shape))
+    for _ in range(2, ntypes):
+        conc.append(cpool * rng.uniform(0.9, 1.1, shape))
+    source = [rng.uniform(-0.01, 0.05, shape) for _ in range(ntypes)]
+    is_cwd = [i in cwd_pools for i in range(npools)]
+    return dict(nc=nc, nlevdecomp=nd, nlevdecomp_full=nlevdecomp_full, npools=npools,
+                ntypes=ntypes, is_cwd=is_cwd,
+                mask_bgc_soilc=(rng.uniform(size=nc) < 0.9).astype(np.uint8),
+                nbedrock=rng.integers(3, nd + 1, nc).astype(np.int64),
+                col_gridcell=rng.integers(1, ng + 1, nc).astype(np.int64),
+                latdeg=rng.uniform(-80.0, 80.0, ng),
+                altmax=alt, altmax_lastyear=np.where(alt > 0, rng.uniform(0.0, 4.0, nc), 0.0),
+                zsoi=zs[:nd].copy(), dzsoi_decomp=dzs[:nd].copy(), zisoi=zis[:nd + 2].copy(),
+                som_diffus_coef=np.full((nd + 1, nc), 7.0), som_adv_coef=np.full((nd + 1, nc), 7.0),
+                conc=conc, source=source, tendency=[np.full(shape, 7.0) for _ in range(ntypes)])

@@ -21,6 +21,7 @@
 #include "../../include/clmsoil_hydro.h"
 #include "../../include/clmsoil_surfhyd.h"
 #include "../../include/clmsoil_lake.h"
+#include "../../include/clmsoil_lvt.h"
 #include "common.hpp"
 #include "heat.hpp"
 #include "richards.hpp"
@@ -31,6 +32,7 @@
 #include "hydro.hpp"
 #include "surfhyd.hpp"
 #include "lake.hpp"
+#include "lvt.hpp"
 
 using namespace clmsoil;
 
@@ -607,6 +609,38 @@ int clmsoil_lake_temperature_solve(const clmsoil_lake_solve_args *a, const clmso
     if (st) return st;
     return launch_lake_temperature_solve(*a, lev->nlevsno, lev->nlevgrnd, dtime, s->err,
                                          (hipStream_t)stream) == hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
+}
+
+// litter_vert_transp! without the matrix-CN path (litter_vert_transp.jl:479-639).
+int clmsoil_litter_vert_transp(const clmsoil_lvt_args *a, double dtime,
+                               const clmsoil_lvt_params *p, void *stream) {
+    if (!a || !p) return CLMSOIL_ERR_NULL;
+    if (p->use_soil_matrixcn) return CLMSOIL_ERR_METHOD;
+    if (p->nlevdecomp < 1 || p->nlevdecomp > 62 || p->nlevdecomp_full < p->nlevdecomp ||
+        p->npools < 1 || p->npools > CLMSOIL_LVT_MAXPOOLS || p->ntypes < 1 ||
+        p->ntypes > CLMSOIL_LVT_MAXTYPES)
+        return CLMSOIL_ERR_SIZE;
+    if (a->nc < 0 || a->ld < a->nc) return CLMSOIL_ERR_SIZE;
+    if (!(dtime > 0.0) || !std::isfinite(dtime)) return CLMSOIL_ERR_ARG;
+    if (a->nc == 0) return CLMSOIL_OK;
+    const void *req[] = {a->nbedrock, a->altmax, a->altmax_lastyear, a->zsoi, a->dzsoi_decomp,
+                         a->zisoi, a->som_diffus_coef, a->som_adv_coef};
+    for (const void *q : req)
+        if (!q) return CLMSOIL_ERR_NULL;
+    for (int t = 0; t < p->ntypes; ++t)
+        if (!a->conc[t] || !a->source[t] || !a->tendency[t]) return CLMSOIL_ERR_NULL;
+    if (p->spinup_state >= 1) {
+        bool lat = false;
+        for (int s = 0; s < p->npools; ++s)
+            if (!p->is_cwd[s] && std::fabs(p->spinup_factor[s] - 1.0) > 1.0e-6) lat = true;
+        if (lat && (!a->col_gridcell || !a->latdeg)) return CLMSOIL_ERR_NULL;
+    }
+    DeviceState *s = nullptr;
+    int st = current_device_state(&s);
+    if (st) return st;
+    return launch_litter_vert_transp(*a, *p, dtime, s->err, (hipStream_t)stream) == hipSuccess
                ? CLMSOIL_OK
                : CLMSOIL_ERR_HIP;
 }

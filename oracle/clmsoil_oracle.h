@@ -19,6 +19,7 @@
 #include "../include/clmsoil_hydro.h"
 #include "../include/clmsoil_surfhyd.h"
 #include "../include/clmsoil_lake.h"
+#include "../include/clmsoil_lvt.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -122,6 +123,10 @@ int orc_compute_effec_rootfrac_and_vert_tran_sink(const clmsoil_rootsink_args *a
  * dbg NULL or 11*N*nc doubles of the assembly's scratch arrays. */
 int orc_lake_temperature_solve(const clmsoil_lake_solve_args *a, const clmsoil_levels *lev,
                                double dtime, double *dbg);
+
+/* Soil organic matter vertical transport, litter_vert_transp.jl:93-639 (lvt_oracle.c) */
+double orc_jl_pow5(double x);
+int orc_litter_vert_transp(const clmsoil_lvt_args *a, double dtime, const clmsoil_lvt_params *p);
 /* Test hook: a non-zero seed moves every pow() of the Zeng-Decker step one ulp
  * (direction from a hash of the operands and the seed); 0 restores exact pow(). */
 void orc_set_zd_pow_jitter(uint64_t seed);

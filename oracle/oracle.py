@@ -17,7 +17,7 @@ sys.path.insert(0, str(REPO / "clm.jl_amd"))
 
 from clmsoil import _abi  # noqa: E402
 from clmsoil.bind import (byref, fractions_args, ghf_args, heat_args, levels, mf_args,  # noqa: E402
-                          lake_args, rootsink_args, seb_args, surfhyd_args, surfhyd_params, theta_wt_args,
+                          lake_args, lvt_args, lvt_params, rootsink_args, seb_args, surfhyd_args, surfhyd_params, theta_wt_args,
                           water_table_args, zd_args)
 
 LIB_PATH = ORACLE_DIR / "liboracle_clmsoil.so"
@@ -215,3 +215,16 @@ def lake_temperature_solve(d: dict, dtime: float, dbg: bool = False):
                                        None if out is None else out.ctypes.data_as(C.c_void_p)),
           "lake_temperature_solve")
     return dict(zip(LAKE_DBG, out)) if dbg else None
+
+
+def litter_vert_transp(d: dict, dtime: float, **params) -> None:
+    L = lib()
+    check(L.orc_litter_vert_transp(byref(lvt_args(d)), float(dtime), byref(lvt_params(**params))),
+          "litter_vert_transp")
+
+
+def jl_pow5(x: float) -> float:
+    L = lib()
+    L.orc_jl_pow5.argtypes = [C.c_double]
+    L.orc_jl_pow5.restype = C.c_double
+    return float(L.orc_jl_pow5(float(x)))
