@@ -27,7 +27,8 @@ def run_gpu(d, dtime, **kw):
     conc = [torch.from_numpy(x).cuda() for x in d["conc"]]
     src = [torch.from_numpy(x).cuda() for x in d["source"]]
     tend = [torch.from_numpy(x).cuda() for x in d["tendency"]]
-    api.litter_vert_transp(t | {"nc": d["nc"], "conc": conc, "source": src, "tendency": tend},
+    api.litter_vert_transp(t | {"nc": d["nc"], "ld": d.get("ld", d["nc"]), "conc": conc,
+                                "source": src, "tendency": tend},
                            dtime, **params(d, **kw))
     torch.cuda.synchronize()
     out = api.to_host(t)
@@ -106,3 +107,39 @@ def test_lvt_bad_nbedrock_flags(gpu):
     ref["mask_bgc_soilc"][11] = 0
     ref = run_oracle(ref, 1800.0)
     assert np.array_equal(got["conc"][0], ref["conc"][0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nd", [10, 7])
+def test_lvt_other_level_counts(gpu, nd):
+    """10 levels take the shared-matrix kernel (one factorisation per column for all
+    pools and tracers), 7 the per-system kernel; both bit-exact, with and without a
+    mix of spinup factors (the pools with factor 1 stay on the shared matrix)."""
+    d = inputs.lvt_columns(nc=1003, seed=121 + nd, nlevdecomp=nd)
+    for kw in ({}, dict(spinup_state=1, spinup_factor=[1.0, 1.0, 1.0, 1.0, 5.0, 1.0, 30.0])):
+        kw = dict(kw, som_diffus=1e-8, cryoturb_diffusion_k=5e-8, som_adv_flux=3e-10)
+        got = run_gpu(d, 3.0e7, **kw)
+        ref = run_oracle(d, 3.0e7, **kw)
+        spin = "spinup_state" in kw
+        for t in range(d["ntypes"]):
+            for s in range(d["npools"]):
+                a, b = got["conc"][t][s], ref["conc"][t][s]
+                if spin and s in (4, 6):                 # exp() of the latitude term
+                    assert rel_err(a, b, floor=1e-12) <= 1e-9, (t, s)
+                else:
+                    assert np.array_equal(a, b), (t, s)
+                    assert np.array_equal(got["tendency"][t][s], ref["tendency"][t][s]), (t, s)
+
+
+@pytest.mark.gpu
+def test_lvt_padded_leading_dimension(gpu):
+    """nc below the arrays' leading dimension: the columns past nc stay as they were
+    and the others match the oracle bit for bit."""
+    d = inputs.lvt_columns(nc=700, seed=124)
+    d["ld"], d["nc"] = 700, 611
+    got = run_gpu(d, 1800.0)
+    ref = run_oracle(d, 1800.0)
+    for t in range(d["ntypes"]):
+        assert np.array_equal(got["conc"][t][:, :, 611:], d["conc"][t][:, :, 611:]), t
+        assert np.array_equal(got["conc"][t], ref["conc"][t]), t
+        assert np.array_equal(got["tendency"][t], ref["tendency"][t]), t
