@@ -430,8 +430,10 @@ def litter_transport_step(nc, steps):
     nd, npools, nt = d["nlevdecomp"], d["npools"], d["ntypes"]
     # per filtered column: nbedrock, altmax twice, the two coefficient rows written
     # and read back once; per (tracer, pool, level): concentration and source in,
-    # concentration and tendency out; plus the mask
-    by = nc + on * (3 * 8 + 2 * 2 * 8 * (nd + 1)) + on * nt * npools * nd * 4 * 8
+    # concentration and tendency out (coarse woody debris: no tendency); plus the mask
+    ncwd = int(sum(bool(x) for x in d["is_cwd"]))
+    by = (nc + on * (3 * 8 + 2 * 2 * 8 * (nd + 1))
+          + on * nt * nd * 8 * (4 * (npools - ncwd) + 3 * ncwd))
     return {"workload": f"{nc} columns x {npools} pools x {nt} tracers x {nd} levels",
             "ms_per_launch": ms, "column_steps_per_s": nc / (ms / 1e3),
             "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
