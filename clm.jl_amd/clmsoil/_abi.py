@@ -148,6 +148,14 @@ LVT_FIELDS = ["mask_bgc_soilc", "nbedrock", "col_gridcell", "latdeg", "altmax", 
               "zsoi", "dzsoi_decomp", "zisoi", "som_diffus_coef", "som_adv_coef"]
 LVT_MAXPOOLS, LVT_MAXTYPES = 16, 4
 
+CH4_FIELDS = ["mask", "col_gridcell", "jwt", "c_atm", "watsat", "h2osoi_vol", "h2osoi_liq",
+              "h2osoi_ice", "bsw", "cellorg", "t_soisno", "dz", "h2osfc", "t_grnd", "t_h2osfc",
+              "frac_h2osfc", "grnd_ch4_cond", "ch4_prod_depth", "o2_aere_depth", "o2_decomp_depth",
+              "o2_oxid_depth", "ch4_oxid_depth", "ch4_aere_depth", "ch4_ebul_depth", "o2stress",
+              "ch4stress", "conc_ch4", "conc_o2", "ch4_ebul_total", "ch4_surf_aere", "ch4_surf_ebul",
+              "ch4_surf_diff"]
+CH4_INT_FIELDS = ["jwt"]
+
 
 class FractionsArgs(C.Structure):
     """clmsoil_fractions_args (include/clmsoil_hydro.h)."""
@@ -207,6 +215,21 @@ class LvtParams(C.Structure):
                 ("spinup_factor", C.c_double * LVT_MAXPOOLS), ("is_cwd", C.c_uint8 * LVT_MAXPOOLS)]
 
 
+class Ch4TranArgs(C.Structure):
+    """clmsoil_ch4_tran_args (include/clmsoil_ch4.h)."""
+    _fields_ = [("nc", C.c_int64), ("ld", C.c_int64), ("ldg", C.c_int64)] + \
+        [(f, _p) for f in CH4_FIELDS]
+
+
+class Ch4TranParams(C.Structure):
+    """clmsoil_ch4_tran_params (include/clmsoil_ch4.h)."""
+    _fields_ = [("nlevsoi", C.c_int32), ("sat", C.c_int32), ("lake", C.c_int32),
+                ("ch4frzout", C.c_int32), ("use_aereoxid_prog", C.c_int32), ("reserved", C.c_int32),
+                ("satpow", C.c_double), ("scale_factor_gasdiff", C.c_double),
+                ("scale_factor_liqdiff", C.c_double), ("capthick", C.c_double),
+                ("aereoxid", C.c_double), ("om_frac_sf", C.c_double), ("organic_max", C.c_double)]
+
+
 class HeatConfig(C.Structure):
     _fields_ = [("snow_thermal_cond_method", C.c_int32), ("reserved", C.c_int32)]
 
@@ -253,6 +276,10 @@ def declare_common(lib: C.CDLL, prefix: str) -> C.CDLL:
     f.restype = C.c_int
     f = getattr(lib, prefix + "litter_vert_transp")
     f.argtypes = [C.POINTER(LvtArgs), dbl, C.POINTER(LvtParams)] + ([P] if prefix == "clmsoil_" else [])
+    f.restype = C.c_int
+    f = getattr(lib, prefix + "ch4_tran")
+    f.argtypes = [C.POINTER(Ch4TranArgs), dbl, C.POINTER(Ch4TranParams)] + \
+        ([P] if prefix == "clmsoil_" else [])
     f.restype = C.c_int
     f = getattr(lib, prefix + "soil_energy_balance")
     f.argtypes = [C.POINTER(SebArgs), C.POINTER(Levels), dbl] + ([P] if prefix == "clmsoil_" else [])

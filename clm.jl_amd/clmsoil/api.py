@@ -24,7 +24,8 @@ from dataclasses import dataclass, field
 
 from . import _abi
 from .bind import (byref, fractions_args, ghf_args, heat_args, heat_config, levels, mf_args,
-                   lake_args, lvt_args, lvt_params, rootsink_args, seb_args, surfhyd_args, surfhyd_params, swm_config, theta_wt_args,
+                   ch4_tran_args, ch4_tran_params, lake_args, lvt_args, lvt_params, rootsink_args,
+                   seb_args, surfhyd_args, surfhyd_params, swm_config, theta_wt_args,
                    water_table_args, zd_args)
 
 ZENGDECKER_2009, MOISTURE_FORM = _abi.ZENGDECKER_2009, _abi.MOISTURE_FORM
@@ -288,6 +289,20 @@ def litter_vert_transp(state: dict, dtime: float, **params) -> None:
     _abi.check(L, L.clmsoil_litter_vert_transp(byref(lvt_args(d)), float(dtime),
                                                byref(lvt_params(**params)), _stream()),
                "litter_vert_transp")
+
+
+def ch4_tran(state: dict, dtime: float, **params) -> None:
+    """ch4_tran! (methane.jl:2005-2092): the CH4 and O2 reaction-diffusion step of one
+    fraction (sat 0 unsaturated, 1 saturated; lake for the lake call). `state` maps
+    the clmsoil_ch4_tran_args fields (include/clmsoil_ch4.h) to device tensors,
+    [nlevsoi, nc] for the level arrays and [3, ldg] for c_atm, plus `nc` and `ldg`;
+    `params` are the keywords of bind.ch4_tran_params (nlevsoi required)."""
+    d = dict(state)
+    if int(d["nc"]) == 0:
+        return
+    L = _lib()
+    _abi.check(L, L.clmsoil_ch4_tran(byref(ch4_tran_args(d)), float(dtime),
+                                     byref(ch4_tran_params(**params)), _stream()), "ch4_tran")
 
 
 def lake_temperature_solve(state: dict, dtime: float, nlevsno: int = 12, nlevsoi: int = 20,

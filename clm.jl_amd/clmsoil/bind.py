@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import (FRAC_FIELDS, GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_FIELDS, MF_FIELDS,
+from ._abi import (CH4_FIELDS, CH4_INT_FIELDS, Ch4TranArgs, Ch4TranParams, FRAC_FIELDS, GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_FIELDS, MF_FIELDS,
                    LAKE_FIELDS, LVT_FIELDS, LVT_MAXPOOLS, LVT_MAXTYPES, LvtArgs, LvtParams,
                    RS_FIELDS, RS_INT_FIELDS, RS_U8_FIELDS, SEB_FIELDS, SH_ALL,
                    SH_FIELDS, LakeSolveArgs,
@@ -20,7 +20,7 @@ from ._abi import (FRAC_FIELDS, GHF_FIELDS, GHF_INT_FIELDS, GHF_U8_FIELDS, HEAT_
                    WaterTableArgs, ZdArgs)
 
 _INT_FIELDS = ({"nbedrock", "imelt", "frac_veg_nosno", "col_gridcell"} | set(GHF_INT_FIELDS)
-               | set(SH_INT_FIELDS) | set(RS_INT_FIELDS))
+               | set(SH_INT_FIELDS) | set(RS_INT_FIELDS) | set(CH4_INT_FIELDS))
 _U8_FIELDS = ({"mask", "mask_urbanp", "mask_lakec", "mask_bgc_soilc"} | set(GHF_U8_FIELDS)
               | set(SH_U8_FIELDS) | set(RS_U8_FIELDS))
 
@@ -184,6 +184,36 @@ def lvt_params(nlevdecomp, nlevdecomp_full, npools, ntypes, is_cwd, spinup_facto
     for i in range(npools):
         p.spinup_factor[i] = float(sf[i])
         p.is_cwd[i] = int(bool(is_cwd[i]))
+    return p
+
+
+def ch4_tran_args(d: dict) -> Ch4TranArgs:
+    """clmsoil_ch4_tran_args from a dict of arrays: the CH4_FIELDS plus `nc` and
+    optionally `ld` and `ldg` (the gridcell count, c_atm's leading dimension)."""
+    a = Ch4TranArgs()
+    a.nc = int(d["nc"])
+    a.ld = int(d.get("ld", a.nc))
+    a.ldg = int(d["ldg"])
+    for f in CH4_FIELDS:
+        x = d.get(f)
+        _check_dtype(f, x)
+        setattr(a, f, _ptr(x))
+    return a
+
+
+def ch4_tran_params(nlevsoi, sat=0, lake=False, ch4frzout=False, use_aereoxid_prog=True,
+                    satpow=2.0, scale_factor_gasdiff=1.0, scale_factor_liqdiff=1.0,
+                    capthick=100.0, aereoxid=0.5, om_frac_sf=1.0,
+                    organic_max=130.0) -> Ch4TranParams:
+    """clmsoil_ch4_tran_params. The defaults are CH4Params' (methane.jl:214-236),
+    CH4VarCon's (ch4frzout false, use_aereoxid_prog true) and the driver's
+    organic_max of 130 kg/m3 (clm_initialize.jl:586)."""
+    p = Ch4TranParams()
+    p.nlevsoi, p.sat, p.lake = int(nlevsoi), int(sat), int(bool(lake))
+    p.ch4frzout, p.use_aereoxid_prog = int(bool(ch4frzout)), int(bool(use_aereoxid_prog))
+    p.satpow, p.scale_factor_gasdiff = float(satpow), float(scale_factor_gasdiff)
+    p.scale_factor_liqdiff, p.capthick = float(scale_factor_liqdiff), float(capthick)
+    p.aereoxid, p.om_frac_sf, p.organic_max = float(aereoxid), float(om_frac_sf), float(organic_max)
     return p
 
 

@@ -808,3 +808,62 @@ def lvt_columns(nc=4096, seed=110, nlevdecomp=NLEVSOI, nlevdecomp_full=NLEVGRND,
                 zsoi=zs[:nd].copy(), dzsoi_decomp=dzs[:nd].copy(), zisoi=zis[:nd + 2].copy(),
                 som_diffus_coef=np.full((nd + 1, nc), 7.0), som_adv_coef=np.full((nd + 1, nc), 7.0),
                 conc=conc, source=source, tendency=[np.full(shape, 7.0) for _ in range(ntypes)])
+
+
+def ch4_columns(nc=4096, seed=130, nlevsoi=NLEVSOI, sat=0, ng=None):
+    """Columns for the methane and oxygen transport step (one fraction of ch4!):
+    the CLM5 grid's first nlevsoi levels; porosity U(0.4, 0.6), volumetric water
+    U(0.1, 1.05) of it (some above saturation), a tenth of the levels frozen
+    (ice share U(0.3, 1)), Clapp-Hornberger b U(3, 12), organic matter
+    U(0, 160) kg/m3 (some above the peat density 130), soil temperatures
+    U(265, 300) K; ponded water U(0, 200) mm on U(0, 1) of the area (a third
+    dry); ground conductance U(0, 0.05) m/s (a tenth 0). Atmospheric CH4 1.6-2.0
+    ppm and O2 20.9 % at U(250, 305) K and 1 atm. Concentrations: CH4
+    U(0, 0.5) mol/m3, O2 U(0.1, 8) mol/m3; process rates (mol/m3/s): production
+    U(0, 1e-6), oxidation U(0, 5e-7) with twice that O2, aerenchyma U(0, 2e-7)
+    (O2 in U(0, 1e-6)), ebullition U(0, 1e-7) on half the levels, decomposition
+    O2 demand U(0, 1e-5); a fifth of the levels starved (concentrations
+    scaled by 1e-4) so that the O2/CH4 competition binds. jwt U{0..nlevsoi} for
+    the unsaturated fraction (sat 0), 0 for the saturated one. 90 % of the columns
+    in the filter."""
+    rng = np.random.default_rng(seed)
+    dzs, _, _ = clm5_grid(NLEVSOI, NLEVGRND)
+    nl = nlevsoi
+    ng = ng or max(1, nc // 4)
+    shp = (nl, nc)
+    u = lambda lo, hi: rng.uniform(lo, hi, shp)
+    dz = np.repeat(dzs[:nl, None], nc, axis=1).copy()
+    watsat = u(0.4, 0.6)
+    vol = watsat * u(0.1, 1.05)
+    frozen = rng.uniform(size=shp) < 0.1
+    icef = np.where(frozen, u(0.3, 1.0), 0.0)
+    t_soisno = np.where(frozen, u(263.0, 273.15), u(265.0, 300.0))
+    liq = np.minimum(vol, watsat) * dz * DENH2O * (1.0 - icef)
+    ice = np.minimum(vol, watsat) * dz * DENICE * icef
+    starved = rng.uniform(size=shp) < 0.2
+    scale = np.where(starved, 1e-4, 1.0)
+    tatm = rng.uniform(250.0, 305.0, ng)
+    rgas = 8.3145
+    c_atm = np.stack([rng.uniform(1.6e-6, 2.0e-6, ng) * 101325.0 / rgas / tatm,
+                      0.209 * 101325.0 / rgas / tatm,
+                      4.0e-4 * 101325.0 / rgas / tatm])
+    frac = np.where(rng.uniform(size=nc) < 1.0 / 3.0, 0.0, rng.uniform(0.0, 1.0, nc))
+    gcond = np.where(rng.uniform(size=nc) < 0.1, 0.0, rng.uniform(0.0, 0.05, nc))
+    oxid = u(0.0, 5e-7)
+    jwt = rng.integers(0, nl + 1, nc) if sat == 0 else np.zeros(nc, np.int64)
+    return dict(nc=nc, ldg=ng, nlevsoi=nl,
+                mask=(rng.uniform(size=nc) < 0.9).astype(np.uint8),
+                col_gridcell=rng.integers(1, ng + 1, nc).astype(np.int64),
+                jwt=jwt.astype(np.int64), c_atm=c_atm,
+                watsat=watsat, h2osoi_vol=vol, h2osoi_liq=liq, h2osoi_ice=ice, bsw=u(3.0, 12.0),
+                cellorg=u(0.0, 160.0), t_soisno=t_soisno, dz=dz,
+                h2osfc=rng.uniform(0.0, 200.0, nc), t_grnd=rng.uniform(265.0, 305.0, nc),
+                t_h2osfc=rng.uniform(268.0, 300.0, nc), frac_h2osfc=frac, grnd_ch4_cond=gcond,
+                ch4_prod_depth=u(0.0, 1e-6), o2_aere_depth=u(0.0, 1e-6),
+                o2_decomp_depth=u(0.0, 1e-5), o2_oxid_depth=2.0 * oxid, ch4_oxid_depth=oxid,
+                ch4_aere_depth=u(0.0, 2e-7),
+                ch4_ebul_depth=np.where(rng.uniform(size=shp) < 0.5, u(0.0, 1e-7), 0.0),
+                o2stress=np.full(shp, 7.0), ch4stress=np.full(shp, 7.0),
+                conc_ch4=u(0.0, 0.5) * scale, conc_o2=u(0.1, 8.0) * scale,
+                ch4_ebul_total=np.full(nc, 7.0), ch4_surf_aere=np.full(nc, 7.0),
+                ch4_surf_ebul=np.full(nc, 7.0), ch4_surf_diff=np.full(nc, 7.0))

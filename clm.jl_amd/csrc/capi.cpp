@@ -33,6 +33,7 @@
 #include "surfhyd.hpp"
 #include "lake.hpp"
 #include "lvt.hpp"
+#include "ch4.hpp"
 
 using namespace clmsoil;
 
@@ -643,6 +644,31 @@ int clmsoil_litter_vert_transp(const clmsoil_lvt_args *a, double dtime,
     return launch_litter_vert_transp(*a, *p, dtime, s->err, (hipStream_t)stream) == hipSuccess
                ? CLMSOIL_OK
                : CLMSOIL_ERR_HIP;
+}
+
+// ch4_tran! (methane.jl:2005-2092).
+int clmsoil_ch4_tran(const clmsoil_ch4_tran_args *a, double dtime, const clmsoil_ch4_tran_params *p,
+                     void *stream) {
+    if (!a || !p) return CLMSOIL_ERR_NULL;
+    if (p->nlevsoi < 1 || p->nlevsoi > 60) return CLMSOIL_ERR_SIZE;
+    if (p->sat != 0 && p->sat != 1) return CLMSOIL_ERR_ARG;
+    if (a->nc < 0 || a->ld < a->nc || a->ldg < 1) return CLMSOIL_ERR_SIZE;
+    if (!(dtime > 0.0) || !std::isfinite(dtime)) return CLMSOIL_ERR_ARG;
+    if (a->nc == 0) return CLMSOIL_OK;
+    const void *req[] = {a->col_gridcell, a->jwt, a->c_atm, a->watsat, a->h2osoi_vol, a->h2osoi_liq,
+                         a->h2osoi_ice, a->bsw, a->cellorg, a->t_soisno, a->dz, a->h2osfc, a->t_grnd,
+                         a->t_h2osfc, a->frac_h2osfc, a->grnd_ch4_cond, a->ch4_prod_depth,
+                         a->o2_aere_depth, a->o2_decomp_depth, a->o2_oxid_depth, a->ch4_oxid_depth,
+                         a->ch4_aere_depth, a->ch4_ebul_depth, a->o2stress, a->ch4stress, a->conc_ch4,
+                         a->conc_o2, a->ch4_ebul_total, a->ch4_surf_aere, a->ch4_surf_ebul,
+                         a->ch4_surf_diff};
+    for (const void *q : req)
+        if (!q) return CLMSOIL_ERR_NULL;
+    DeviceState *s = nullptr;
+    int st = current_device_state(&s);
+    if (st) return st;
+    return launch_ch4_tran(*a, *p, dtime, s->err, (hipStream_t)stream) == hipSuccess ? CLMSOIL_OK
+                                                                                     : CLMSOIL_ERR_HIP;
 }
 
 // water_table! (soil_hydrology.jl:849-902).

@@ -136,6 +136,7 @@ enum DeviceError : unsigned {
     DEV_ERR_ITER = 4u,
     DEV_ERR_TAPE = 8u,
     DEV_ERR_SUBGRID = 16u,
+    DEV_ERR_JWT = 32u,
 };
 
 }  // namespace clmsoil

@@ -21,6 +21,7 @@ extern "C" {
 #define CLMSOIL_DEVERR_ITER     4u   /* substep iteration cap reached              */
 #define CLMSOIL_DEVERR_TAPE     8u   /* adjoint: more accepted substeps than the tape holds */
 #define CLMSOIL_DEVERR_SUBGRID 16u   /* ground heat flux: patch ranges and patch columns disagree */
+#define CLMSOIL_DEVERR_JWT     32u   /* methane transport: jwt outside 0..nlevsoi   */
 
 /* Synchronize `stream`, return and clear the current device's error bits. The
  * reference raises these conditions as Julia bounds errors inside its kernels. */

@@ -20,6 +20,7 @@
 #include "../include/clmsoil_surfhyd.h"
 #include "../include/clmsoil_lake.h"
 #include "../include/clmsoil_lvt.h"
+#include "../include/clmsoil_ch4.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -127,6 +128,14 @@ int orc_lake_temperature_solve(const clmsoil_lake_solve_args *a, const clmsoil_l
 /* Soil organic matter vertical transport, litter_vert_transp.jl:93-639 (lvt_oracle.c) */
 double orc_jl_pow5(double x);
 int orc_litter_vert_transp(const clmsoil_lvt_args *a, double dtime, const clmsoil_lvt_params *p);
+
+/* Methane and oxygen transport, methane.jl:101-129, 1663-2092 (ch4_oracle.c).
+ * orc_set_ch4_jitter moves every exp/log/pow result one ulp (seed 0: off);
+ * orc_ch4_debug sets the copy-outs of the next calls (see ch4_oracle.c). */
+int orc_ch4_tran(const clmsoil_ch4_tran_args *a, double dtime, const clmsoil_ch4_tran_params *p);
+void orc_set_ch4_jitter(uint64_t seed);
+void orc_ch4_debug(double *err_raw, double *rows, double *khcc);
+double orc_ch4_jl_pow(double x, double y);
 /* Test hook: a non-zero seed moves every pow() of the Zeng-Decker step one ulp
  * (direction from a hash of the operands and the seed); 0 restores exact pow(). */
 void orc_set_zd_pow_jitter(uint64_t seed);

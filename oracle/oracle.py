@@ -17,7 +17,7 @@ sys.path.insert(0, str(REPO / "clm.jl_amd"))
 
 from clmsoil import _abi  # noqa: E402
 from clmsoil.bind import (byref, fractions_args, ghf_args, heat_args, levels, mf_args,  # noqa: E402
-                          lake_args, lvt_args, lvt_params, rootsink_args, seb_args, surfhyd_args, surfhyd_params, theta_wt_args,
+                          ch4_tran_args, ch4_tran_params, lake_args, lvt_args, lvt_params, rootsink_args, seb_args, surfhyd_args, surfhyd_params, theta_wt_args,
                           water_table_args, zd_args)
 
 LIB_PATH = ORACLE_DIR / "liboracle_clmsoil.so"
@@ -221,6 +221,45 @@ def litter_vert_transp(d: dict, dtime: float, **params) -> None:
     L = lib()
     check(L.orc_litter_vert_transp(byref(lvt_args(d)), float(dtime), byref(lvt_params(**params))),
           "litter_vert_transp")
+
+
+def ch4_tran(d: dict, dtime: float, dbg: bool = False, **params):
+    """ch4_tran! on the host arrays of `d` (clmsoil_ch4_tran_args fields plus `nc`,
+    `ldg`); with `dbg` also the CH4 balance error before its correction (nc,), the
+    rows at, bt, ct, rt and the solved relative concentration before the clip
+    (nc, 2, nlevsoi+1, 5) and k_h_cc (nc, 2, nlevsoi+1)."""
+    import numpy as np
+    L = lib()
+    L.orc_ch4_debug.argtypes = [C.c_void_p] * 3
+    L.orc_ch4_debug.restype = None
+    nc, nl = int(d["nc"]), int(params["nlevsoi"])
+    out = None
+    if dbg:
+        out = dict(err_raw=np.full(nc, np.nan), rows=np.full((nc, 2, nl + 1, 5), np.nan),
+                   khcc=np.full((nc, 2, nl + 1), np.nan))
+        L.orc_ch4_debug(*(out[k].ctypes.data_as(C.c_void_p) for k in ("err_raw", "rows", "khcc")))
+    try:
+        check(L.orc_ch4_tran(byref(ch4_tran_args(d)), float(dtime), byref(ch4_tran_params(**params))),
+              "ch4_tran")
+    finally:
+        if dbg:
+            L.orc_ch4_debug(None, None, None)
+    return out
+
+
+def set_ch4_jitter(seed: int) -> None:
+    """Move every exp/log/pow of the methane transport one ulp (0 = exact)."""
+    L = lib()
+    L.orc_set_ch4_jitter.argtypes = [C.c_uint64]
+    L.orc_set_ch4_jitter.restype = None
+    L.orc_set_ch4_jitter(int(seed))
+
+
+def ch4_jl_pow(x: float, y: float) -> float:
+    L = lib()
+    L.orc_ch4_jl_pow.argtypes = [C.c_double, C.c_double]
+    L.orc_ch4_jl_pow.restype = C.c_double
+    return float(L.orc_ch4_jl_pow(float(x), float(y)))
 
 
 def jl_pow5(x: float) -> float:

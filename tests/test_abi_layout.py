@@ -30,6 +30,8 @@ STRUCTS = {
     "clmsoil_lake_solve_args": _abi.LakeSolveArgs,
     "clmsoil_lvt_args": _abi.LvtArgs,
     "clmsoil_lvt_params": _abi.LvtParams,
+    "clmsoil_ch4_tran_args": _abi.Ch4TranArgs,
+    "clmsoil_ch4_tran_params": _abi.Ch4TranParams,
 }
 
 
@@ -40,7 +42,7 @@ def c_layout(tmp_path_factory):
         pytest.skip("no C compiler")
     lines = ["#include <stdio.h>", "#include <stddef.h>",
              '#include "clmsoil.h"', '#include "clmsoil_zd09.h"', '#include "clmsoil_ghf.h"',
-             '#include "clmsoil_hydro.h"', '#include "clmsoil_surfhyd.h"', '#include "clmsoil_lake.h"', '#include "clmsoil_lvt.h"',
+             '#include "clmsoil_hydro.h"', '#include "clmsoil_surfhyd.h"', '#include "clmsoil_lake.h"', '#include "clmsoil_lvt.h"', '#include "clmsoil_ch4.h"',
              "int main(void) {"]
     for cname, py in STRUCTS.items():
         lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
