@@ -400,6 +400,41 @@ def lake_solve_step(nc, steps):
                          "bytes_per_launch": by}}
 
 
+def ch4_transport_step(nc, steps):
+    """Methane and oxygen transport (the unsaturated fraction of ch4!, 20 soil
+    levels, water tables at every level) on `nc` columns; `steps` launches, each
+    from the same state and bracketed by HIP events."""
+    import torch
+    from clmsoil import api, inputs
+    from clmsoil._abi import CH4_FIELDS
+    d = inputs.ch4_columns(nc=nc, seed=57)
+    t = api.to_device({k: d[k] for k in CH4_FIELDS})
+    st = t | {"nc": nc, "ldg": d["ldg"]}
+    t0 = {k: v.clone() for k, v in t.items() if hasattr(v, "clone")}
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps + 2)]
+    for e in ev:
+        for k in t0:
+            t[k].copy_(t0[k])
+        e[0].record()
+        api.ch4_tran(st, 1800.0, nlevsoi=d["nlevsoi"])
+        e[1].record()
+    torch.cuda.synchronize()
+    assert api.device_errors() == 0
+    ms = sum(e[0].elapsed_time(e[1]) for e in ev[2:]) / steps
+    on = int(d["mask"].sum())
+    nl = d["nlevsoi"]
+    # per filtered column and level: 8 forcing and 9 state arrays in, the 7 scaled
+    # rates and stresses and the 2 concentrations out; per filtered column: the
+    # gridcell and jwt indices, 4 surface values and the ground conductance in, c_atm
+    # (CH4 and O2), 5 column values out; plus the mask
+    by = nc + on * (nl * 8 * (17 + 9) + 8 * (2 + 5) + 16 + 8 * 5)
+    return {"workload": f"{nc} columns x {nl} levels, unsaturated fraction, jwt 0-{nl}",
+            "ms_per_launch": ms, "column_steps_per_s": nc / (ms / 1e3),
+            "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                         "bytes_per_launch": by}}
+
+
 def litter_transport_step(nc, steps):
     """Vertical transport of the soil organic matter pools (the CENTURY cascade's
     7 pools of carbon and nitrogen, 20 decomposition levels) on `nc` columns;
@@ -496,7 +531,7 @@ def main(argv=None):
     ap.add_argument("--hydro-only", action="store_true",
                     help="run only the soil hydrology sections (aquifer step, water tables "
                          "without an aquifer, surface hydrology, root uptake, lake solve, "
-                         "litter transport)")
+                         "litter transport, methane transport)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -535,7 +570,8 @@ def main(argv=None):
             "surface_hydrology": surface_hydrology_step(nc, args.steps),
             "root_sink": root_sink_step(nc, args.steps),
             "lake_solve": lake_solve_step(nc, args.steps),
-            "litter_transport": litter_transport_step(nc, args.steps)}), flush=True)
+            "litter_transport": litter_transport_step(nc, args.steps),
+            "ch4_transport": ch4_transport_step(nc, args.steps)}), flush=True)
         return 0
     host, dev, gen_s = make_state(nc, args.seed)
     print(f"# state ready: {nc} columns in {gen_s:.1f} s", file=sys.stderr, flush=True)
@@ -628,6 +664,7 @@ def main(argv=None):
         out["root_sink"] = root_sink_step(nc, args.steps)
         out["lake_solve"] = lake_solve_step(nc, args.steps)
         out["litter_transport"] = litter_transport_step(nc, args.steps)
+        out["ch4_transport"] = ch4_transport_step(nc, args.steps)
     if not args.no_cpu_baseline:
         v, cores, secs = cpu_baseline(host, min(args.cpu_sample, nc), args.cpu_steps)
         out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": cores,

@@ -76,22 +76,23 @@ struct Ch4Col {
         const double k_h_inv = exp(-C_H_INV[s] * (1.0 / t - 1.0 / 298.0) + logth);
         return t / k_h_inv * 0.0821;
     }
-    __device__ double vol_min(int j) const { return jl_min(at(a.watsat, j), at(a.h2osoi_vol, j)); }
-    __device__ double liqfrac(int j) const {
-        if (!p.ch4frzout) return 1.0;
-        const double liq = at(a.h2osoi_liq, j);
-        return jl_max(0.05, (liq / CH_DENH2O + CH_SN) / (liq / CH_DENH2O + at(a.h2osoi_ice, j) / CH_DENICE + CH_SN));
-    }
-    // epsilon_t (:1750-1757) with k the level's k_h_cc
-    __device__ double eps(int j, double k, double vmin, double lf) const {
-        return j <= jwt ? at(a.watsat, j) - (1.0 - k) * vmin * lf : at(a.watsat, j) * lf;
-    }
 };
 
 // One level of a species' column in the sweep: what its own row and its
 // neighbours' rows read.
 struct Ch4Lev {
     double dz, D, k, e, src, cw;
+};
+
+// The inputs of one level, loaded a step ahead of their use.
+struct Ch4In {                 // competition
+    double conc_o2, conc_ch4, o2_aere, prod, o2_decomp, o2_oxid, ch4_oxid, ch4_aere, ch4_ebul, dz;
+};
+struct Ch4Raw {                // a species' sweep; r1..r4 its source terms
+    double dz, t, ws, vol, liq, ice, bsw, corg, r1, r2, r3, r4, conc;
+};
+struct Ch4Fin {                // the clip / bounds pass
+    double t, ws, vol, liq, ice, dz, conc;
 };
 
 __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_args a,
@@ -117,13 +118,20 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
 
     // --- 1. competition for O2 and CH4 (:1671-1724), totals (:1735-1774) ---
     double ebul_total = 0.0, surf_aere = 0.0;
+    auto in_at = [&](int j) {
+        return Ch4In{L.at(a.conc_o2, j),         L.at(a.conc_ch4, j),       L.at(a.o2_aere_depth, j),
+                     L.at(a.ch4_prod_depth, j),  L.at(a.o2_decomp_depth, j), L.at(a.o2_oxid_depth, j),
+                     L.at(a.ch4_oxid_depth, j),  L.at(a.ch4_aere_depth, j), L.at(a.ch4_ebul_depth, j),
+                     L.at(a.dz, j)};
+    };
+    Ch4In qa = in_at(1);
     for (int j = 1; j <= nl; ++j) {
-        const double conc_o2 = L.at(a.conc_o2, j), conc_ch4 = L.at(a.conc_ch4, j);
-        const double o2_aere = L.at(a.o2_aere_depth, j), prod = L.at(a.ch4_prod_depth, j);
-        double o2_decomp = L.at(a.o2_decomp_depth, j), o2_oxid = L.at(a.o2_oxid_depth, j);
-        double ch4_oxid = L.at(a.ch4_oxid_depth, j), ch4_aere = L.at(a.ch4_aere_depth, j);
-        double ch4_ebul = L.at(a.ch4_ebul_depth, j);
-        const double dz = L.at(a.dz, j);
+        const Ch4In r = qa;
+        if (j < nl) qa = in_at(j + 1);
+        const double conc_o2 = r.conc_o2, conc_ch4 = r.conc_ch4, o2_aere = r.o2_aere, prod = r.prod;
+        double o2_decomp = r.o2_decomp, o2_oxid = r.o2_oxid, ch4_oxid = r.ch4_oxid, ch4_aere = r.ch4_aere;
+        double ch4_ebul = r.ch4_ebul;
+        const double dz = r.dz;
         const double o2demand = o2_decomp + o2_oxid;
         double o2s, ch4s;
         if (o2demand > 0.0)
@@ -200,54 +208,88 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
 
         // the level values of level j (:1741-1793 for epsilon, source and the
         // relative concentration; :1818-1840 for the diffusivity)
-        auto level = [&](int j) {
-            Ch4Lev v;
-            v.dz = L.at(a.dz, j);
-            const double t = L.at(a.t_soisno, j);
-            v.k = L.khcc(t, s, logth);
-            const double vmin = L.vol_min(j), lf = L.liqfrac(j);
-            v.e = L.eps(j, v.k, vmin, lf);
+        auto raw = [&](int j) {
+            Ch4Raw r{};
+            r.dz = L.at(a.dz, j);
+            r.t = L.at(a.t_soisno, j);
+            r.ws = L.at(a.watsat, j);
+            r.vol = L.at(a.h2osoi_vol, j);
+            if (p.ch4frzout || j > jwt) {
+                r.liq = L.at(a.h2osoi_liq, j);
+                r.ice = L.at(a.h2osoi_ice, j);
+            }
+            if (j <= jwt) {
+                r.bsw = L.at(a.bsw, j);
+                r.corg = L.at(a.cellorg, j);
+            }
             if (s == 0) {
-                v.src = L.at(a.ch4_prod_depth, j) - L.at(a.ch4_oxid_depth, j) - L.at(a.ch4_aere_depth, j) -
-                        L.at(a.ch4_ebul_depth, j);
+                r.r1 = L.at(a.ch4_prod_depth, j);
+                r.r2 = L.at(a.ch4_oxid_depth, j);
+                r.r3 = L.at(a.ch4_aere_depth, j);
+                r.r4 = L.at(a.ch4_ebul_depth, j);
+            } else {
+                r.r1 = L.at(a.o2_oxid_depth, j);
+                r.r2 = L.at(a.o2_decomp_depth, j);
+                r.r3 = L.at(a.o2_aere_depth, j);
+            }
+            r.conc = L.at(conc, j);
+            return r;
+        };
+        auto level = [&](const Ch4Raw &r, int j) {
+            Ch4Lev v;
+            v.dz = r.dz;
+            const double t = r.t;
+            v.k = L.khcc(t, s, logth);
+            const double vmin = jl_min(r.ws, r.vol);
+            const double lf = !p.ch4frzout ? 1.0
+                                           : jl_max(0.05, (r.liq / CH_DENH2O + CH_SN) /
+                                                              (r.liq / CH_DENH2O + r.ice / CH_DENICE + CH_SN));
+            v.e = j <= jwt ? r.ws - (1.0 - v.k) * vmin * lf : r.ws * lf;
+            if (s == 0) {
+                v.src = r.r1 - r.r2 - r.r3 - r.r4;
                 if (j == jwt) v.src += ebul_total / v.dz;
             } else {
-                v.src = -L.at(a.o2_oxid_depth, j) - L.at(a.o2_decomp_depth, j) + L.at(a.o2_aere_depth, j);
+                v.src = -r.r1 - r.r2 + r.r3;
             }
-            v.cw = L.at(conc, j) / v.e;
+            v.cw = r.conc / v.e;
             const double tsc = t - CH_TFRZ;
             double d;
             if (j <= jwt) {
-                const double ws = L.at(a.watsat, j);
+                const double ws = r.ws;
                 const double f_a = 1.0 - vmin / ws;
                 const double e = ws - vmin;
-                const double om_frac =
-                    p.organic_max > 0.0 ? jl_min(p.om_frac_sf * L.at(a.cellorg, j) / p.organic_max, 1.0) : 1.0;
+                const double om_frac = p.organic_max > 0.0 ? jl_min(p.om_frac_sf * r.corg / p.organic_max, 1.0) : 1.0;
                 d = (D_CON_G[s][0] + D_CON_G[s][1] * tsc) * 1.0e-4 *
-                    (om_frac * jl_pow(f_a, 10.0 / 3.0) / (ws * ws) +
-                     (1.0 - om_frac) * (e * e) * jl_pow(f_a, 3.0 / L.at(a.bsw, j))) *
+                    (om_frac * jl_pow(f_a, 10.0 / 3.0) / (ws * ws) + (1.0 - om_frac) * (e * e) * jl_pow(f_a, 3.0 / r.bsw)) *
                     p.scale_factor_gasdiff;
             } else {
-                const double e = L.at(a.watsat, j);
+                const double e = r.ws;
                 d = jl_pow(e, p.satpow) * (D_CON_W[s][0] + D_CON_W[s][1] * tsc + D_CON_W[s][2] * (tsc * tsc)) *
                     1.0e-9 * p.scale_factor_liqdiff;
-                if (t <= CH_TFRZ) {
-                    const double liq = L.at(a.h2osoi_liq, j);
-                    d *= (liq / CH_DENH2O + CH_SN) / (liq / CH_DENH2O + L.at(a.h2osoi_ice, j) / CH_DENICE + CH_SN);
-                }
+                if (t <= CH_TFRZ)
+                    d *= (r.liq / CH_DENH2O + CH_SN) / (r.liq / CH_DENH2O + r.ice / CH_DENICE + CH_SN);
             }
             v.D = jl_max(d, CH_SN);
             return v;
         };
 
         // ground node (jj = 1): k_h_cc at t_grnd, the atmosphere's concentration
-        Ch4Lev prev{0.0, 0.0, L.khcc(a.t_grnd[c], s, logth), 0.0, 0.0, catm};
-        Ch4Lev cur = level(1);
-        Ch4Lev nxt = nl >= 2 ? level(2) : Ch4Lev{};
+        Ch4Lev prev{0.0, 0.0, L.khcc(a.t_grnd[c], s, logth), 0.0, 0.0, catm}, cur{}, nxt{};
         // Thomas forward sweep from the ground row a = 0, b = 1, c = 0, r = c_atm (:1940-1951)
         double cp_prev = 0.0 / 1.0, dp_prev = catm / 1.0;
         s_cp[0] = cp_prev;
-        for (int j = 1; j <= nl; ++j) {
+        // iteration j forms level j + 1 and then the row of level j
+        Ch4Raw qr = raw(1);                                // inputs of the level after next
+        for (int j = 0; j <= nl; ++j) {
+            if (j < nl) {
+                const Ch4Raw r = qr;
+                if (j + 2 <= nl) qr = raw(j + 2);
+                nxt = level(r, j + 1);
+            }
+            if (j == 0) {
+                cur = nxt;
+                continue;
+            }
             // dm1_zm1 / dp1_zp1 (:1843-1871)
             double dm, dp = 0.0;
             if (j == 1 && j != jwt && j != jwt + 1) {
@@ -313,7 +355,6 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
             *L.ptr(conc, j) = dp_prev;                    // parked until the back substitution
             prev = cur;
             cur = nxt;
-            if (j + 2 <= nl) nxt = level(j + 2);
         }
         // back substitution (:1953-1957); the ground node's value is not used
         double u_next = dp_prev;
@@ -335,13 +376,33 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
             }
         }
         double deficit_tot = 0.0;
+        auto fin_at = [&](int j) {
+            Ch4Fin f{};
+            f.t = L.at(a.t_soisno, j);
+            f.ws = L.at(a.watsat, j);
+            f.vol = L.at(a.h2osoi_vol, j);
+            if (p.ch4frzout) {
+                f.liq = L.at(a.h2osoi_liq, j);
+                f.ice = L.at(a.h2osoi_ice, j);
+            }
+            if (s == 0) f.dz = L.at(a.dz, j);
+            f.conc = *L.ptr(conc, j);
+            return f;
+        };
+        Ch4Fin qf = fin_at(1);
         for (int j = 1; j <= nl; ++j) {
-            const double k = L.khcc(L.at(a.t_soisno, j), s, logth);
-            const double e = L.eps(j, k, L.vol_min(j), L.liqfrac(j));
-            double cw = *L.ptr(conc, j);
+            const Ch4Fin f = qf;
+            if (j < nl) qf = fin_at(j + 1);
+            const double k = L.khcc(f.t, s, logth);
+            const double vmin = jl_min(f.ws, f.vol);
+            const double lf = !p.ch4frzout ? 1.0
+                                           : jl_max(0.05, (f.liq / CH_DENH2O + CH_SN) /
+                                                              (f.liq / CH_DENH2O + f.ice / CH_DENICE + CH_SN));
+            const double e = j <= jwt ? f.ws - (1.0 - k) * vmin * lf : f.ws * lf;
+            double cw = f.conc;
             if (s == 0) {                                 // negative-mass clip (:1972-1981)
                 if (cw < 0.0) {
-                    deficit_tot += -cw * e * L.at(a.dz, j);
+                    deficit_tot += -cw * e * f.dz;
                     cw = 0.0;
                 }
             } else {                                      // O2 bounds (:1984-1988)
@@ -355,11 +416,18 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
 
     // CH4 balance and its correction (:1990-2001)
     double errch4 = 0.0;
+    double bq[4] = {L.at(a.dz, 1), L.at(a.conc_ch4, 1), L.at(a.ch4_prod_depth, 1), L.at(a.ch4_oxid_depth, 1)};
     for (int j = 1; j <= nl; ++j) {
-        const double dz = L.at(a.dz, j);
-        errch4 += (L.at(a.conc_ch4, j) - s_bef[(int64_t)(j - 1) * CH_BLOCK]) * dz;
-        errch4 -= L.at(a.ch4_prod_depth, j) * dz * dtime;
-        errch4 += L.at(a.ch4_oxid_depth, j) * dz * dtime;
+        const double dz = bq[0], cn = bq[1], pr = bq[2], ox = bq[3];
+        if (j < nl) {
+            bq[0] = L.at(a.dz, j + 1);
+            bq[1] = L.at(a.conc_ch4, j + 1);
+            bq[2] = L.at(a.ch4_prod_depth, j + 1);
+            bq[3] = L.at(a.ch4_oxid_depth, j + 1);
+        }
+        errch4 += (cn - s_bef[(int64_t)(j - 1) * CH_BLOCK]) * dz;
+        errch4 -= pr * dz * dtime;
+        errch4 += ox * dz * dtime;
     }
     errch4 += (surf_aere + surf_ebul + surf_diff) * dtime;
     if (fabs(errch4) < 1.0e-8) surf_diff -= errch4 / dtime;
