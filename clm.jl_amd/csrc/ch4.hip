@@ -13,8 +13,9 @@
 //      elimination consumes the row at once. Multipliers go to a lane-private LDS
 //      column ([level][lane]); right-hand sides are parked in the species'
 //      concentration slots, which the back substitution overwrites;
-//   3. per species, the CH4 clip or the O2 bounds and the absolute concentrations;
-//      then the CH4 balance against the concentrations held in LDS.
+//   3. per species, the back substitution bottom up, with the CH4 clip or the O2
+//      bounds and the absolute concentration of each level as it is solved; then
+//      the CH4 balance, whose concentration terms were kept in LDS.
 // The arithmetic is the reference's operation by operation; exp, log and pow are
 // the device library's.
 #include "ch4.hpp"
@@ -356,26 +357,13 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
             prev = cur;
             cur = nxt;
         }
-        // back substitution (:1953-1957); the ground node's value is not used
-        double u_next = dp_prev;
-        *L.ptr(conc, nl) = u_next;
-        for (int j = nl - 1; j >= 1; --j) {
-            u_next = *L.ptr(conc, j) - s_cp[(int64_t)j * CH_BLOCK] * u_next;
-            *L.ptr(conc, j) = u_next;
-        }
-
-        // --- 3. the clip or the bounds, and the absolute concentrations ---
-        if (s == 0) {
-            // surface fluxes (:1959-1966); u_next is the new relative CH4 of level 1
-            if (jwt != 0) {
-                surf_diff = dm1_first * ((u_next + cw1_old) / 2.0 - catm0);
-                surf_ebul = 0.0;
-            } else {
-                surf_diff = dm1_first * ((u_next + cw1_old) / 2.0 - catm0 * L.khcc(a.t_grnd[c], 0, logth0));
-                surf_ebul = ebul_total;
-            }
-        }
-        double deficit_tot = 0.0;
+        // --- 3. back substitution (:1953-1957) bottom up, with the CH4 clip or the
+        // O2 bounds and the absolute concentration of each level as it is solved
+        // (:1967-1996). The recursion carries the unclipped values, as the
+        // reference's solve finishes before its clip. A CH4 level's clipped deficit
+        // goes to its multiplier's LDS slot (used by then) and is summed top down
+        // below, in the reference's order; its balance term (new - old) * dz goes to
+        // the slot of its old concentration. The ground node's value is not used.
         auto fin_at = [&](int j) {
             Ch4Fin f{};
             f.t = L.at(a.t_soisno, j);
@@ -386,46 +374,66 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
                 f.ice = L.at(a.h2osoi_ice, j);
             }
             if (s == 0) f.dz = L.at(a.dz, j);
-            f.conc = *L.ptr(conc, j);
+            f.conc = *L.ptr(conc, j);                     // the parked right-hand side
             return f;
         };
-        Ch4Fin qf = fin_at(1);
-        for (int j = 1; j <= nl; ++j) {
+        double u_next = dp_prev;
+        Ch4Fin qf = fin_at(nl);
+        for (int j = nl; j >= 1; --j) {
             const Ch4Fin f = qf;
-            if (j < nl) qf = fin_at(j + 1);
+            if (j > 1) qf = fin_at(j - 1);
+            if (j < nl) u_next = f.conc - s_cp[(int64_t)j * CH_BLOCK] * u_next;
             const double k = L.khcc(f.t, s, logth);
             const double vmin = jl_min(f.ws, f.vol);
             const double lf = !p.ch4frzout ? 1.0
                                            : jl_max(0.05, (f.liq / CH_DENH2O + CH_SN) /
                                                               (f.liq / CH_DENH2O + f.ice / CH_DENICE + CH_SN));
             const double e = j <= jwt ? f.ws - (1.0 - k) * vmin * lf : f.ws * lf;
-            double cw = f.conc;
-            if (s == 0) {                                 // negative-mass clip (:1972-1981)
+            double cw = u_next;
+            if (s == 0) {                                 // negative-mass clip
+                double deficit = 0.0;
                 if (cw < 0.0) {
-                    deficit_tot += -cw * e * f.dz;
+                    deficit = -cw * e * f.dz;
                     cw = 0.0;
                 }
-            } else {                                      // O2 bounds (:1984-1988)
+                s_cp[(int64_t)j * CH_BLOCK] = deficit;
+                const double cnew = cw * e;
+                *L.ptr(conc, j) = cnew;
+                double *bef = s_bef + (int64_t)(j - 1) * CH_BLOCK;
+                *bef = (cnew - *bef) * f.dz;
+            } else {                                      // O2 bounds
                 cw = jl_max(cw, 1.0e-12);
                 cw = jl_min(cw, catm1 / e);
+                *L.ptr(conc, j) = cw * e;
             }
-            *L.ptr(conc, j) = cw * e;                     // (:1993-1996)
         }
-        if (s == 0) surf_diff -= deficit_tot / dtime;
+        if (s == 0) {
+            // surface fluxes (:1959-1966); u_next is the new relative CH4 of level 1
+            if (jwt != 0) {
+                surf_diff = dm1_first * ((u_next + cw1_old) / 2.0 - catm0);
+                surf_ebul = 0.0;
+            } else {
+                surf_diff = dm1_first * ((u_next + cw1_old) / 2.0 - catm0 * L.khcc(a.t_grnd[c], 0, logth0));
+                surf_ebul = ebul_total;
+            }
+            // the clipped deficit, levels in order (unclipped levels add +0.0)
+            double deficit_tot = 0.0;
+            for (int j = 1; j <= nl; ++j) deficit_tot += s_cp[(int64_t)j * CH_BLOCK];
+            surf_diff -= deficit_tot / dtime;
+        }
     }
 
     // CH4 balance and its correction (:1990-2001)
     double errch4 = 0.0;
-    double bq[4] = {L.at(a.dz, 1), L.at(a.conc_ch4, 1), L.at(a.ch4_prod_depth, 1), L.at(a.ch4_oxid_depth, 1)};
+    double bq[3] = {L.at(a.dz, 1), L.at(a.ch4_prod_depth, 1), L.at(a.ch4_oxid_depth, 1)};
     for (int j = 1; j <= nl; ++j) {
-        const double dz = bq[0], cn = bq[1], pr = bq[2], ox = bq[3];
+        const double dz = bq[0], pr = bq[1], ox = bq[2];
         if (j < nl) {
             bq[0] = L.at(a.dz, j + 1);
-            bq[1] = L.at(a.conc_ch4, j + 1);
-            bq[2] = L.at(a.ch4_prod_depth, j + 1);
-            bq[3] = L.at(a.ch4_oxid_depth, j + 1);
+            bq[1] = L.at(a.ch4_prod_depth, j + 1);
+            bq[2] = L.at(a.ch4_oxid_depth, j + 1);
         }
-        errch4 += (cn - s_bef[(int64_t)(j - 1) * CH_BLOCK]) * dz;
+        errch4 += s_bef[(int64_t)(j - 1) * CH_BLOCK];     // (new - old) * dz
         errch4 -= pr * dz * dtime;
         errch4 += ox * dz * dtime;
     }
