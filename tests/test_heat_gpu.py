@@ -6,7 +6,11 @@ for quantities the phase change forms as a difference of near-equal terms: a
 layer's ice and liquid (ice = wice0 - xm when a layer almost melts out) are
 compared relative to the layer's water mass wmass0, the melt/freeze fluxes
 relative to wmass0/dtime (their column sums to the largest layer's), and the
-latent-heat sums relative to hfus times that."""
+latent-heat sums relative to hfus times that. The surface water's freezing
+energy and ice flux are formed from tinc = TFRZ - t_h2osfc, which cancels to a
+few mK near freezing: they are compared relative to the same expression with
+TFRZ in place of tinc, frac_h2osfc * (|dhsdT| + c_h2osfc/dtime) * TFRZ (and that
+over hfus for the ice flux)."""
 import numpy as np
 import pytest
 
@@ -36,21 +40,25 @@ def run_gpu(d, method="Jordan1991"):
     return api.to_host(t)
 
 
-def scales(h0):
+def scales(h0, ref=None):
     """Per-element magnitude floors from the input state (see module doc)."""
     ns, dt = h0["nlevsno"], h0["dtime"]
+    extra = {}
+    if ref is not None and "c_h2osfc" in ref:
+        hm = h0["frac_h2osfc"] * (np.abs(h0["dhsdT"]) + np.nan_to_num(ref["c_h2osfc"]) / dt) * 273.15
+        extra = {"xmf_h2osfc": hm, "qflx_h2osfc_to_ice": hm / 3.337e5}
     m0 = np.nan_to_num(h0["h2osoi_ice"]) + np.nan_to_num(h0["h2osoi_liq"])
     flux = m0[:ns] / dt
     colflux = flux.max(axis=0)
     return {"h2osoi_ice": m0, "h2osoi_liq": m0, "qflx_snomelt_lyr": flux, "qflx_snofrz_lyr": flux,
             "qflx_snomelt": colflux, "qflx_snofrz": colflux, "qflx_snow_drain": colflux,
             "eflx_snomelt": 3.337e5 * colflux, "eflx_snomelt_r": 3.337e5 * colflux,
-            "xmf": np.maximum(3.337e5 * (m0 / dt).max(axis=0), 1.0e3)}
+            "xmf": np.maximum(3.337e5 * (m0 / dt).max(axis=0), 1.0e3)} | extra
 
 
 def compare(got, ref, keys=OUT, h0=None):
     bad = {}
-    sc = scales(h0) if h0 is not None else {}
+    sc = scales(h0, ref) if h0 is not None else {}
     for k in keys:
         if k not in ref:
             continue
@@ -77,6 +85,17 @@ def test_config1_vs_oracle(gpu, G, variant):
     ref = clone(h)
     oracle.soil_temperature(ref, bind.heat_config())
     api.set_tuning(4, 256, G, 256)
+    compare(run_gpu(clone(h)), ref, h0=h)
+
+
+@pytest.mark.gpu
+def test_config1_full_size_vs_oracle(gpu):
+    """The heat workload at its full size: 65,536 columns, 0-12 snow layers on 25
+    ground layers, every column against the oracle, at the production tuning."""
+    h = heat_case(65536, 37, True, nbedrock="random")
+    ref = clone(h)
+    oracle.soil_temperature(ref, bind.heat_config())
+    api.set_tuning(4, 256, 4, 256)
     compare(run_gpu(clone(h)), ref, h0=h)
 
 

@@ -132,3 +132,35 @@ def test_learned_first_substep_input(gpu):
     oracle.soilwm_moisture_form_solve(ref, cfg.to_abi())
     api.set_tuning(4, 256, 4, 256)
     compare(run_gpu(d, cfg), ref, d)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bc", [api.BC_ZERO_FLUX, api.BC_WATERTABLE])
+def test_water_balance_1m_columns(gpu, bc):
+    """The production day's size (2^20 columns, bedrock at levels 5-20), persistent
+    launch at the production tuning: every column's water change equals infiltration
+    minus the root sink minus the bottom flux over the step, to 1e-13 of its water
+    (a size-independent property of the moisture-form solve, which the oracle meets
+    to 1.3e-16), and a strided sample of 1,052 columns matches the oracle."""
+    d = inputs.richards_config0(nc=1 << 20, seed=60 + bc, nbedrock="random", lower_bc=bc)
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=bc)
+    api.set_tuning(4, 256, 4, 256)
+    api.set_richards_schedule(True)
+    got = run_gpu(clone(d), cfg)
+    ns, n = d["nlevsno"], d["nlevsoi"]
+    dliq = (got["h2osoi_liq"][ns:ns + n] - d["h2osoi_liq"][ns:ns + n]).sum(0)
+    expect = (d["qflx_infl"] - d["qflx_rootsoi"][:n].sum(0) - got["qcharge"]) * d["dtime"]
+    scale = np.abs(d["h2osoi_liq"][ns:ns + n]).sum(0)
+    assert np.all(np.isfinite(dliq))
+    assert np.max(np.abs(dliq - expect) / scale) < 1e-13
+    if bc == api.BC_ZERO_FLUX:
+        assert np.all(got["qcharge"] == 0.0)
+    idx = np.arange(0, 1 << 20, 997)
+    sub = {k: (v[..., idx].copy() if isinstance(v, np.ndarray) and v.shape[-1] == d["nc"] else v)
+           for k, v in d.items()}
+    sub["nc"] = idx.size
+    ref = clone(sub)
+    oracle.soilwm_moisture_form_solve(ref, cfg.to_abi())
+    compare({k: got[k][..., idx] for k in ("h2osoi_liq", "smp_l", "hk_l", "qcharge", "nsubsteps")},
+            ref, d)
