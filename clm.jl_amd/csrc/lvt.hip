@@ -147,12 +147,12 @@ __device__ __forceinline__ void lvt_row(const LvtGrid &g, int j, int nbr, double
 __global__ void __launch_bounds__(LV_BLOCK) lvt_transport_kernel(clmsoil_lvt_args a,
                                                                  clmsoil_lvt_params p,
                                                                  LvtPools pools, double dtime) {
-    extern __shared__ double lds[];                      // [nj][LV_BLOCK] multipliers
+    extern __shared__ double lds[];                      // [nd + 2][LV_BLOCK] multipliers
     const int lane = threadIdx.x;
     const int64_t c = (int64_t)blockIdx.x * LV_BLOCK + lane;
     const int s = pools.pool[blockIdx.y] + 1, t = blockIdx.z;
     if (c >= a.nc || (a.mask_bgc_soilc && !a.mask_bgc_soilc[c])) return;
-    const int nd = p.nlevdecomp, nj = nd + 2;
+    const int nd = p.nlevdecomp;
     const int64_t nbr64 = a.nbedrock[c];
     if (nbr64 < 1 || nbr64 > nd) return;                 // flagged by the coefficient kernel
     const int nbr = (int)nbr64;
@@ -164,7 +164,6 @@ __global__ void __launch_bounds__(LV_BLOCK) lvt_transport_kernel(clmsoil_lvt_arg
     const LvtGrid g{a.zisoi - 1, a.zsoi - 1, a.dzsoi_decomp - 1, nd};
     const double *dzd = g.dzd;
     double *cpl = lds + lane;                            // [jj - 1] at stride LV_BLOCK
-    (void)nj;
     if (p.is_cwd[s - 1]) {
         // CWD: no transport, the source is added in place (:296-300)
         for (int j = 1; j <= nd; ++j)
