@@ -110,3 +110,14 @@ def test_ch4_tran_bad_jwt_flags(gpu):
     ref["mask"][17] = 0
     oracle.ch4_tran(ref, 1800.0, nlevsoi=d["nlevsoi"])
     assert np.array_equal(got["ch4stress"], ref["ch4stress"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nl,sat", [(1, 0), (1, 1), (2, 0), (2, 1)])
+def test_ch4_tran_one_and_two_levels(gpu, nl, sat):
+    """The shortest columns: the level below the ground node is also the bottom
+    row, and jwt at 0, 1 or 2 takes every branch of the interface conductances."""
+    d = inputs.ch4_columns(nc=3001, seed=146 + 2 * nl + sat, nlevsoi=nl, sat=sat)
+    got = run_gpu(d, 1800.0, sat=sat)
+    ref, band = oracle_band(d, 1800.0, sat=sat)
+    compare(d, got, ref, band, 1800.0)
