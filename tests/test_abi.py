@@ -83,3 +83,43 @@ def test_api_config_validation():
                                       lower_boundary_condition=api.BC_ZERO_FLUX)
     assert not api.use_aquifer_layer(cfg)
     assert api.use_aquifer_layer(api.SoilWaterMovementConfig(lower_boundary_condition=api.BC_WATERTABLE))
+
+
+def test_transport_entries_validate_on_the_host():
+    """The litter and methane entries reject bad sizes, methods and missing arrays
+    before touching a device (an empty call returns at once)."""
+    import numpy as np
+    from clmsoil import inputs
+    from clmsoil._abi import load_library
+    lib = load_library()
+    ok = _abi.OK if hasattr(_abi, "OK") else 0
+    # litter
+    d = inputs.lvt_columns(nc=8, seed=1)
+    a = bind.lvt_args(d | {"nc": 8})
+    p = bind.lvt_params(d["nlevdecomp"], d["nlevdecomp_full"], d["npools"], d["ntypes"], d["is_cwd"])
+    assert lib.clmsoil_litter_vert_transp(C.byref(a), 0.0, C.byref(p), None) == _abi.ERR_ARG
+    pm = bind.lvt_params(d["nlevdecomp"], d["nlevdecomp_full"], d["npools"], d["ntypes"], d["is_cwd"],
+                         use_soil_matrixcn=True)
+    assert lib.clmsoil_litter_vert_transp(C.byref(a), 1800.0, C.byref(pm), None) == _abi.ERR_METHOD
+    pbig = bind.lvt_params(63, 63, d["npools"], d["ntypes"], d["is_cwd"])
+    assert lib.clmsoil_litter_vert_transp(C.byref(a), 1800.0, C.byref(pbig), None) == _abi.ERR_SIZE
+    a0 = bind.lvt_args(d | {"nc": 8})
+    a0.zisoi = None
+    assert lib.clmsoil_litter_vert_transp(C.byref(a0), 1800.0, C.byref(p), None) == _abi.ERR_NULL
+    ae = bind.lvt_args(d | {"nc": 0, "ld": 8})
+    assert lib.clmsoil_litter_vert_transp(C.byref(ae), 1800.0, C.byref(p), None) == ok
+    # methane
+    m = inputs.ch4_columns(nc=8, seed=2)
+    b = bind.ch4_tran_args(m)
+    q = bind.ch4_tran_params(m["nlevsoi"])
+    assert lib.clmsoil_ch4_tran(C.byref(b), -1.0, C.byref(q), None) == _abi.ERR_ARG
+    assert lib.clmsoil_ch4_tran(C.byref(b), 1800.0, C.byref(bind.ch4_tran_params(61)), None) == _abi.ERR_SIZE
+    assert lib.clmsoil_ch4_tran(C.byref(b), 1800.0, C.byref(bind.ch4_tran_params(m["nlevsoi"], sat=2)),
+                                None) == _abi.ERR_ARG
+    b0 = bind.ch4_tran_args(m)
+    b0.conc_o2 = None
+    assert lib.clmsoil_ch4_tran(C.byref(b0), 1800.0, C.byref(q), None) == _abi.ERR_NULL
+    bl = bind.ch4_tran_args(m | {"ld": 4})
+    assert lib.clmsoil_ch4_tran(C.byref(bl), 1800.0, C.byref(q), None) == _abi.ERR_SIZE
+    be = bind.ch4_tran_args(m | {"nc": 0, "ld": 8})
+    assert lib.clmsoil_ch4_tran(C.byref(be), 1800.0, C.byref(q), None) == ok
