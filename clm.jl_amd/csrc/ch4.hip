@@ -171,8 +171,8 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
             ch4_aere -= p.aereoxid * ch4_aere;
         }
         surf_aere += ch4_aere * dz;
-        *L.ptr(a.o2stress, j) = o2s;
-        *L.ptr(a.ch4stress, j) = ch4s;
+        __builtin_nontemporal_store(o2s, L.ptr(a.o2stress, j));      // final: not read again
+        __builtin_nontemporal_store(ch4s, L.ptr(a.ch4stress, j));
         *L.ptr(a.o2_decomp_depth, j) = o2_decomp;
         *L.ptr(a.o2_oxid_depth, j) = o2_oxid;
         *L.ptr(a.ch4_oxid_depth, j) = ch4_oxid;
@@ -398,13 +398,13 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
                 }
                 s_cp[(int64_t)j * CH_BLOCK] = deficit;
                 const double cnew = cw * e;
-                *L.ptr(conc, j) = cnew;
+                __builtin_nontemporal_store(cnew, L.ptr(conc, j));
                 double *bef = s_bef + (int64_t)(j - 1) * CH_BLOCK;
                 *bef = (cnew - *bef) * f.dz;
             } else {                                      // O2 bounds
                 cw = jl_max(cw, 1.0e-12);
                 cw = jl_min(cw, catm1 / e);
-                *L.ptr(conc, j) = cw * e;
+                __builtin_nontemporal_store(cw * e, L.ptr(conc, j));
             }
         }
         if (s == 0) {

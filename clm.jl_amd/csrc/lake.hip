@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(LK_BLOCK) lake_solve_kernel(clmsoil_lake_solve
         else
             t = L.ptr(a.t_soisno, je - nlevlak + nlevsno);
         u_next = *t - s_cp[(je - jtop) * LK_BLOCK + lane] * u_next;
-        *t = u_next;
+        __builtin_nontemporal_store(u_next, t);          // final: not read again
     }
 }
 

@@ -283,9 +283,11 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t lv_rows(const double *p) {
 __device__ __forceinline__ double lv_load(__amdgpu_buffer_rsrc_t r, uint32_t boff, uint32_t roff) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, boff, roff, 0));
 }
+// Stores of final values (nothing reads them again in the launch) are
+// non-temporal (cache policy bit 1).
 __device__ __forceinline__ void lv_store(__amdgpu_buffer_rsrc_t r, uint32_t boff, uint32_t roff,
                                          double v) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(lv_u32x2, v), r, boff, roff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(lv_u32x2, v), r, boff, roff, 2);
 }
 
 // _lvt_column_kernel! (:142-471) for the nfast pools packed four bits each in
