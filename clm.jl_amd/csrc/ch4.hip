@@ -6,13 +6,15 @@
 // walks its column three times:
 //   1. the O2/CH4 competition, the scaled process rates, the ebullition and
 //      aerenchyma totals, level by level (each sum in the reference's level order);
+//   (then, O2 before CH4, as the two are independent after step 1:)
 //   2. per species, the levels top to bottom with a one-level lookahead: the
 //      level's Henry partition, epsilon, source, diffusivity and relative
 //      concentration are formed once, the interface conductances and the
 //      Crank-Nicolson row from the three levels around it, and the Thomas forward
 //      elimination consumes the row at once. Multipliers go to a lane-private LDS
-//      column ([level][lane]); right-hand sides are parked in the species'
-//      concentration slots, which the back substitution overwrites;
+//      column ([level][lane]); O2's right-hand sides are parked in its concentration
+//      slots and its epsilon in a second LDS column, CH4's right-hand sides in that
+//      column, so that CH4's old concentrations stay in memory for its balance;
 //   3. per species, the back substitution bottom up, with the CH4 clip or the O2
 //      bounds and the absolute concentration of each level as it is solved; then
 //      the CH4 balance, whose concentration terms were kept in LDS.
@@ -112,7 +114,9 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
     const Ch4Col L{a, p, c, (int)jw};
     const int jwt = (int)jw;
     double *s_cp = lds + lane;                            // [jj - 1] at stride CH_BLOCK
-    double *s_bef = lds + (int64_t)(nl + 1) * CH_BLOCK + lane;   // [j - 1]
+    // [j - 1]: O2's epsilon during its pass, then CH4's parked right-hand side and,
+    // once that is used, its balance term (new - old) * dz
+    double *s_aux = lds + (int64_t)(nl + 1) * CH_BLOCK + lane;
     const int64_t g = a.col_gridcell[c] - 1;
     const double catm0 = a.c_atm[g], catm1 = a.c_atm[g + a.ldg];
     const double logth0 = log(KH_THETA[0]), logth1 = log(KH_THETA[1]);
@@ -178,18 +182,22 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
         *L.ptr(a.ch4_oxid_depth, j) = ch4_oxid;
         *L.ptr(a.ch4_aere_depth, j) = ch4_aere;
         *L.ptr(a.ch4_ebul_depth, j) = ch4_ebul;
-        s_bef[(int64_t)(j - 1) * CH_BLOCK] = conc_ch4;
     }
     a.ch4_ebul_total[c] = ebul_total;
     a.ch4_surf_aere[c] = surf_aere;
 
+    // the ground conductance floor (:1800-1802), which the reference applies before
+    // CH4 and O2 then sees too
     double gcond = a.grnd_ch4_cond[c];
+    if (gcond < CH_SN) gcond = CH_SN;
     double spec0 = 0.0, dm1_first = 0.0, cw1_old = 0.0, surf_diff = 0.0, surf_ebul = 0.0;
-    for (int s = 0; s < 2; ++s) {
+    // O2 first (the species are independent from here on): CH4's old concentrations
+    // stay in memory for its balance while its right-hand sides are parked in LDS
+    for (int si = 0; si < 2; ++si) {
+        const int s = 1 - si;
         double *conc = s == 0 ? a.conc_ch4 : a.conc_o2;
         const double catm = s == 0 ? catm0 : catm1, logth = s == 0 ? logth0 : logth1;
         // --- 2. snow/pond resistance and ground conductance (:1800-1815) ---
-        if (gcond < CH_SN && s == 0) gcond = CH_SN;
         double pondres = 0.0;
         const double frac = a.frac_h2osfc[c];
         if (!p.lake && p.sat == 1 && frac > 0.0) {
@@ -353,28 +361,36 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
             dp_prev = (rv - av * dp_prev) / denom;
             s_cp[(int64_t)j * CH_BLOCK] = cp_prev;
             if (s == 0 && j == 1) cw1_old = cur.cw;
-            *L.ptr(conc, j) = dp_prev;                    // parked until the back substitution
+            if (s == 1) {                                 // parked until the back substitution
+                *L.ptr(conc, j) = dp_prev;
+                s_aux[(int64_t)(j - 1) * CH_BLOCK] = cur.e;
+            } else {
+                s_aux[(int64_t)(j - 1) * CH_BLOCK] = dp_prev;
+            }
             prev = cur;
             cur = nxt;
         }
         // --- 3. back substitution (:1953-1957) bottom up, with the CH4 clip or the
         // O2 bounds and the absolute concentration of each level as it is solved
         // (:1967-1996). The recursion carries the unclipped values, as the
-        // reference's solve finishes before its clip. A CH4 level's clipped deficit
-        // goes to its multiplier's LDS slot (used by then) and is summed top down
-        // below, in the reference's order; its balance term (new - old) * dz goes to
-        // the slot of its old concentration. The ground node's value is not used.
+        // reference's solve finishes before its clip. O2 takes its epsilon from LDS;
+        // CH4 forms it again. A CH4 level's clipped deficit goes to its multiplier's
+        // LDS slot (used by then) and is summed top down below, in the reference's
+        // order; its balance term goes to the slot of its right-hand side. The ground
+        // node's value is not used.
         auto fin_at = [&](int j) {
             Ch4Fin f{};
-            f.t = L.at(a.t_soisno, j);
-            f.ws = L.at(a.watsat, j);
-            f.vol = L.at(a.h2osoi_vol, j);
-            if (p.ch4frzout) {
-                f.liq = L.at(a.h2osoi_liq, j);
-                f.ice = L.at(a.h2osoi_ice, j);
+            if (s == 0) {
+                f.t = L.at(a.t_soisno, j);
+                f.ws = L.at(a.watsat, j);
+                f.vol = L.at(a.h2osoi_vol, j);
+                if (p.ch4frzout) {
+                    f.liq = L.at(a.h2osoi_liq, j);
+                    f.ice = L.at(a.h2osoi_ice, j);
+                }
+                f.dz = L.at(a.dz, j);
             }
-            if (s == 0) f.dz = L.at(a.dz, j);
-            f.conc = *L.ptr(conc, j);                     // the parked right-hand side
+            f.conc = *L.ptr(conc, j);     // O2: the parked right-hand side; CH4: the old value
             return f;
         };
         double u_next = dp_prev;
@@ -382,16 +398,17 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
         for (int j = nl; j >= 1; --j) {
             const Ch4Fin f = qf;
             if (j > 1) qf = fin_at(j - 1);
-            if (j < nl) u_next = f.conc - s_cp[(int64_t)j * CH_BLOCK] * u_next;
-            const double k = L.khcc(f.t, s, logth);
-            const double vmin = jl_min(f.ws, f.vol);
-            const double lf = !p.ch4frzout ? 1.0
-                                           : jl_max(0.05, (f.liq / CH_DENH2O + CH_SN) /
-                                                              (f.liq / CH_DENH2O + f.ice / CH_DENICE + CH_SN));
-            const double e = j <= jwt ? f.ws - (1.0 - k) * vmin * lf : f.ws * lf;
+            double *aux = s_aux + (int64_t)(j - 1) * CH_BLOCK;
+            if (j < nl) u_next = (s == 1 ? f.conc : *aux) - s_cp[(int64_t)j * CH_BLOCK] * u_next;
             double cw = u_next;
-            if (s == 0) {                                 // negative-mass clip
-                double deficit = 0.0;
+            if (s == 0) {
+                const double k = L.khcc(f.t, s, logth);
+                const double vmin = jl_min(f.ws, f.vol);
+                const double lf = !p.ch4frzout ? 1.0
+                                               : jl_max(0.05, (f.liq / CH_DENH2O + CH_SN) /
+                                                                  (f.liq / CH_DENH2O + f.ice / CH_DENICE + CH_SN));
+                const double e = j <= jwt ? f.ws - (1.0 - k) * vmin * lf : f.ws * lf;
+                double deficit = 0.0;                     // negative-mass clip
                 if (cw < 0.0) {
                     deficit = -cw * e * f.dz;
                     cw = 0.0;
@@ -399,9 +416,9 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
                 s_cp[(int64_t)j * CH_BLOCK] = deficit;
                 const double cnew = cw * e;
                 __builtin_nontemporal_store(cnew, L.ptr(conc, j));
-                double *bef = s_bef + (int64_t)(j - 1) * CH_BLOCK;
-                *bef = (cnew - *bef) * f.dz;
+                *aux = (cnew - f.conc) * f.dz;
             } else {                                      // O2 bounds
+                const double e = *aux;
                 cw = jl_max(cw, 1.0e-12);
                 cw = jl_min(cw, catm1 / e);
                 __builtin_nontemporal_store(cw * e, L.ptr(conc, j));
@@ -433,7 +450,7 @@ __global__ void __launch_bounds__(CH_BLOCK) ch4_tran_kernel(clmsoil_ch4_tran_arg
             bq[1] = L.at(a.ch4_prod_depth, j + 1);
             bq[2] = L.at(a.ch4_oxid_depth, j + 1);
         }
-        errch4 += s_bef[(int64_t)(j - 1) * CH_BLOCK];     // (new - old) * dz
+        errch4 += s_aux[(int64_t)(j - 1) * CH_BLOCK];     // (new - old) * dz
         errch4 -= pr * dz * dtime;
         errch4 += ox * dz * dtime;
     }
