@@ -365,7 +365,7 @@ def root_sink_step(nc, steps):
                          "bytes_per_launch": by}}
 
 
-def lake_solve_step(nc, steps):
+def lake_solve_step(nc, steps, cpu_sample=0):
     """The lake temperature solve (assembly, tridiagonal solve and unpack in one
     launch) on `nc` lake columns: 0-6 snow layers, 10 lake levels, 25 soil
     levels; `steps` launches on the same inputs, each bracketed by HIP events."""
@@ -393,14 +393,58 @@ def lake_solve_step(nc, steps):
     # sabg_lyr_col and T in, T out; per soil level z, cv, tk and T in, T out; per
     # lake level z, dz, cv, tk, phi and T in, T out; plus the mask of every column
     by = nc + int(on.sum()) * (4 * 8 + 25 * 5 * 8 + 10 * 7 * 8) + int(ns.sum()) * 6 * 8
-    return {"workload": f"{nc} lake columns: 0-6 snow, 10 lake, 25 soil levels",
-            "ms_per_launch": ms, "columns_per_s": nc / (ms / 1e3),
-            "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
-                         "bytes_per_launch": by}}
+    out = {"workload": f"{nc} lake columns: 0-6 snow, 10 lake, 25 soil levels",
+           "ms_per_launch": ms, "columns_per_s": nc / (ms / 1e3),
+           "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                        "bytes_per_launch": by}}
+    if cpu_sample:
+        import oracle
+        out["cpu_baseline"] = _oracle_baseline(
+            d, nc, cpu_sample, lambda s: oracle.lake_temperature_solve(s, 1800.0))
+    return out
 
 
-def ch4_transport_step(nc, steps):
+def _first_columns(d, nc, m):
+    """The first m columns of a section's host inputs: arrays whose last axis is the
+    column axis (and lists of them) are cut, everything else is kept."""
+    import numpy as np
+
+    def cut(v):
+        if isinstance(v, np.ndarray) and v.ndim >= 1 and v.shape[-1] == nc:
+            return np.ascontiguousarray(v[..., :m])
+        if isinstance(v, list) and v and isinstance(v[0], np.ndarray):
+            return [cut(x) for x in v]
+        return v
+    out = {k: cut(v) for k, v in d.items()}
+    out["nc"] = m
+    if "ld" in out:
+        out["ld"] = m
+    return out
+
+
+def _oracle_baseline(d, nc, m, run):
+    """The oracle (one thread) timed on one step of the first m columns, best of 2."""
+    import numpy as np
+    sys.path.insert(0, str(REPO / "oracle"))
+    sub = _first_columns(d, nc, min(m, nc))
+
+    def fresh():
+        return {k: (v.copy() if isinstance(v, np.ndarray)
+                    else [x.copy() if isinstance(x, np.ndarray) else x for x in v]
+                    if isinstance(v, list) else v)
+                for k, v in sub.items()}
+    best = float("inf")
+    for _ in range(2):
+        s = fresh()
+        t0 = time.perf_counter()
+        run(s)
+        best = min(best, time.perf_counter() - t0)
+    return {"value": sub["nc"] / best, "unit": "column-steps/s", "cores": 1, "kind": "port",
+            "sample": f"first {sub['nc']} columns, one step, best of 2 ({best:.2f} s)"}
+
+
+def ch4_transport_step(nc, steps, cpu_sample=0):
     """Methane and oxygen transport (the unsaturated fraction of ch4!, 20 soil
     levels, water tables at every level) on `nc` columns; `steps` launches, each
     from the same state and bracketed by HIP events."""
@@ -428,14 +472,19 @@ def ch4_transport_step(nc, steps):
     # gridcell and jwt indices, 4 surface values and the ground conductance in, c_atm
     # (CH4 and O2), 5 column values out; plus the mask
     by = nc + on * (nl * 8 * (17 + 9) + 8 * (2 + 5) + 16 + 8 * 5)
-    return {"workload": f"{nc} columns x {nl} levels, unsaturated fraction, jwt 0-{nl}",
-            "ms_per_launch": ms, "column_steps_per_s": nc / (ms / 1e3),
-            "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
-                         "bytes_per_launch": by}}
+    out = {"workload": f"{nc} columns x {nl} levels, unsaturated fraction, jwt 0-{nl}",
+           "ms_per_launch": ms, "column_steps_per_s": nc / (ms / 1e3),
+           "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                        "bytes_per_launch": by}}
+    if cpu_sample:
+        import oracle
+        out["cpu_baseline"] = _oracle_baseline(
+            d, nc, cpu_sample, lambda s: oracle.ch4_tran(s, 1800.0, nlevsoi=nl))
+    return out
 
 
-def litter_transport_step(nc, steps):
+def litter_transport_step(nc, steps, cpu_sample=0):
     """Vertical transport of the soil organic matter pools (the CENTURY cascade's
     7 pools of carbon and nitrogen, 20 decomposition levels) on `nc` columns;
     `steps` launches on the same inputs, each bracketed by HIP events."""
@@ -469,11 +518,16 @@ def litter_transport_step(nc, steps):
     ncwd = int(sum(bool(x) for x in d["is_cwd"]))
     by = (nc + on * (3 * 8 + 2 * 2 * 8 * (nd + 1))
           + on * nt * nd * 8 * (4 * (npools - ncwd) + 3 * ncwd))
-    return {"workload": f"{nc} columns x {npools} pools x {nt} tracers x {nd} levels",
-            "ms_per_launch": ms, "column_steps_per_s": nc / (ms / 1e3),
-            "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
-                         "bytes_per_launch": by}}
+    out = {"workload": f"{nc} columns x {npools} pools x {nt} tracers x {nd} levels",
+           "ms_per_launch": ms, "column_steps_per_s": nc / (ms / 1e3),
+           "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                        "bytes_per_launch": by}}
+    if cpu_sample:
+        import oracle
+        out["cpu_baseline"] = _oracle_baseline(
+            d, nc, cpu_sample, lambda s: oracle.litter_vert_transp(s, 1800.0, **kw))
+    return out
 
 
 def water_tables_moisture_form(nc, steps):
@@ -556,6 +610,8 @@ def main(argv=None):
         print(json.dumps({"copy_bytes_each_way": n * 8, "copies": 4}), flush=True)
         return 0
     nc = args.nc
+    # the oracle is timed beside each hydrology section on its first 65,536 columns
+    sec_cpu = 0 if args.no_cpu_baseline else 65536
     if args.surface_only:
         print(json.dumps({"surface_energy": surface_energy(nc, args.steps)}), flush=True)
         return 0
@@ -569,9 +625,9 @@ def main(argv=None):
             "water_tables_moisture_form": water_tables_moisture_form(nc, args.steps),
             "surface_hydrology": surface_hydrology_step(nc, args.steps),
             "root_sink": root_sink_step(nc, args.steps),
-            "lake_solve": lake_solve_step(nc, args.steps),
-            "litter_transport": litter_transport_step(nc, args.steps),
-            "ch4_transport": ch4_transport_step(nc, args.steps)}), flush=True)
+            "lake_solve": lake_solve_step(nc, args.steps, sec_cpu),
+            "litter_transport": litter_transport_step(nc, args.steps, sec_cpu),
+            "ch4_transport": ch4_transport_step(nc, args.steps, sec_cpu)}), flush=True)
         return 0
     host, dev, gen_s = make_state(nc, args.seed)
     print(f"# state ready: {nc} columns in {gen_s:.1f} s", file=sys.stderr, flush=True)
@@ -662,9 +718,9 @@ def main(argv=None):
         out["water_tables_moisture_form"] = water_tables_moisture_form(nc, args.steps)
         out["surface_hydrology"] = surface_hydrology_step(nc, args.steps)
         out["root_sink"] = root_sink_step(nc, args.steps)
-        out["lake_solve"] = lake_solve_step(nc, args.steps)
-        out["litter_transport"] = litter_transport_step(nc, args.steps)
-        out["ch4_transport"] = ch4_transport_step(nc, args.steps)
+        out["lake_solve"] = lake_solve_step(nc, args.steps, sec_cpu)
+        out["litter_transport"] = litter_transport_step(nc, args.steps, sec_cpu)
+        out["ch4_transport"] = ch4_transport_step(nc, args.steps, sec_cpu)
     if not args.no_cpu_baseline:
         v, cores, secs = cpu_baseline(host, min(args.cpu_sample, nc), args.cpu_steps)
         out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": cores,

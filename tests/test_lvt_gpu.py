@@ -143,3 +143,17 @@ def test_lvt_padded_leading_dimension(gpu):
         assert np.array_equal(got["conc"][t][:, :, 611:], d["conc"][t][:, :, 611:]), t
         assert np.array_equal(got["conc"][t], ref["conc"][t]), t
         assert np.array_equal(got["tendency"][t], ref["tendency"][t]), t
+
+
+@pytest.mark.gpu
+def test_lvt_sixteen_pools_three_tracers(gpu):
+    """The most pools the interface takes (16, two of them coarse woody debris) and
+    three tracer types: 48 systems per column, so each of the shared-matrix kernel's
+    8 wavefronts solves six; bit-exact."""
+    d = inputs.lvt_columns(nc=777, seed=125, npools=16, ntypes=3, cwd_pools=(3, 11))
+    kw = dict(som_diffus=1e-8, cryoturb_diffusion_k=5e-8, som_adv_flux=3e-10)
+    got = run_gpu(d, 3.0e7, **kw)
+    ref = run_oracle(d, 3.0e7, **kw)
+    for t in range(3):
+        assert np.array_equal(got["conc"][t], ref["conc"][t]), t
+        assert np.array_equal(got["tendency"][t], ref["tendency"][t]), t
