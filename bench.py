@@ -303,7 +303,7 @@ SURFHYD_READ_BYTES = 3 * 1 + 6 * 8 + 1 + 25 * 8
 SURFHYD_WRITE_BYTES = 17 * 8
 
 
-def surface_hydrology_step(nc, steps):
+def surface_hydrology_step(nc, steps, cpu_sample=0):
     """The surface hydrology before the Richards step (flood water, saturated
     and infiltration excess runoff, surface water, infiltration, total runoff,
     urban ponding) in one launch on `nc` columns of mixed landunits (55 % soil,
@@ -327,14 +327,19 @@ def surface_hydrology_step(nc, steps):
     torch.cuda.synchronize()
     ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
     by = (SURFHYD_READ_BYTES + SURFHYD_WRITE_BYTES) * nc
-    return {"workload": f"{nc} columns, mixed soil/crop/urban/lake/wetland landunits",
-            "ms_per_launch": ms, "columns_per_s": nc / (ms / 1e3),
-            "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
-                         "bytes_per_column": SURFHYD_READ_BYTES + SURFHYD_WRITE_BYTES}}
+    out = {"workload": f"{nc} columns, mixed soil/crop/urban/lake/wetland landunits",
+           "ms_per_launch": ms, "columns_per_s": nc / (ms / 1e3),
+           "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                        "bytes_per_column": SURFHYD_READ_BYTES + SURFHYD_WRITE_BYTES}}
+    if cpu_sample:
+        import oracle
+        out["cpu_baseline"] = _oracle_baseline(
+            d, nc, cpu_sample, lambda s: oracle.surface_hydrology(s, 1800.0))
+    return out
 
 
-def root_sink_step(nc, steps):
+def root_sink_step(nc, steps, cpu_sample=0):
     """Root water uptake (default method) on `nc` columns of 0-12 patches each
     (CLM's patch layout); `steps` launches, each bracketed by HIP events."""
     import torch
@@ -358,11 +363,17 @@ def root_sink_step(nc, steps):
     # qflx_rootsoi_col out over the soil layers. Per patch: active, wtcol,
     # transpiration and the root profile over the soil layers in.
     by = nc * (1 + 3 * 8 + 2 * 8 * n) + d["np"] * (1 + 2 * 8 + 8 * n)
-    return {"workload": f"{nc} columns, {d['np']} patches (0-12 per column)",
-            "ms_per_launch": ms, "columns_per_s": nc / (ms / 1e3),
-            "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
-                         "bytes_per_launch": by}}
+    out = {"workload": f"{nc} columns, {d['np']} patches (0-12 per column)",
+           "ms_per_launch": ms, "columns_per_s": nc / (ms / 1e3),
+           "roofline": {"bound": "hbm", "achieved": by / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": by / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                        "bytes_per_launch": by}}
+    if cpu_sample:
+        import oracle
+        # the patch arrays stay whole: the first columns index their own patches
+        out["cpu_baseline"] = _oracle_baseline(
+            d, nc, cpu_sample, lambda s: oracle.compute_effec_rootfrac_and_vert_tran_sink(s))
+    return out
 
 
 def lake_solve_step(nc, steps, cpu_sample=0):
@@ -530,7 +541,7 @@ def litter_transport_step(nc, steps, cpu_sample=0):
     return out
 
 
-def water_tables_moisture_form(nc, steps):
+def water_tables_moisture_form(nc, steps, cpu_sample=0):
     """The water tables after a step without an aquifer layer (perched,
     theta-based, bedrock clamp in one launch) on `nc` columns of 20 soil layers
     with frozen layers under unfrozen ones on 30 % of the columns and the 90 %
@@ -556,10 +567,14 @@ def water_tables_moisture_form(nc, steps):
     torch.cuda.synchronize()
     assert api.device_errors() == 0
     ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
-    return {"workload": f"{nc} columns, 20 soil layers, nbedrock 3-20, 30% frozen columns",
-            "ms_per_launch": ms, "columns_per_s": nc / (ms / 1e3),
-            "note": "per-column searches up and down the saturation and temperature "
-                    "profiles: the bytes read depend on the state"}
+    out = {"workload": f"{nc} columns, 20 soil layers, nbedrock 3-20, 30% frozen columns",
+           "ms_per_launch": ms, "columns_per_s": nc / (ms / 1e3),
+           "note": "per-column searches up and down the saturation and temperature "
+                   "profiles: the bytes read depend on the state"}
+    if cpu_sample:
+        import oracle
+        out["cpu_baseline"] = _oracle_baseline(d, nc, cpu_sample, oracle.water_table_moisture_form)
+    return out
 
 
 def main(argv=None):
@@ -622,9 +637,9 @@ def main(argv=None):
     if args.hydro_only:
         print(json.dumps({"zengdecker_2009": zengdecker_2009(
             nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample),
-            "water_tables_moisture_form": water_tables_moisture_form(nc, args.steps),
-            "surface_hydrology": surface_hydrology_step(nc, args.steps),
-            "root_sink": root_sink_step(nc, args.steps),
+            "water_tables_moisture_form": water_tables_moisture_form(nc, args.steps, sec_cpu),
+            "surface_hydrology": surface_hydrology_step(nc, args.steps, sec_cpu),
+            "root_sink": root_sink_step(nc, args.steps, sec_cpu),
             "lake_solve": lake_solve_step(nc, args.steps, sec_cpu),
             "litter_transport": litter_transport_step(nc, args.steps, sec_cpu),
             "ch4_transport": ch4_transport_step(nc, args.steps, sec_cpu)}), flush=True)
@@ -715,9 +730,9 @@ def main(argv=None):
     if not args.no_zd:
         out["zengdecker_2009"] = zengdecker_2009(
             nc, args.steps, 0 if args.no_cpu_baseline else args.cpu_sample)
-        out["water_tables_moisture_form"] = water_tables_moisture_form(nc, args.steps)
-        out["surface_hydrology"] = surface_hydrology_step(nc, args.steps)
-        out["root_sink"] = root_sink_step(nc, args.steps)
+        out["water_tables_moisture_form"] = water_tables_moisture_form(nc, args.steps, sec_cpu)
+        out["surface_hydrology"] = surface_hydrology_step(nc, args.steps, sec_cpu)
+        out["root_sink"] = root_sink_step(nc, args.steps, sec_cpu)
         out["lake_solve"] = lake_solve_step(nc, args.steps, sec_cpu)
         out["litter_transport"] = litter_transport_step(nc, args.steps, sec_cpu)
         out["ch4_transport"] = ch4_transport_step(nc, args.steps, sec_cpu)
