@@ -121,3 +121,15 @@ def test_ch4_tran_one_and_two_levels(gpu, nl, sat):
     got = run_gpu(d, 1800.0, sat=sat)
     ref, band = oracle_band(d, 1800.0, sat=sat)
     compare(d, got, ref, band, 1800.0)
+
+
+@pytest.mark.gpu
+def test_ch4_tran_without_mask(gpu):
+    """No filter (a NULL mask): every column is stepped."""
+    d = inputs.ch4_columns(nc=1500, seed=151)
+    d["mask"][:] = 1
+    ref, band = oracle_band(d, 1800.0)
+    e = dict(d)
+    e["mask"] = None
+    got = run_gpu(e, 1800.0)
+    compare(d, got, ref, band, 1800.0)

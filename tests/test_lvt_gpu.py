@@ -157,3 +157,17 @@ def test_lvt_sixteen_pools_three_tracers(gpu):
     for t in range(3):
         assert np.array_equal(got["conc"][t], ref["conc"][t]), t
         assert np.array_equal(got["tendency"][t], ref["tendency"][t]), t
+
+
+@pytest.mark.gpu
+def test_lvt_without_mask(gpu):
+    """No filter (a NULL mask): every column is transported; bit-exact."""
+    d = inputs.lvt_columns(nc=900, seed=126)
+    d["mask_bgc_soilc"][:] = 1
+    ref = run_oracle(d, 1800.0)
+    e = dict(d)
+    e["mask_bgc_soilc"] = None
+    got = run_gpu(e, 1800.0)
+    for t in range(d["ntypes"]):
+        assert np.array_equal(got["conc"][t], ref["conc"][t]), t
+        assert np.array_equal(got["tendency"][t], ref["tendency"][t]), t
