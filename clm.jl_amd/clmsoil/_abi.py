@@ -14,7 +14,8 @@ PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "libclmsoil.so"
 
 # status codes
-OK, ERR_NULL, ERR_BC, ERR_METHOD, ERR_SIZE, ERR_HIP, ERR_WORKSPACE, ERR_ARG, ERR_NOT_TRAINED = range(9)
+(OK, ERR_NULL, ERR_BC, ERR_METHOD, ERR_SIZE, ERR_HIP, ERR_WORKSPACE, ERR_ARG, ERR_NOT_TRAINED,
+ ERR_INIT) = range(10)
 
 # soil_water_movement.jl:23-34
 ZENGDECKER_2009, MOISTURE_FORM, MIXED_FORM, HEAD_FORM = 0, 1, 2, 3
@@ -45,7 +46,8 @@ MF_FIELDS = ["mask", "nbedrock", "h2osoi_liq", "smp_l", "hk_l", "qcharge", "nsub
 
 
 class MfArgs(C.Structure):
-    _fields_ = [("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in MF_FIELDS]
+    _fields_ = ([("nc", C.c_int64), ("ld", C.c_int64)] + [(f, _p) for f in MF_FIELDS] +
+                [("workspace", C.c_void_p), ("workspace_bytes", C.c_int64)])
 
 
 HEAT_FIELDS = ["mask", "snl", "nbedrock", "z", "dz", "zi", "t_soisno", "h2osoi_liq",
@@ -315,6 +317,8 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
             "The clmsoil product path has no CPU fallback.")
     lib = C.CDLL(str(p))
     declare_common(lib, "clmsoil_")
+    lib.clmsoil_init.restype = C.c_int
+    lib.clmsoil_richards_workspace_bytes.restype = C.c_int64
     lib.clmsoil_abi_version.restype = C.c_int
     lib.clmsoil_strerror.restype = C.c_char_p
     lib.clmsoil_strerror.argtypes = [C.c_int]
@@ -325,6 +329,8 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
                                                        C.c_double, C.POINTER(SwmConfig), C.c_void_p]
     lib.clmsoil_soilwm_moisture_form_solve.restype = C.c_int
     _declare_extended(lib)
+    # the library's one-time device set-up (its only allocation); no-op without a GPU
+    check(lib, lib.clmsoil_init(), "init")
     if path is None:
         _LIB = lib
     return lib

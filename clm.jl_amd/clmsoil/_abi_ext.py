@@ -11,6 +11,8 @@ def declare(lib) -> None:
     P, I = C.c_void_p, C.c_int
     lib.clmsoil_device_errors.argtypes = [P, C.POINTER(C.c_uint32)]
     lib.clmsoil_device_errors.restype = I
+    lib.clmsoil_workspace_errors.argtypes = [P, P, C.POINTER(C.c_uint32)]
+    lib.clmsoil_workspace_errors.restype = I
     lib.clmsoil_set_tuning.argtypes = [I, I, I, I]
     lib.clmsoil_set_tuning.restype = I
     lib.clmsoil_get_tuning.argtypes = [C.POINTER(I)] * 4
@@ -21,6 +23,8 @@ def declare(lib) -> None:
     lib.clmsoil_set_richards_residency.restype = I
     lib.clmsoil_set_zd_block.argtypes = [I]
     lib.clmsoil_set_zd_block.restype = I
+    lib.clmsoil_math_eval.argtypes = [I, P, P, P, C.c_int64, P]
+    lib.clmsoil_math_eval.restype = I
     _declare_learned(lib)
     _declare_adjoint(lib)
     _declare_day(lib)

@@ -115,6 +115,13 @@ def _lib():
     return _abi.load_library()
 
 
+def richards_workspace(device=None):
+    """A fresh workspace for one Richards call at a time (clmsoil_mf_args.workspace)."""
+    import torch
+    n = int(_lib().clmsoil_richards_workspace_bytes())
+    return torch.empty(n, dtype=torch.uint8, device=device or "cuda")
+
+
 def _mf_dict(st: MfState, mfin: MfIn, mask, nbedrock, nlevsno, nlevsoi, nlevgrnd):
     nc = nbedrock.shape[0]
     d = dict(nc=nc, nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd, mask=mask,
@@ -137,8 +144,11 @@ def _check_moisture_form_bc(cfg: SoilWaterMovementConfig):
 
 def soilwm_moisture_form_solve(st: MfState, mfin: MfIn, mask, nbedrock, joff: int, joff_zi: int,
                                nlevsoi: int, dtime: float, cfg: SoilWaterMovementConfig,
-                               nlevgrnd: int | None = None) -> None:
-    """One adaptive-substep moisture-form Richards step for every masked column."""
+                               nlevgrnd: int | None = None, workspace=None) -> None:
+    """One adaptive-substep moisture-form Richards step for every masked column.
+    `workspace` (richards_workspace()) is the call's scratch; a fresh one is
+    taken from torch's allocator when omitted (the reference allocates its
+    scratch per call too, soil_water_movement.jl:1555-1561)."""
     _check_moisture_form_bc(cfg)
     if joff_zi != joff + 1:
         raise CLMSoilError("joff_zi must equal joff + 1 (zi has one more level)")
@@ -148,13 +158,15 @@ def soilwm_moisture_form_solve(st: MfState, mfin: MfIn, mask, nbedrock, joff: in
     if d["nc"] == 0:
         return
     L = _lib()
+    d["workspace"] = workspace if workspace is not None else richards_workspace(nbedrock.device)
     a, lev = mf_args(d), levels(d)
     _abi.check(L, L.clmsoil_soilwm_moisture_form_solve(byref(a), byref(lev), float(dtime), byref(c),
                                                        _stream()), "soilwm_moisture_form_solve")
 
 
 def soilwater_moisture_form(st: MfState, mfin: MfIn, mask, nbedrock, nlevsno: int, nlevsoi: int,
-                            nlevgrnd: int, dtime: float, cfg: SoilWaterMovementConfig) -> None:
+                            nlevgrnd: int, dtime: float, cfg: SoilWaterMovementConfig,
+                            workspace=None) -> None:
     """The solve followed by the aquifer-recharge pass when use_aquifer_layer(cfg)."""
     _check_moisture_form_bc(cfg)
     c = cfg.to_abi()
@@ -163,6 +175,7 @@ def soilwater_moisture_form(st: MfState, mfin: MfIn, mask, nbedrock, nlevsno: in
     if d["nc"] == 0:
         return
     L = _lib()
+    d["workspace"] = workspace if workspace is not None else richards_workspace(nbedrock.device)
     a, lev = mf_args(d), levels(d)
     _abi.check(L, L.clmsoil_soilwater_moisture_form(byref(a), byref(lev), float(dtime), byref(c),
                                                     _stream()), "soilwater_moisture_form")
@@ -439,6 +452,16 @@ def batched_band_solve(tvector, bmatrix, rvector, jtop, jbot, mask, kl: int, ku:
                                                int(nc), _stream()), "batched_band_solve")
 
 
+def workspace_errors(workspace) -> int:
+    """Synchronize the current stream and return the error bits of the last
+    Richards call that used `workspace`."""
+    L = _lib()
+    flags = C.c_uint32(0)
+    _abi.check(L, L.clmsoil_workspace_errors(workspace.data_ptr(), _stream(), C.byref(flags)),
+               "workspace_errors")
+    return int(flags.value)
+
+
 def device_errors() -> int:
     """Synchronize the current stream and return/clear the device error bits."""
     L = _lib()
@@ -474,6 +497,29 @@ def get_tuning() -> tuple:
     v = [C.c_int() for _ in range(4)]
     _abi.check(L, L.clmsoil_get_tuning(*[C.byref(x) for x in v]), "get_tuning")
     return tuple(int(x.value) for x in v)
+
+
+_MATH_OPS = {"pow": 0, "log": 1, "log10": 2, "exp": 3, "jl_max": 4, "jl_min": 5}
+
+
+def math_eval(op: str, x, y=None):
+    """The kernels' FP64 pow/log/log10/exp (clm_math.h) and Julia max/min
+    evaluated on the device, element-wise over float64 CUDA tensors; returns a
+    new tensor."""
+    import torch
+    x = x.contiguous()
+    if y is not None:
+        y = y.contiguous()
+        if y.shape != x.shape or y.dtype != torch.float64:
+            raise ValueError("y must be a float64 tensor shaped like x")
+    if x.dtype != torch.float64 or not x.is_cuda:
+        raise ValueError("x must be a float64 CUDA tensor")
+    out = torch.empty_like(x)
+    L = _lib()
+    _abi.check(L, L.clmsoil_math_eval(_MATH_OPS[op], x.data_ptr(),
+                                      None if y is None else y.data_ptr(), out.data_ptr(),
+                                      x.numel(), _stream()), "math_eval")
+    return out
 
 
 def to_device(d: dict, device="cuda"):

@@ -57,6 +57,8 @@ def levels(d) -> Levels:
 
 
 def mf_args(d: dict) -> MfArgs:
+    """clmsoil_mf_args from a dict of arrays; `workspace` (optional) is a device
+    buffer of at least clmsoil_richards_workspace_bytes() bytes."""
     nc = int(d["nc"])
     a = MfArgs()
     a.nc = nc
@@ -65,6 +67,10 @@ def mf_args(d: dict) -> MfArgs:
         x = d.get(f)
         _check_dtype(f, x)
         setattr(a, f, _ptr(x))
+    ws = d.get("workspace")
+    if ws is not None:
+        a.workspace = _ptr(ws)
+        a.workspace_bytes = int(ws.numel() * ws.element_size()) if hasattr(ws, "numel") else ws.nbytes
     return a
 
 

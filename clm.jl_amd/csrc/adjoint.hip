@@ -76,10 +76,10 @@ __device__ void substep_vjp(const ColIn &col, const RichardsParams &P,
         act1[j] = r1 < 1.0 && m > 0.01;
         const double b = col.bsw[j];
         const double e = 2.0 * b + 3.0;
-        pw[j] = pow(s1[j], e);
+        pw[j] = clm_pow(s1[j], e);
         hk[j] = col.khat[j] * pw[j];
         dhk[j] = e * hk[j] / s1[j];
-        q[j] = pow(s2[j], -b);
+        q[j] = clm_pow(s2[j], -b);
         smp[j] = -col.sucsat[j] * q[j];
         dsmp[j] = (-b * smp[j] / s2[j]) / col.watsat[j];
     }
@@ -192,14 +192,14 @@ __device__ void substep_vjp(const ColIn &col, const RichardsParams &P,
         s2b[j] += dsb * (b * smp[j] / (s2[j] * s2[j]));
         // smp = -sucsat s2^-b
         s2b[j] += smpb[j] * col.sucsat[j] * b * q[j] / s2[j];
-        gb[j] += smpb[j] * col.sucsat[j] * q[j] * log(s2[j]);
+        gb[j] += smpb[j] * col.sucsat[j] * q[j] * clm_log(s2[j]);
         // dhkdw = e hk / s1 ; hk = khat s1^e ; e = 2b + 3
         double eb = dhkb[j] * hk[j] / s1[j];
         const double hkt = hkb[j] + dhkb[j] * e / s1[j];
         double s1b = -dhkb[j] * e * hk[j] / (s1[j] * s1[j]);
         gk[j] += hkt * pw[j];
         s1b += hkt * e * hk[j] / s1[j];
-        eb += hkt * hk[j] * log(s1[j]);
+        eb += hkt * hk[j] * clm_log(s1[j]);
         gb[j] += 2.0 * eb;
         const double r1b = act1[j] ? s1b : 0.0;
         if (j < n - 1) {

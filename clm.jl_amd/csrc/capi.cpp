@@ -7,6 +7,7 @@
 // recorded in a per-device error word that clmsoil_device_errors() reads back.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <initializer_list>
 #include <cstdlib>
@@ -42,24 +43,51 @@ namespace clmsoil {
 Tuning g_tuning;   // set by clmsoil_set_tuning (the autotuner), read at launch
 
 namespace {
-std::mutex g_mu;
-std::vector<DeviceState> g_dev;
+std::mutex g_mu;                     // serialises clmsoil_init
+std::vector<DeviceState> g_dev;      // one entry per device, filled by clmsoil_init
+std::atomic<bool> g_ready{false};    // g_dev is complete and no longer changes
 }  // namespace
 
+int init_devices() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_ready.load(std::memory_order_acquire)) return CLMSOIL_OK;
+    int n = 0;
+    const hipError_t ec = hipGetDeviceCount(&n);
+    if (ec == hipErrorNoDevice) n = 0;
+    else if (ec != hipSuccess) return CLMSOIL_ERR_HIP;
+    int cur = 0;
+    if (n > 0 && hipGetDevice(&cur) != hipSuccess) return CLMSOIL_ERR_HIP;
+    std::vector<DeviceState> dev((size_t)n);
+    int st = CLMSOIL_OK;
+    for (int d = 0; d < n && st == CLMSOIL_OK; ++d) {
+        void *p = nullptr;
+        if (hipSetDevice(d) != hipSuccess || hipMalloc(&p, 256) != hipSuccess ||
+            hipMemset(p, 0, 256) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+            st = CLMSOIL_ERR_HIP;
+            if (p) (void)hipFree(p);
+            break;
+        }
+        dev[(size_t)d].err = static_cast<unsigned *>(p);
+    }
+    if (n > 0 && hipSetDevice(cur) != hipSuccess) st = CLMSOIL_ERR_HIP;
+    if (st != CLMSOIL_OK) {
+        for (int d = 0; d < n; ++d)
+            if (dev[(size_t)d].err && hipSetDevice(d) == hipSuccess) (void)hipFree(dev[(size_t)d].err);
+        if (n > 0) (void)hipSetDevice(cur);
+        return st;
+    }
+    g_dev.swap(dev);
+    g_ready.store(true, std::memory_order_release);
+    return CLMSOIL_OK;
+}
+
+// The state clmsoil_init() set up for the current device; allocates nothing.
 int current_device_state(DeviceState **out) {
+    if (!g_ready.load(std::memory_order_acquire)) return CLMSOIL_ERR_INIT;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return CLMSOIL_ERR_HIP;
-    std::lock_guard<std::mutex> lk(g_mu);
-    if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
-    DeviceState &s = g_dev[dev];
-    if (!s.err) {
-        // one-time per-device allocation of the 256-byte status block
-        void *p = nullptr;
-        if (hipMalloc(&p, 256) != hipSuccess) return CLMSOIL_ERR_HIP;
-        if (hipMemset(p, 0, 256) != hipSuccess) return CLMSOIL_ERR_HIP;
-        s.err = static_cast<unsigned *>(p);
-    }
-    *out = &s;
+    if (dev < 0 || (size_t)dev >= g_dev.size() || !g_dev[(size_t)dev].err) return CLMSOIL_ERR_INIT;
+    *out = &g_dev[(size_t)dev];
     return CLMSOIL_OK;
 }
 
@@ -91,6 +119,11 @@ int validate_mf(const clmsoil_mf_args *a, const clmsoil_levels *lev, double dtim
         !a->bsw || !a->sucsat || !a->icefrac || !a->qflx_infl)
         return CLMSOIL_ERR_NULL;
     if (bc == CLMSOIL_BC_WATERTABLE && (!a->zwt || !a->zi)) return CLMSOIL_ERR_NULL;
+    if (a->workspace && (a->workspace_bytes < RICHARDS_WORKSPACE_BYTES ||
+                         (reinterpret_cast<uintptr_t>(a->workspace) & 255u) != 0))
+        return CLMSOIL_ERR_WORKSPACE;
+    // the per-XCD work counters are 32-bit: every XCD's range stays below 2^31
+    if (a->nc > (int64_t)NUM_XCD << 31) return CLMSOIL_ERR_SIZE;
     return CLMSOIL_OK;
 }
 
@@ -113,7 +146,10 @@ RichardsParams richards_params(const clmsoil_mf_args *a, const clmsoil_levels *l
     P.dtime = dtime; P.e_ice = cfg->e_ice; P.dtmin = cfg->dtmin;
     P.xtu = cfg->xTolerUpper; P.xtl = cfg->xTolerLower; P.verysmall = cfg->verySmall;
     P.err = err;
-    P.work_counters = err + 8;
+    // the workspace (if any) holds the call's error word and the work counters
+    unsigned *ws = static_cast<unsigned *>(a->workspace);
+    P.call_err = ws;
+    P.work_counters = ws ? ws + 8 : nullptr;
     return P;
 }
 
@@ -139,16 +175,21 @@ __global__ void zero_words_kernel(unsigned *p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
+// The resident-grid launch needs the workspace's counters; without a workspace
+// the step runs as a plain launch (identical results).
 int run_richards(const RichardsParams &P, hipStream_t stream) {
-    const bool persistent = g_tuning.richards_persistent != 0;
-    if (persistent) {
-        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, P.work_counters,
-                           NUM_XCD);
+    const bool persistent =
+        g_tuning.richards_persistent.load(std::memory_order_relaxed) != 0 && P.work_counters;
+    if (P.call_err) {
+        // clear the call's error word and the counters (a kernel, not a memset,
+        // so that a captured graph replays it)
+        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, P.call_err, 16);
         if (hipGetLastError() != hipSuccess) return CLMSOIL_ERR_HIP;
     }
-    return launch_richards(P, g_tuning.richards_g, g_tuning.richards_block, persistent,
-                           g_tuning.richards_blocks_per_cu, stream) ==
-                   hipSuccess
+    return launch_richards(P, g_tuning.richards_g.load(std::memory_order_relaxed),
+                           g_tuning.richards_block.load(std::memory_order_relaxed), persistent,
+                           g_tuning.richards_blocks_per_cu.load(std::memory_order_relaxed),
+                           stream) == hipSuccess
                ? CLMSOIL_OK
                : CLMSOIL_ERR_HIP;
 }
@@ -162,7 +203,8 @@ int run_heat(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime
     P.snow_method = cfg ? cfg->snow_thermal_cond_method : 1;
     P.dtime = dtime;
     P.err = err;
-    return launch_heat(P, g_tuning.heat_g, g_tuning.heat_block, stream) == hipSuccess
+    return launch_heat(P, g_tuning.heat_g.load(std::memory_order_relaxed),
+                       g_tuning.heat_block.load(std::memory_order_relaxed), stream) == hipSuccess
                ? CLMSOIL_OK
                : CLMSOIL_ERR_HIP;
 }
@@ -170,6 +212,10 @@ int run_heat(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime
 }  // namespace clmsoil
 
 extern "C" {
+
+int clmsoil_init(void) { return init_devices(); }
+
+int64_t clmsoil_richards_workspace_bytes(void) { return RICHARDS_WORKSPACE_BYTES; }
 
 int clmsoil_abi_version(void) { return CLMSOIL_ABI_VERSION; }
 
@@ -184,6 +230,7 @@ const char *clmsoil_strerror(int status) {
         case CLMSOIL_ERR_WORKSPACE: return "workspace missing or too small";
         case CLMSOIL_ERR_ARG: return "invalid argument";
         case CLMSOIL_ERR_NOT_TRAINED: return "learned controller has no weights";
+        case CLMSOIL_ERR_INIT: return "clmsoil_init() has not set up this device";
         default: return "unknown status";
     }
 }
@@ -213,42 +260,52 @@ int clmsoil_device_errors(void *stream, uint32_t *flags) {
     return CLMSOIL_OK;
 }
 
+int clmsoil_workspace_errors(const void *workspace, void *stream, uint32_t *flags) {
+    if (!workspace || !flags) return CLMSOIL_ERR_NULL;
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return CLMSOIL_ERR_HIP;
+    unsigned h = 0;
+    if (hipMemcpy(&h, workspace, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
+        return CLMSOIL_ERR_HIP;
+    *flags = h;
+    return CLMSOIL_OK;
+}
+
 int clmsoil_set_tuning(int richards_g, int richards_block, int heat_g, int heat_block) {
     auto okg = [](int g) { return g == 1 || g == 2 || g == 4; };
     auto okb = [](int b) { return b == 64 || b == 128 || b == 256; };
     if (!okg(richards_g) || !okg(heat_g) || !okb(richards_block) || !okb(heat_block))
         return CLMSOIL_ERR_ARG;
-    g_tuning.richards_g = richards_g;
-    g_tuning.richards_block = richards_block;
-    g_tuning.heat_g = heat_g;
-    g_tuning.heat_block = heat_block;
+    g_tuning.richards_g.store(richards_g);
+    g_tuning.richards_block.store(richards_block);
+    g_tuning.heat_g.store(heat_g);
+    g_tuning.heat_block.store(heat_block);
     return CLMSOIL_OK;
 }
 
 int clmsoil_set_richards_schedule(int persistent) {
     if (persistent != 0 && persistent != 1) return CLMSOIL_ERR_ARG;
-    g_tuning.richards_persistent = persistent;
+    g_tuning.richards_persistent.store(persistent);
     return CLMSOIL_OK;
 }
 
 int clmsoil_set_richards_residency(int blocks_per_cu) {
     if (blocks_per_cu < 0 || blocks_per_cu > 32) return CLMSOIL_ERR_ARG;
-    g_tuning.richards_blocks_per_cu = blocks_per_cu;
+    g_tuning.richards_blocks_per_cu.store(blocks_per_cu);
     return CLMSOIL_OK;
 }
 
 int clmsoil_set_zd_block(int block) {
     if (block != 64 && block != 128 && block != 256) return CLMSOIL_ERR_ARG;
-    g_tuning.zd_block = block;
+    g_tuning.zd_block.store(block);
     return CLMSOIL_OK;
 }
 
 int clmsoil_get_tuning(int *richards_g, int *richards_block, int *heat_g, int *heat_block) {
     if (!richards_g || !richards_block || !heat_g || !heat_block) return CLMSOIL_ERR_NULL;
-    *richards_g = g_tuning.richards_g;
-    *richards_block = g_tuning.richards_block;
-    *heat_g = g_tuning.heat_g;
-    *heat_block = g_tuning.heat_block;
+    *richards_g = g_tuning.richards_g.load();
+    *richards_block = g_tuning.richards_block.load();
+    *heat_g = g_tuning.heat_g.load();
+    *heat_block = g_tuning.heat_block.load();
     return CLMSOIL_OK;
 }
 
@@ -373,7 +430,8 @@ int clmsoil_soilwater_zengdecker2009(const clmsoil_zd_args *a, const clmsoil_lev
                a->t_soisno, a->zwt, a->qflx_infl, a->qflx_rootsoi, a->icefrac, a->smp_l,
                a->hk_l, a->qcharge, a->qflx_deficit, lev->nlevsno, lev->nlevsoi,
                lev->nlevgrnd, dtime, cfg->e_ice};
-    return launch_zengdecker2009(P, g_tuning.zd_block, (hipStream_t)stream) == hipSuccess
+    return launch_zengdecker2009(P, g_tuning.zd_block.load(std::memory_order_relaxed),
+                                 (hipStream_t)stream) == hipSuccess
                ? CLMSOIL_OK
                : CLMSOIL_ERR_HIP;
 }
@@ -405,7 +463,8 @@ int clmsoil_ground_heat_flux(const clmsoil_ghf_args *a, const clmsoil_levels *le
     int st = current_device_state(&s);
     if (st) return st;
     GhfParams P{*a, lev->nlevsno, s->err};
-    return launch_ground_heat_flux(P, g_tuning.ghf_block, (hipStream_t)stream) == hipSuccess
+    return launch_ground_heat_flux(P, g_tuning.ghf_block.load(std::memory_order_relaxed),
+                                   (hipStream_t)stream) == hipSuccess
                ? CLMSOIL_OK
                : CLMSOIL_ERR_HIP;
 }

@@ -6,6 +6,12 @@
 
 #include "../../include/clmsoil.h"
 
+// The project's FP64 log/log10/exp/pow (clm_math.h) as device functions with
+// device-resident tables; the CPU restatement compiles the same source.
+#define CLM_MATH_FN __device__ static __forceinline__
+#define CLM_MATH_TAB __device__ static const
+#include "clm_math.h"
+
 namespace clmsoil {
 
 // Physical constants, src/constants/varcon.jl:15-70.
@@ -118,6 +124,18 @@ __device__ __forceinline__ double group_next<4>(double v) {
 template <>
 __device__ __forceinline__ double group_prev<4>(double v) {
     return dpp_f64<(3 | 0 << 2 | 1 << 4 | 2 << 6)>(v);
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+// Min of an int over the G lanes of a group.
+template <int G>
+__device__ __forceinline__ int group_min_i(int v) {
+    if constexpr (G >= 2) v = min(v, dpp_i32<(1 | 0 << 2 | 3 << 4 | 2 << 6)>(v));
+    if constexpr (G >= 4) v = min(v, dpp_i32<(2 | 3 << 2 | 0 << 4 | 1 << 6)>(v));
+    return v;
 }
 
 // Max over the G lanes of a group.

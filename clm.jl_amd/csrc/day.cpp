@@ -2,8 +2,10 @@
 //
 // The capture runs on a private non-blocking stream (default priority), so
 // callers may pass any stream, including the null stream, to the launch. The
-// persistent Richards launch clears its work counters with a small kernel,
-// never a memset node, so the graph replays correctly every time.
+// plan owns the Richards workspace its graph uses (allocated when the plan is
+// created), so replays share nothing with eager calls or other plans; every
+// step clears its work counters with a small kernel, never a memset node, so
+// the graph replays correctly every time.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -33,8 +35,16 @@ using namespace clmsoil;
 struct clmsoil_day_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    void *workspace = nullptr;   // the Richards steps' workspace (plan-owned)
     int nsteps = 0;
 };
+
+static void free_plan(clmsoil_day_plan *p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->workspace) (void)hipFree(p->workspace);
+    delete p;
+}
 
 extern "C" {
 
@@ -59,20 +69,27 @@ int clmsoil_day_plan_create(const clmsoil_heat_args *heat, const clmsoil_mf_args
     if (learned && mf->dtsub0) return CLMSOIL_ERR_ARG;
     DeviceState *s = nullptr;
     if ((st = current_device_state(&s))) return st;
+    auto *p = new (std::nothrow) clmsoil_day_plan();
+    if (!p) return CLMSOIL_ERR_HIP;
+    p->nsteps = nsteps;
+    if (hipMalloc(&p->workspace, RICHARDS_WORKSPACE_BYTES) != hipSuccess) {
+        p->workspace = nullptr;
+        free_plan(p);
+        return CLMSOIL_ERR_HIP;
+    }
     clmsoil_mf_args m = *mf;
     if (learned) m.dtsub0 = dtsub0;
+    m.workspace = p->workspace;
+    m.workspace_bytes = RICHARDS_WORKSPACE_BYTES;
     const int bc = swm_cfg->lower_boundary_condition;
     const bool aquifer = bc == CLMSOIL_BC_WATERTABLE || bc == CLMSOIL_BC_AQUIFER;
     RichardsParams P = richards_params(&m, lev, dtime, swm_cfg, aquifer, s->err);
 
     hipStream_t cs = nullptr;
-    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return CLMSOIL_ERR_HIP;
-    auto *p = new (std::nothrow) clmsoil_day_plan();
-    if (!p) {
-        hipStreamDestroy(cs);
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+        free_plan(p);
         return CLMSOIL_ERR_HIP;
     }
-    p->nsteps = nsteps;
     st = CLMSOIL_OK;
     if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         st = CLMSOIL_ERR_HIP;
@@ -90,11 +107,9 @@ int clmsoil_day_plan_create(const clmsoil_heat_args *heat, const clmsoil_mf_args
     }
     if (st == CLMSOIL_OK && hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess)
         st = CLMSOIL_ERR_HIP;
-    hipStreamDestroy(cs);
+    (void)hipStreamDestroy(cs);
     if (st != CLMSOIL_OK) {
-        if (p->exec) hipGraphExecDestroy(p->exec);
-        if (p->graph) hipGraphDestroy(p->graph);
-        delete p;
+        free_plan(p);
         return st;
     }
     *plan = p;
@@ -109,9 +124,9 @@ int clmsoil_day_plan_launch(clmsoil_day_plan *plan, void *stream) {
 
 int clmsoil_day_plan_destroy(clmsoil_day_plan *plan) {
     if (!plan) return CLMSOIL_OK;
-    if (plan->exec) hipGraphExecDestroy(plan->exec);
-    if (plan->graph) hipGraphDestroy(plan->graph);
-    delete plan;
+    // the graph may still run on the caller's stream: wait before freeing
+    if (hipDeviceSynchronize() != hipSuccess) return CLMSOIL_ERR_HIP;
+    free_plan(plan);
     return CLMSOIL_OK;
 }
 

@@ -37,6 +37,16 @@
 // pivoting fallback in LDS; diagnostics are written as soon as they are known.
 #include <atomic>
 
+#include <hip/hip_runtime.h>
+// The log and exp tables of clm_math.h are read from a per-block copy in LDS
+// (8 KB per block; same values, same results): the table gathers of log10 and
+// pow sit on the critical path of every level's conductivity. With the pivot
+// tiles this fits two 256-thread blocks per CU.
+static __shared__ double clm_lds_log_tab[256][3];
+static __shared__ double clm_lds_exp_tab[128][2];
+#define CLM_LOG_TAB(i, k) clm_lds_log_tab[i][k]
+#define CLM_EXP_TAB(j, k) clm_lds_exp_tab[j][k]
+
 #include "common.hpp"
 #include "heat.hpp"
 
@@ -159,12 +169,12 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             satw = jl_min(1.0, satw);
             if (satw > 1.0e-7) {
                 double dke;
-                if (t[i] >= TFRZ) dke = jl_max(0.0, log10(satw) + 1.0);
+                if (t[i] >= TFRZ) dke = jl_max(0.0, clm_log10(satw) + 1.0);
                 else dke = satw;
                 const double fl_denom = liq / (DENH2O * dzi) + ice / (DENICE * dzi) +
                                         exc / (DENICE * dzi);
                 const double fl = (liq / (DENH2O * dzi)) / fl_denom;
-                const double dksat = tkmg * pow(TKWAT, fl * wsat) * pow(TKICE, (1.0 - fl) * wsat);
+                const double dksat = tkmg * clm_pow(TKWAT, fl * wsat) * clm_pow(TKICE, (1.0 - fl) * wsat);
                 th = dke * dksat + (1.0 - dke) * tkdry;
             } else {
                 th = tkdry;
@@ -684,7 +694,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             if (exc_i > 0.0 && tt > tfrz) { imelt[i] = 1; tinc = tfrz - tt; tt = tfrz; }
             if (tt < tfrz) {
                 const double smp = hfus * (tfrz - tt) / (GRAV * tt) * 1000.0;
-                supercool = wsat_i * pow(smp / sucsat_i, -1.0 / bsw_i);
+                supercool = wsat_i * clm_pow(smp / sucsat_i, -1.0 / bsw_i);
                 supercool *= dz_i * 1000.0;
             }
             if (liq[i] > supercool && tt < tfrz) { imelt[i] = 2; tinc = tfrz - tt; tt = tfrz; }
@@ -865,6 +875,11 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
 template <int G>
 __global__ __launch_bounds__(256, CLMSOIL_HEAT_WAVES) void heat_kernel(HeatParams P) {
     extern __shared__ double heat_lds[];
+    for (int t = threadIdx.x; t < 256 * 3; t += blockDim.x)
+        (&clm_lds_log_tab[0][0])[t] = (&clm_log_tab[0][0])[t];
+    for (int t = threadIdx.x; t < 128 * 2; t += blockDim.x)
+        (&clm_lds_exp_tab[0][0])[t] = (&clm_exp_tab[0][0])[t];
+    __syncthreads();
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t c = gtid / G;
     const int k = (int)(threadIdx.x % G);

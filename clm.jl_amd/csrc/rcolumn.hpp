@@ -86,47 +86,75 @@ struct RColumn {
     __device__ __forceinline__ bool load(const RichardsParams &P, int64_t col, int k) {
         c = col;
         ld_ = P.ld;
-        n = (int)P.nbedrock[c];
-        if (n < 1 || n > P.nlevsoi) {
-            if (k == 0) atomicOr(P.err, (unsigned)DEV_ERR_NBEDROCK);
-            return false;
-        }
         const int64_t ld = P.ld;
         const int joff = P.nlevsno, joff_zi = P.nlevsno + 1;
-        // water-table index (BC_WATERTABLE, :1346-1350)
+        n = (int)P.nbedrock[c];
+        // water-table index (BC_WATERTABLE, :1346-1350): the first interface jj in
+        // 1..nlevsoi with zwt <= zi[jj] gives jwt = jj - 1 (nlevsoi when none).
+        // Each lane tests its own interfaces and the group takes the minimum, so
+        // every load is in flight at once (a refill stalls the whole wavefront).
         jwt = P.nlevsoi;
         zwt = 0.0;
         if (P.lower_bc == CLMSOIL_BC_WATERTABLE || P.aquifer_pass) {
             zwt = P.zwt[c];
-            for (int jj = 1; jj <= P.nlevsoi; ++jj)
-                if (zwt <= P.zi[c + (int64_t)(joff_zi + jj - 1) * ld]) { jwt = jj - 1; break; }
+            int jf = P.nlevsoi;
+#pragma unroll
+            for (int i = 0; i < S; ++i) {
+                const int jj = i * G + k + 1;
+                if (jj <= P.nlevsoi) {
+                    const double zij = P.zi[c + (int64_t)(joff_zi + jj - 1) * ld];
+                    if (zwt <= zij && jj - 1 < jf) jf = jj - 1;
+                }
+            }
+            jwt = group_min_i<G>(jf);
+        }
+        if (n < 1 || n > P.nlevsoi) {
+            if (k == 0) P.flag((unsigned)DEV_ERR_NBEDROCK);
+            return false;
         }
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             const int j = i * G + k + 1;
+            double lq = 0.0, dzv = 1.0, ws = 1.0, hks = 0.0, icf = 0.0, icf1 = 0.0, zj = 0.0,
+                   zj1 = 0.0, bs = 1.0, su = 0.0, rt = 0.0;
+            // every level of the soil grid is read, above bedrock or not: the loads
+            // then need only the column index and are all in flight at once
+            if (j <= P.nlevsoi) {
+                const int64_t o = soil(j);
+                const int64_t os = c + (int64_t)(joff + j - 1) * ld;
+                lq = P.h2osoi_liq[os];
+                dzv = __builtin_nontemporal_load(&P.dz[os]);
+                ws = __builtin_nontemporal_load(&P.watsat[o]);
+                hks = __builtin_nontemporal_load(&P.hksat[o]);
+                icf = __builtin_nontemporal_load(&P.icefrac[o]);
+                zj = __builtin_nontemporal_load(&P.z[os]);
+                if (j < P.nlevsoi) {
+                    icf1 = __builtin_nontemporal_load(&P.icefrac[o + ld]);
+                    zj1 = __builtin_nontemporal_load(&P.z[os + ld]);
+                }
+                if constexpr (LC) {
+                    bs = P.bsw[o];
+                    su = P.sucsat[o];
+                    rt = P.qflx_rootsoi[o];
+                }
+            }
             if (j <= n) {
-                const int64_t o = soil(j);                               // soil-array offset
-                const int64_t os = c + (int64_t)(joff + j - 1) * ld;    // snow+soil offset
-                liq[i] = P.h2osoi_liq[os];
-                mdz[i] = M_TO_MM * __builtin_nontemporal_load(&P.dz[os]);
-                watsat[i] = __builtin_nontemporal_load(&P.watsat[o]);
-                const double hks = __builtin_nontemporal_load(&P.hksat[o]);
-                const double icf = __builtin_nontemporal_load(&P.icefrac[o]);
-                const double zj = __builtin_nontemporal_load(&P.z[os]);
+                liq[i] = lq;
+                mdz[i] = M_TO_MM * dzv;
+                watsat[i] = ws;
                 double imp;
                 if (j == n) {
-                    imp = pow(10.0, -P.e_ice * icf);
+                    imp = clm_pow(10.0, -P.e_ice * icf);
                     dn(i) = M_TO_MM * (zwt - zj);                      // :1357 (WATERTABLE)
                 } else {
-                    const double icf1 = __builtin_nontemporal_load(&P.icefrac[o + ld]);
-                    imp = pow(10.0, -P.e_ice * (0.5 * (icf + icf1)));
-                    dn(i) = M_TO_MM * (__builtin_nontemporal_load(&P.z[os + ld]) - zj);   // :1338
+                    imp = clm_pow(10.0, -P.e_ice * (0.5 * (icf + icf1)));
+                    dn(i) = M_TO_MM * (zj1 - zj);                      // :1338
                 }
                 kh(i) = imp * hks;   // (imp * hksat) * s1^(2b+3), the reference's grouping
                 if constexpr (LC) {
-                    cache(LC_BSW, i) = P.bsw[o];
-                    cache(LC_SUC, i) = P.sucsat[o];
-                    cache(LC_ROOT, i) = P.qflx_rootsoi[o];
+                    cache(LC_BSW, i) = bs;
+                    cache(LC_SUC, i) = su;
+                    cache(LC_ROOT, i) = rt;
                 }
             } else {
                 liq[i] = 0.0; mdz[i] = M_TO_MM; watsat[i] = 1.0; kh(i) = 0.0; dn(i) = 1.0;
@@ -182,7 +210,7 @@ struct RColumn {
             }
             bsw[i] = b;
             const double sn = s2[i];
-            const double smpv = -suc * pow(sn, -b);
+            const double smpv = -suc * clm_pow(sn, -b);
             const double dsmpds = -b * smpv / sn;
             smp[i] = smpv;
             dsmpdw[i] = dsmpds / watsat[i];
@@ -199,7 +227,7 @@ struct RColumn {
             s1 = jl_min(s1, 1.0);
             s1 = jl_max(0.01, s1);
             const double b = bsw[i];
-            const double hkv = kh(i) * pow(s1, 2.0 * b + 3.0);
+            const double hkv = kh(i) * clm_pow(s1, 2.0 * b + 3.0);
             const double dhkdw = (2.0 * b + 3.0) * hkv / s1;
             if constexpr (LC) cache(LC_HK, i) = hkv;
             else if (j <= n && P.hk_l) P.hk_l[soil(j)] = hkv;
@@ -271,7 +299,7 @@ struct RColumn {
         const int lower_bc = P.lower_bc;
         const double dtime = P.dtime;
         if (iter >= P.max_iter) {
-            if (k == 0) atomicOr(P.err, (unsigned)DEV_ERR_ITER);
+            if (k == 0) P.flag((unsigned)DEV_ERR_ITER);
             return true;
         }
         ++iter;
@@ -441,7 +469,7 @@ struct RColumn {
                 const double wh_zwt = -P.sucsat[o] - zwt * M_TO_MM;
                 double s1 = jl_max(vwc / P.watsat[o], 0.01);
                 s1 = jl_min(1.0, s1);
-                const double ka = 0.0 * P.hksat[o] * pow(s1, 2.0 * P.bsw[o] + 3.0);
+                const double ka = 0.0 * P.hksat[o] * clm_pow(s1, 2.0 * P.bsw[o] + 3.0);
                 const double smpmin = P.smpmin ? P.smpmin[c] : -1.0e8;
                 const double smp1 = jl_max(smpmin, 0.0);
                 const int jm = jwt > 1 ? jwt : 1;

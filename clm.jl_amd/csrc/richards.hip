@@ -30,13 +30,33 @@
 // groups move on to other XCDs' ranges at the end).
 #include <atomic>
 
+#include <hip/hip_runtime.h>
+// The log and exp tables of clm_math.h are read from a per-block copy in LDS:
+// every property update gathers 5 table entries per pow at lane-dependent
+// indices, and LDS answers those gathers far sooner than the cache hierarchy.
+// 8 KB per block; the values (and so the results) are the same.
+static __shared__ double clm_lds_log_tab[256][3];
+static __shared__ double clm_lds_exp_tab[128][2];
+#define CLM_LOG_TAB(i, k) clm_lds_log_tab[i][k]
+#define CLM_EXP_TAB(j, k) clm_lds_exp_tab[j][k]
+
 #include "rcolumn.hpp"
 
 namespace clmsoil {
 
+// Copy the math tables to the block's LDS (all threads, before any returns).
+__device__ __forceinline__ void fill_math_tables() {
+    for (int t = threadIdx.x; t < 256 * 3; t += blockDim.x)
+        (&clm_lds_log_tab[0][0])[t] = (&clm_log_tab[0][0])[t];
+    for (int t = threadIdx.x; t < 128 * 2; t += blockDim.x)
+        (&clm_lds_exp_tab[0][0])[t] = (&clm_exp_tab[0][0])[t];
+    __syncthreads();
+}
+
 // LDS cache stride of a launch: the block size for G = 4 at 256 threads per block
-// (35 doubles per lane, 70 KB per block), none otherwise (it would not fit at
-// G = 1 or 2, and other block sizes keep the global-memory path).
+// (35 doubles per lane, 70 KB per block beside the 8 KB of math tables), none
+// otherwise (it would not fit at G = 1 or 2, and other block sizes keep the
+// global-memory path).
 template <int G>
 constexpr int richards_cache_stride(int block) {
     return (G == 4 && block == 256) ? 256 : 0;
@@ -50,6 +70,7 @@ static size_t richards_lds_bytes() {
 // One column per group for the group's lifetime.
 template <int G, bool EXP, int LCS>
 __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kernel(RichardsParams P) {
+    fill_math_tables();
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t c = gtid / G;
     const int k = (int)(threadIdx.x % G);
@@ -70,6 +91,7 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kerne
 // Every wavefront leaves the loop once all eight ranges are exhausted.
 template <int G, bool EXP, int LCS>
 __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persistent(RichardsParams P) {
+    fill_math_tables();
     const int k = (int)(threadIdx.x % G);
     const int lane = (int)__lane_id();
     const int xcd = (int)(blockIdx.x % NUM_XCD);

@@ -19,7 +19,13 @@ struct RichardsParams {
     int max_iter;           // substep-iteration cap (hang guard)
     double dtime, e_ice, dtmin, xtu, xtl, verysmall;
     unsigned *err;          // device error word
-    unsigned *work_counters;  // NUM_XCD zeroed counters (persistent launch)
+    unsigned *call_err;     // the call's own error word (workspace), or NULL
+    unsigned *work_counters;  // NUM_XCD zeroed counters (persistent launch), or NULL
+    // record an input the reference would reject (bits OR-ed, never cleared here)
+    __device__ __forceinline__ void flag(unsigned f) const {
+        atomicOr(err, f);
+        if (call_err) atomicOr(call_err, f);
+    }
 };
 
 // G = lanes per column (1, 2 or 4); block = threads per workgroup; persistent

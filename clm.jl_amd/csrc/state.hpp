@@ -2,29 +2,36 @@
 #pragma once
 #include <stdint.h>
 
+#include <atomic>
+
 #include "../../include/clmsoil.h"
 
 namespace clmsoil {
 
 // Lanes per column (G) and workgroup size of the two fused kernels. Defaults are
-// the tuned values for MI355X; clmsoil_set_tuning() replaces them.
+// the tuned values for MI355X; clmsoil_set_tuning() replaces them. Atomic: a
+// launch reads them once; any value gives the same results.
 struct Tuning {
-    int richards_g = 4;
-    int richards_block = 256;
-    int heat_g = 4;
-    int heat_block = 256;
-    int richards_persistent = 1;   // resident grid with per-group column refill
-    int richards_blocks_per_cu = 0;  // persistent grid size; 0 = occupancy query
-    int zd_block = 128;              // Zeng-Decker 2009 step, one lane per column
-    int ghf_block = 256;             // ground heat flux
+    std::atomic<int> richards_g{4};
+    std::atomic<int> richards_block{256};
+    std::atomic<int> heat_g{4};
+    std::atomic<int> heat_block{256};
+    std::atomic<int> richards_persistent{1};   // resident grid with per-group column refill
+    std::atomic<int> richards_blocks_per_cu{0};  // persistent grid size; 0 = occupancy query
+    std::atomic<int> zd_block{128};              // Zeng-Decker 2009 step, one lane per column
+    std::atomic<int> ghf_block{256};             // ground heat flux
 };
 extern Tuning g_tuning;
 
-// Allocated once per device on first use: a 256-byte status block (word 0 device
-// error bits, words 8..15 per-XCD work counters of the persistent Richards launch).
+// Set up once per device by clmsoil_init(): the 256-byte status block whose
+// word 0 holds the device's error bits.
 struct DeviceState {
     unsigned *err = nullptr;
 };
+
+// Bytes of a Richards call's workspace: word 0 the call's error bits, words
+// 8..15 the per-XCD work counters of the resident-grid launch.
+constexpr int64_t RICHARDS_WORKSPACE_BYTES = 256;
 
 int current_device_state(DeviceState **out);
 int validate_levels(const clmsoil_levels *lev);

@@ -12,9 +12,13 @@
  * Integer state (`nbedrock`, `snl`, `imelt`) is Int64 as in Julia; masks are Bool
  * (one byte). Streams are `hipStream_t` passed as `void*` (NULL = default stream).
  *
- * All calls are asynchronous on the given stream, never free caller memory, and
- * allocate nothing on the device except where a function says so. They return a
- * status code; `clmsoil_strerror` names it.
+ * All calls are asynchronous on the given stream, never free caller memory and
+ * never allocate: clmsoil_init() allocates the library's only device memory (a
+ * 256-byte status block per device) once, before any other call; scratch that
+ * a solve needs comes from a caller workspace. They return a status code;
+ * `clmsoil_strerror` names it. Calls are re-entrant: calls on different streams
+ * (or threads) share nothing but the per-device error flags, which kernels
+ * only OR into.
  *
  * Reference interfaces replaced (paths relative to the CLM.jl tree):
  *   soilwm_moisture_form_solve!   src/biogeophys/soil_water_movement.jl:1498-1517
@@ -35,7 +39,7 @@
 extern "C" {
 #endif
 
-#define CLMSOIL_ABI_VERSION 1
+#define CLMSOIL_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------- */
 enum {
@@ -47,7 +51,8 @@ enum {
     CLMSOIL_ERR_HIP = 5,         /* a HIP runtime call failed                     */
     CLMSOIL_ERR_WORKSPACE = 6,   /* workspace missing or too small                */
     CLMSOIL_ERR_ARG = 7,         /* other invalid argument                        */
-    CLMSOIL_ERR_NOT_TRAINED = 8  /* learned controller used without weights       */
+    CLMSOIL_ERR_NOT_TRAINED = 8, /* learned controller used without weights       */
+    CLMSOIL_ERR_INIT = 9         /* clmsoil_init() has not run on this device      */
 };
 
 /* ---- method / boundary-condition codes (soil_water_movement.jl:23-34) --- */
@@ -114,6 +119,15 @@ typedef struct clmsoil_mf_args {
     const double *dtsub0;       /* [nc] first substep length (s); NULL = dtime
                                    (soil_water_movement.jl:1259). Used by the
                                    learned controller.                            */
+    /* Scratch of the call (the reference allocates its MfScr per call,
+     * soil_water_movement.jl:1555-1561): device memory of at least
+     * clmsoil_richards_workspace_bytes() bytes, 256-byte aligned, owned by the
+     * caller and used by one call at a time. It holds the work counters of the
+     * resident-grid launch and the call's own error flags
+     * (clmsoil_workspace_errors). NULL: the step runs as a plain launch with one
+     * column per lane group (same results, needs no scratch).              */
+    void *workspace;
+    int64_t workspace_bytes;
 } clmsoil_mf_args;
 
 /* Heat step state and forcing (soil_temperature.jl:358-593, non-urban soil
@@ -178,6 +192,12 @@ typedef struct clmsoil_heat_config {
 } clmsoil_heat_config;
 
 /* ---- library ---------------------------------------------------------- */
+/* Allocate the per-device status block on every visible GPU. Call once, before
+ * any other call (graph capture included); idempotent and thread-safe. Other
+ * entry points return CLMSOIL_ERR_INIT on a device it has not set up. */
+int         clmsoil_init(void);
+/* Bytes of the workspace clmsoil_mf_args.workspace points to. */
+int64_t     clmsoil_richards_workspace_bytes(void);
 int         clmsoil_abi_version(void);
 const char *clmsoil_strerror(int status);
 /* Number of GPUs visible to the library (0 when none). */

@@ -11,8 +11,11 @@
  *
  * The graph holds the argument values and launch shapes (clmsoil_set_tuning &
  * co.) of the moment it was created; change them, and you must create a new
- * plan. All state stays in the caller's arrays. Two launches of plans sharing
- * arrays or the device's work counters must not overlap.
+ * plan. All state stays in the caller's arrays; creating a plan allocates the
+ * 256-byte workspace of its Richards steps (mf->workspace is not used), which
+ * destroying it frees after the device is idle. Two launches of one plan, or of
+ * plans sharing arrays, must not overlap; plans on disjoint arrays may run
+ * concurrently on different streams.
  */
 #ifndef CLMSOIL_DAY_H
 #define CLMSOIL_DAY_H

@@ -23,12 +23,19 @@ extern "C" {
 #define CLMSOIL_DEVERR_SUBGRID 16u   /* ground heat flux: patch ranges and patch columns disagree */
 #define CLMSOIL_DEVERR_JWT     32u   /* methane transport: jwt outside 0..nlevsoi   */
 
-/* Synchronize `stream`, return and clear the current device's error bits. The
- * reference raises these conditions as Julia bounds errors inside its kernels. */
+/* Synchronize `stream`, return and clear the current device's error bits (the
+ * flags of every call on the device since the last read). The reference raises
+ * these conditions as Julia bounds errors inside its kernels. */
 int clmsoil_device_errors(void *stream, uint32_t *flags);
+/* Synchronize `stream` and return the error bits of the last Richards call that
+ * used `workspace` (a call clears them when it starts). */
+int clmsoil_workspace_errors(const void *workspace, void *stream, uint32_t *flags);
 
-/* Lanes per column (1, 2 or 4) and workgroup size (64, 128 or 256) of the fused
- * Richards and heat kernels; process-wide, read at launch. */
+/* Launch tuning. Lanes per column (1, 2 or 4) and workgroup size (64, 128 or
+ * 256) of the fused Richards and heat kernels; process-wide, read once per call
+ * (and fixed in a day plan when it is created). Every setting gives the same
+ * results bit for bit, so changing them while other threads launch work changes
+ * only speed. */
 int clmsoil_set_tuning(int richards_g, int richards_block, int heat_g, int heat_block);
 int clmsoil_get_tuning(int *richards_g, int *richards_block, int *heat_g, int *heat_block);
 /* 1 (default): the Richards step runs as a resident grid whose lane groups each
@@ -41,6 +48,22 @@ int clmsoil_set_richards_residency(int blocks_per_cu);
 /* Workgroup size (64, 128 — the default — or 256) of the Zeng-Decker 2009 step
  * (include/clmsoil_zd09.h). Results are identical. */
 int clmsoil_set_zd_block(int block);
+
+/* The library's scalar FP64 functions evaluated element-wise on device arrays:
+ * the kernels' pow, log, log10 and exp (Tang's table-driven method, one source
+ * shared with the CPU restatement) and Julia's NaN-propagating max/min
+ * (smooth_ad.jl:91-149 on Float64). out[i] = f(x[i], y[i]) for POW, JL_MAX and
+ * JL_MIN, f(x[i]) otherwise (y unused). No counterpart in the reference; it
+ * exists so that the device build of these functions can be checked against
+ * the host build. */
+#define CLMSOIL_MATH_POW    0
+#define CLMSOIL_MATH_LOG    1
+#define CLMSOIL_MATH_LOG10  2
+#define CLMSOIL_MATH_EXP    3
+#define CLMSOIL_MATH_JL_MAX 4
+#define CLMSOIL_MATH_JL_MIN 5
+int clmsoil_math_eval(int op, const double *x, const double *y, double *out, int64_t n,
+                      void *stream);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,9 @@
 #include "clmsoil_oracle.h"
 
 #include <math.h>
+/* The FP64 log/log10/exp/pow of the HIP library (one source for both sides:
+ * Tang's table-driven method); the pins of those functions are in the tests. */
+#include "../clm.jl_amd/csrc/clm_math.h"
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -27,6 +30,22 @@ static const double GRAV = 9.80616, CPLIQ = 4188.0, CPICE = 2117.27,
                     CSOL_BEDROCK = 2.0e6;
 static const double M_TO_MM = 1.0e3;          /* soil_water_movement.jl:37 */
 static const double THIN_SFCLAYER = 1.0e-6;   /* soil_temperature.jl:26   */
+
+/* The shared FP64 functions, element-wise over arrays (pinned by the tests). */
+double orc_jl_max(double x, double y);
+double orc_jl_min(double x, double y);
+void orc_clm_math(int op, const double *x, const double *y, double *out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        switch (op) {
+            case 0: out[i] = clm_pow(x[i], y[i]); break;
+            case 1: out[i] = clm_log(x[i]); break;
+            case 2: out[i] = clm_log10(x[i]); break;
+            case 3: out[i] = clm_exp(x[i]); break;
+            case 4: out[i] = orc_jl_max(x[i], y[i]); break;
+            default: out[i] = orc_jl_min(x[i], y[i]); break;
+        }
+    }
+}
 
 int orc_num_threads(void) {
 #ifdef _OPENMP
@@ -57,24 +76,24 @@ double orc_jl_min(double x, double y) {
 void orc_soil_hk(double hksat, double bsw, double s, double imped, double *hk,
                  double *dhkds) {
     /* :47-53  hk = imped*hksat*s^(2b+3); dhkds = (2b+3)*hk/s */
-    double h = imped * hksat * pow(s, 2.0 * bsw + 3.0);
+    double h = imped * hksat * clm_pow(s, 2.0 * bsw + 3.0);
     *hk = h;
     *dhkds = (2.0 * bsw + 3.0) * h / s;
 }
 void orc_soil_suction(double bsw, double sucsat, double s, double *smp,
                       double *dsmpds) {
     /* :80-86  smp = -sucsat*s^(-b); dsmpds = -b*smp/s */
-    double v = -sucsat * pow(s, -bsw);
+    double v = -sucsat * clm_pow(s, -bsw);
     *smp = v;
     *dsmpds = -bsw * v / s;
 }
 double orc_soil_suction_inverse(double bsw, double sucsat, double smp_target) {
     /* :113-115 */
-    return pow(-smp_target / sucsat, -1.0 / bsw);
+    return clm_pow(-smp_target / sucsat, -1.0 / bsw);
 }
 double orc_ice_impedance(double icefrac, double e_ice) {
     /* soil_water_movement.jl:722-724 */
-    return pow(10.0, -e_ice * icefrac);
+    return clm_pow(10.0, -e_ice * icefrac);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -309,6 +328,19 @@ typedef struct {
         *fluxNet0, *fluxNet1;
 } mf_scr;   /* MfScr, soil_water_movement.jl:1241-1247 (one column's rows) */
 
+/* Test hook: every substep iteration of column g_mf_trace_col appends
+ * (dtsub, errorMax, accepted) to g_mf_trace_buf (at most g_mf_trace_cap rows). */
+static int64_t g_mf_trace_col = -1;
+static double *g_mf_trace_buf = NULL;
+static int g_mf_trace_cap = 0, g_mf_trace_n = 0;
+void orc_set_mf_trace(int64_t col, double *buf, int cap) {
+    g_mf_trace_col = col;
+    g_mf_trace_buf = buf;
+    g_mf_trace_cap = cap;
+    g_mf_trace_n = 0;
+}
+int orc_mf_trace_count(void) { return g_mf_trace_n; }
+
 static void mf_column(const clmsoil_mf_args *a, const clmsoil_levels *lev, int64_t c,
                       double dtime, const clmsoil_swm_config *cfg, mf_scr *S,
                       double *dbg_qin, double *dbg_qout) {
@@ -345,19 +377,19 @@ static void mf_column(const clmsoil_mf_args *a, const clmsoil_levels *lev, int64
             double s1, imp;
             if (j == nlayers) {
                 s1 = SC(s2, j);
-                imp = pow(10.0, -e_ice * A(a->icefrac, c, j));
+                imp = clm_pow(10.0, -e_ice * A(a->icefrac, c, j));
             } else {
                 s1 = 0.5 * (SC(s2, j) + SC(s2, j + 1));
-                imp = pow(10.0, -e_ice * (0.5 * (A(a->icefrac, c, j) + A(a->icefrac, c, j + 1))));
+                imp = clm_pow(10.0, -e_ice * (0.5 * (A(a->icefrac, c, j) + A(a->icefrac, c, j + 1))));
             }
             SC(imped, j) = imp;
             s1 = JMIN(s1, 1.0);
             s1 = JMAX(0.01, s1);
             double b = A(a->bsw, c, j);
-            double hkv = imp * A(a->hksat, c, j) * pow(s1, 2.0 * b + 3.0);
+            double hkv = imp * A(a->hksat, c, j) * clm_pow(s1, 2.0 * b + 3.0);
             double dhkds = (2.0 * b + 3.0) * hkv / s1;
             double sn = SC(s2, j);
-            double smpv = -A(a->sucsat, c, j) * pow(sn, -b);
+            double smpv = -A(a->sucsat, c, j) * clm_pow(sn, -b);
             double dsmpds = -b * smpv / sn;
             SC(hk, j) = hkv;
             SC(smp, j) = smpv;
@@ -469,6 +501,12 @@ static void mf_column(const clmsoil_mf_args *a, const clmsoil_levels *lev, int64
                 A(dbg_qin, c, j) = SC(qin, j);
                 A(dbg_qout, c, j) = SC(qout, j);
             }
+        }
+        if (c == g_mf_trace_col && g_mf_trace_n < g_mf_trace_cap) {
+            double *r = g_mf_trace_buf + 3 * g_mf_trace_n++;
+            r[0] = dtsub;
+            r[1] = errorMax;
+            r[2] = (errorMax > xtu && dtsub > dtmin) ? 0.0 : 1.0;
         }
         if (errorMax > xtu && dtsub > dtmin) {             /* :1460-1463 */
             dtsub = JMAX(dtsub / 2.0, dtmin);
@@ -598,27 +636,42 @@ typedef struct {
     int64_t *imelt;
 } heat_scr;
 
-/* one column; level arrays in scratch are 1-based-offset local vectors */
-static void heat_column(const clmsoil_heat_args *a, const clmsoil_levels *lev, int64_t c,
-                        double dtime, const clmsoil_heat_config *cfg, heat_scr *S) {
-    const int64_t ld = a->ld;
-    const int nlevsno = lev->nlevsno, nlevgrnd = lev->nlevgrnd;
-    const int nlevmaxurbgrnd = nlevgrnd;   /* max(nlevurb, nlevgrnd) with nlevurb <= nlevgrnd */
-    const int joff = nlevsno;
-    const int nlev = nlevsno + nlevmaxurbgrnd;   /* snow+soil levels */
-    const int nlev_total = nlevsno + 1 + nlevmaxurbgrnd;
-    const int64_t snl = a->snl[c];
-    const int nbedrock = (int)a->nbedrock[c];
-    double *t = a->t_soisno, *liq = a->h2osoi_liq, *ice = a->h2osoi_ice;
-    double *exc = a->excess_ice;
-    const double *z = a->z, *zi = a->zi, *dz = a->dz;
-    /* local 1-based level vectors */
+/* Locals every stage of the heat column reads (nlevmaxurbgrnd = max(nlevurb,
+ * nlevgrnd) with nlevurb <= nlevgrnd). */
+#define HEAT_LOCALS                                                                   \
+    const int64_t ld = a->ld;                                                         \
+    const int nlevsno = lev->nlevsno, nlevgrnd = lev->nlevgrnd;                       \
+    const int nlevmaxurbgrnd = nlevgrnd;                                              \
+    const int joff = nlevsno;                                                         \
+    const int nlev = nlevsno + nlevmaxurbgrnd;                                        \
+    const int nlev_total = nlevsno + 1 + nlevmaxurbgrnd;                              \
+    const int64_t snl = a->snl[c];                                                    \
+    const int nbedrock = (int)a->nbedrock[c];                                         \
+    double *t = a->t_soisno, *liq = a->h2osoi_liq, *ice = a->h2osoi_ice;              \
+    double *exc = a->excess_ice;                                                      \
+    const double *z = a->z, *zi = a->zi, *dz = a->dz;                                 \
+    const double frac_sno = a->frac_sno_eff[c];                                       \
+    const double frac_h2osfc = a->frac_h2osfc[c];                                     \
+    const double dhsdT = a->dhsdT[c];                                                 \
+    const double cnfac = CNFAC;                                                       \
+    (void)ld; (void)nlevmaxurbgrnd; (void)joff; (void)nlev; (void)nlev_total;        \
+    (void)snl; (void)nbedrock; (void)t; (void)liq; (void)ice; (void)exc; (void)z;     \
+    (void)zi; (void)dz; (void)frac_sno; (void)frac_h2osfc; (void)dhsdT; (void)cnfac
+/* local 1-based level vectors */
 #define L(X, jj) S->X[(jj) - 1]
+
+/* Stage 1 of soil_temperature! (soil_temperature.jl:480-550): thermal
+ * properties, heat-diffusion factor, RHS and 5-band rows, the banded solve, the
+ * unpack and fn1. Returns c_h2osfc (the h2osfc heat capacity the later stages
+ * use) and tk_h2osfc. */
+static void heat_assemble_solve(const clmsoil_heat_args *a, const clmsoil_levels *lev,
+                                int64_t c, double dtime, const clmsoil_heat_config *cfg,
+                                heat_scr *S, double *c_h2osfc_out, double *tk_h2osfc_out) {
+    HEAT_LOCALS;
     for (int jj = 1; jj <= nlev; ++jj) {
         L(thk, jj) = NAN; L(tk, jj) = 0.0; L(cv, jj) = 0.0;
         L(fact, jj) = NAN; L(fn, jj) = 0.0; L(fn1, jj) = 0.0; L(bw, jj) = NAN;
     }
-    const double frac_sno = a->frac_sno_eff[c];
     /* ---- soil_therm_prop! (:874-942) ---- */
     /* _soil_snow_tk_kernel! (:647-716), non-special soil column, Jordan/Sturm snow */
     for (int jj = 1; jj <= nlevsno + nlevgrnd; ++jj) {
@@ -629,14 +682,14 @@ static void heat_column(const clmsoil_heat_args *a, const clmsoil_levels *lev, i
             satw = JMIN(1.0, satw);
             if (satw > 1.0e-7) {
                 double dke;
-                if (A(t, c, jj) >= TFRZ) dke = JMAX(0.0, log10(satw) + 1.0);
+                if (A(t, c, jj) >= TFRZ) dke = JMAX(0.0, clm_log10(satw) + 1.0);
                 else dke = satw;
                 double fl_denom = A(liq, c, jj) / (DENH2O * A(dz, c, jj)) +
                                   A(ice, c, jj) / (DENICE * A(dz, c, jj)) +
                                   A(exc, c, j) / (DENICE * A(dz, c, jj));
                 double fl = (A(liq, c, jj) / (DENH2O * A(dz, c, jj))) / fl_denom;
-                double dksat = A(a->tkmg, c, j) * pow(TKWAT, fl * A(a->watsat, c, j)) *
-                               pow(TKICE, (1.0 - fl) * A(a->watsat, c, j));
+                double dksat = A(a->tkmg, c, j) * clm_pow(TKWAT, fl * A(a->watsat, c, j)) *
+                               clm_pow(TKICE, (1.0 - fl) * A(a->watsat, c, j));
                 L(thk, jj) = dke * dksat + (1.0 - dke) * A(a->tkdry, c, j);
             } else {
                 L(thk, jj) = A(a->tkdry, c, j);
@@ -697,7 +750,6 @@ static void heat_column(const clmsoil_heat_args *a, const clmsoil_levels *lev, i
         }
     }
     /* ---- ground heat flux: inputs ---- */
-    const double dhsdT = a->dhsdT[c];
     const double hs_top_snow = a->hs_top_snow[c], hs_soil = a->hs_soil[c],
                  hs_h2osfc = a->hs_h2osfc[c];
     const double eflx_bot = a->eflx_bot ? a->eflx_bot[c] : 0.0;
@@ -731,8 +783,6 @@ static void heat_column(const clmsoil_heat_args *a, const clmsoil_levels *lev, i
         c_h2osfc = THIN_SFCLAYER;
         dz_h2osfc = THIN_SFCLAYER;
     }
-    const double frac_h2osfc = a->frac_h2osfc[c];
-    const double cnfac = CNFAC;
     /* ---- set_rhs_vec! (:1365-1542), vectors of length nlev_total ---- */
     for (int k = 1; k <= nlev_total; ++k) { L(rvector, k) = NAN; L(tvector, k) = NAN; }
     for (int s = 1; s <= nlevsno; ++s) {                   /* _rhs_snow_kernel! */
@@ -892,6 +942,16 @@ static void heat_column(const clmsoil_heat_args *a, const clmsoil_levels *lev, i
                 L(fn1, jj) = 0.0;
         }
     }
+    *c_h2osfc_out = c_h2osfc;
+    *tk_h2osfc_out = tk_h2osfc;
+}
+
+/* phase_change_h2osfc! (soil_temperature.jl:1757-1892) for one column, after
+ * the solve; fact_top = fact of the top snow level (joff), used when snow
+ * layers exist. */
+static void heat_pc_h2osfc(const clmsoil_heat_args *a, const clmsoil_levels *lev, int64_t c,
+                           double dtime, double c_h2osfc, double fact_top) {
+    HEAT_LOCALS;
     /* ---- phase_change_h2osfc! (:1757-1826), PHASE_CHANGE_MASS_K clamps exact ---- */
     {
         a->xmf_h2osfc[c] = 0.0;
@@ -928,8 +988,8 @@ static void heat_column(const clmsoil_heat_args *a, const clmsoil_levels *lev, i
                     eflx_h2osfc_to_snow = 0.0;
                 } else {
                     double c1;
-                    if (snl == -1) c1 = frac_sno * (dtime / L(fact, joff) - dhsdT * dtime);
-                    else c1 = frac_sno / L(fact, joff) * dtime;
+                    if (snl == -1) c1 = frac_sno * (dtime / fact_top - dhsdT * dtime);
+                    else c1 = frac_sno / fact_top * dtime;
                     double c2 = frac_h2osfc != 0.0 ? (-CPLIQ * xm - frac_h2osfc * dhsdT * dtime) : 0.0;
                     A(t, c, joff) = (c1 * A(t, c, joff) + c2 * *t_h2osfc) / (c1 + c2);
                     eflx_h2osfc_to_snow = (*t_h2osfc - A(t, c, joff)) * c2 / dtime;
@@ -946,12 +1006,12 @@ static void heat_column(const clmsoil_heat_args *a, const clmsoil_levels *lev, i
                 if (snl == 0) {
                     A(t, c, joff) = *t_h2osfc;
                 } else if (snl == -1) {
-                    double c1 = frac_sno * (dtime / L(fact, joff) - dhsdT * dtime);
+                    double c1 = frac_sno * (dtime / fact_top - dhsdT * dtime);
                     double c2 = frac_h2osfc != 0.0 ? frac_h2osfc * (c_h2osfc - dtime * dhsdT) : 0.0;
                     A(t, c, joff) = (c1 * A(t, c, joff) + c2 * *t_h2osfc) / (c1 + c2);
                     *t_h2osfc = A(t, c, joff);
                 } else {
-                    double c1 = frac_sno / L(fact, joff) * dtime;
+                    double c1 = frac_sno / fact_top * dtime;
                     double c2 = frac_h2osfc != 0.0 ? frac_h2osfc * (c_h2osfc - dtime * dhsdT) : 0.0;
                     A(t, c, joff) = (c1 * A(t, c, joff) + c2 * *t_h2osfc) / (c1 + c2);
                     *t_h2osfc = A(t, c, joff);
@@ -963,6 +1023,13 @@ static void heat_column(const clmsoil_heat_args *a, const clmsoil_levels *lev, i
         }
         if (a->eflx_h2osfc_to_snow) a->eflx_h2osfc_to_snow[c] = eflx_h2osfc_to_snow;
     }
+}
+
+/* phase_change_beta! (soil_temperature.jl:1902-2289) for one column, with the
+ * heat-diffusion factor fact in S. */
+static void heat_pc_beta(const clmsoil_heat_args *a, const clmsoil_levels *lev, int64_t c,
+                         double dtime, heat_scr *S) {
+    HEAT_LOCALS;
     /* ---- phase_change_beta! (:1929-2289), hard (Float64) branch ---- */
     {
         const double tfrz = TFRZ, hfus = HFUS, grav = GRAV, thou = 1000.0;
@@ -1011,7 +1078,7 @@ static void heat_column(const clmsoil_heat_args *a, const clmsoil_levels *lev, i
                 if (A(t, c, jj) < tfrz) {
                     double smp = hfus * (tfrz - A(t, c, jj)) / (grav * A(t, c, jj)) * thou;
                     S->supercool[j - 1] = A(a->watsat, c, j) *
-                                          pow(smp / A(a->sucsat, c, j), -1.0 / A(a->bsw, c, j));
+                                          clm_pow(smp / A(a->sucsat, c, j), -1.0 / A(a->bsw, c, j));
                     S->supercool[j - 1] *= A(dz, c, jj) * thou;
                 }
                 if (A(liq, c, jj) > S->supercool[j - 1] && A(t, c, jj) < tfrz) {
@@ -1139,6 +1206,13 @@ static void heat_column(const clmsoil_heat_args *a, const clmsoil_levels *lev, i
             for (int jj = 1; jj <= nlev; ++jj)
                 if (jj - nlevsno >= snl + 1) A(a->imelt, c, jj) = L(imelt, jj);
     }
+}
+
+/* t_grnd, the soil heat flux diagnostics and the optional copies of the
+ * thermal-property scratch (soil_temperature.jl:584-590, 224-269). */
+static void heat_finish(const clmsoil_heat_args *a, const clmsoil_levels *lev, int64_t c,
+                        double c_h2osfc, heat_scr *S) {
+    HEAT_LOCALS;
     /* ---- _tgrnd_kernel! (:224-247) ---- */
     {
         double tg;
@@ -1179,8 +1253,91 @@ static void heat_column(const clmsoil_heat_args *a, const clmsoil_levels *lev, i
         for (int jj = 1; jj <= nlevsno; ++jj)
             if (jj - nlevsno >= snl + 1) A(a->bw, c, jj) = L(bw, jj);
     if (a->c_h2osfc) a->c_h2osfc[c] = c_h2osfc;
+}
+
+
+/* one column: the stages in the reference's order */
+static void heat_column(const clmsoil_heat_args *a, const clmsoil_levels *lev, int64_t c,
+                        double dtime, const clmsoil_heat_config *cfg, heat_scr *S) {
+    double c_h2osfc, tk_h2osfc;
+    heat_assemble_solve(a, lev, c, dtime, cfg, S, &c_h2osfc, &tk_h2osfc);
+    heat_pc_h2osfc(a, lev, c, dtime, c_h2osfc, S->fact[lev->nlevsno - 1]);
+    heat_pc_beta(a, lev, c, dtime, S);
+    heat_finish(a, lev, c, c_h2osfc, S);
+}
 #undef L
 #undef SABG
+#undef HEAT_LOCALS
+
+static void heat_scr_alloc(heat_scr *S, int nlev_total) {
+    size_t nd = (size_t)nlev_total;
+    double *buf = (double *)calloc(nd * (22 + 5) + nd * nd, sizeof(double));
+    double **f = (double **)S;
+    for (int k = 0; k < 9; ++k) f[k] = buf + (size_t)k * nd;          /* thk..tvector */
+    S->bmatrix = buf + 9 * nd;                                          /* 5*nd */
+    S->Am = buf + 14 * nd;                                              /* nd*nd */
+    double *rest = S->Am + nd * nd;
+    S->x = rest; rest += nd;
+    S->hm = rest; rest += nd;
+    S->xm = rest; rest += nd;
+    S->xm2 = rest; rest += nd;
+    S->wice0 = rest; rest += nd;
+    S->wliq0 = rest; rest += nd;
+    S->wexice0 = rest; rest += nd;
+    S->wmass0 = rest; rest += nd;
+    S->tinc = rest; rest += nd;
+    S->supercool = rest; rest += nd;
+    S->imelt = (int64_t *)calloc(nd, sizeof(int64_t));
+}
+static void heat_scr_free(heat_scr *S) {
+    free(S->imelt);
+    free(S->thk);   /* the buffer starts at thk */
+}
+
+/* Test entries: the stages of one heat column on their own, as the reference's
+ * tests call soil_therm_prop!, set_matrix!, phase_change_h2osfc! and
+ * phase_change_beta! (test/test_soil_temperature.jl). */
+int orc_heat_assemble_probe(const clmsoil_heat_args *a, const clmsoil_levels *lev,
+                            double dtime, const clmsoil_heat_config *cfg, int64_t c,
+                            double *tk, double *cv, double *bmatrix, double *rvector,
+                            double *tk_h2osfc) {
+    int st = validate_heat(a, lev);
+    if (st) return st;
+    if (!cfg || c < 0 || c >= a->nc) return CLMSOIL_ERR_ARG;
+    const int nlev = lev->nlevsno + lev->nlevgrnd, nlev_total = nlev + 1;
+    heat_scr S;
+    heat_scr_alloc(&S, nlev_total);
+    double c_h2osfc, tkh;
+    heat_assemble_solve(a, lev, c, dtime, cfg, &S, &c_h2osfc, &tkh);
+    for (int k = 0; k < nlev; ++k) { tk[k] = S.tk[k]; cv[k] = S.cv[k]; }
+    for (int k = 0; k < 5 * nlev_total; ++k) bmatrix[k] = S.bmatrix[k];
+    for (int k = 0; k < nlev_total; ++k) rvector[k] = S.rvector[k];
+    *tk_h2osfc = tkh;
+    heat_scr_free(&S);
+    return CLMSOIL_OK;
+}
+
+int orc_phase_change_h2osfc(const clmsoil_heat_args *a, const clmsoil_levels *lev,
+                            double dtime, int64_t c, double c_h2osfc, double fact_top) {
+    int st = validate_heat(a, lev);
+    if (st) return st;
+    if (c < 0 || c >= a->nc) return CLMSOIL_ERR_ARG;
+    heat_pc_h2osfc(a, lev, c, dtime, c_h2osfc, fact_top);
+    return CLMSOIL_OK;
+}
+
+int orc_phase_change_beta(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime,
+                          int64_t c, const double *fact) {
+    int st = validate_heat(a, lev);
+    if (st) return st;
+    if (c < 0 || c >= a->nc || !fact) return CLMSOIL_ERR_ARG;
+    const int nlev = lev->nlevsno + lev->nlevgrnd;
+    heat_scr S;
+    heat_scr_alloc(&S, nlev + 1);
+    for (int k = 0; k < nlev; ++k) S.fact[k] = fact[k];
+    heat_pc_beta(a, lev, c, dtime, &S);
+    heat_scr_free(&S);
+    return CLMSOIL_OK;
 }
 
 int orc_soil_temperature(const clmsoil_heat_args *a, const clmsoil_levels *lev,
@@ -1189,7 +1346,7 @@ int orc_soil_temperature(const clmsoil_heat_args *a, const clmsoil_levels *lev,
     if (st) return st;
     if (!cfg) return CLMSOIL_ERR_NULL;
     const int nlevsno = lev->nlevsno, nlevgrnd = lev->nlevgrnd;
-    const int nlev = nlevsno + nlevgrnd, nlev_total = nlev + 1;
+    const int nlev_total = nlevsno + nlevgrnd + 1;
     for (int64_t c = 0; c < a->nc; ++c) {
         if (a->mask && !a->mask[c]) continue;
         if (a->snl[c] < -nlevsno || a->snl[c] > 0) return CLMSOIL_ERR_SIZE;
@@ -1197,31 +1354,13 @@ int orc_soil_temperature(const clmsoil_heat_args *a, const clmsoil_levels *lev,
 #pragma omp parallel
     {
         heat_scr S;
-        size_t nd = (size_t)nlev_total;
-        double *buf = (double *)calloc(nd * (22 + 5) + nd * nd, sizeof(double));
-        double **f = (double **)&S;
-        for (int k = 0; k < 9; ++k) f[k] = buf + (size_t)k * nd;          /* thk..tvector */
-        S.bmatrix = buf + 9 * nd;                                           /* 5*nd */
-        S.Am = buf + 14 * nd;                                               /* nd*nd */
-        double *rest = S.Am + nd * nd;
-        S.x = rest; rest += nd;
-        S.hm = rest; rest += nd;
-        S.xm = rest; rest += nd;
-        S.xm2 = rest; rest += nd;
-        S.wice0 = rest; rest += nd;
-        S.wliq0 = rest; rest += nd;
-        S.wexice0 = rest; rest += nd;
-        S.wmass0 = rest; rest += nd;
-        S.tinc = rest; rest += nd;
-        S.supercool = rest; rest += nd;
-        S.imelt = (int64_t *)calloc(nd, sizeof(int64_t));
+        heat_scr_alloc(&S, nlev_total);
 #pragma omp for schedule(static)
         for (int64_t c = 0; c < a->nc; ++c) {
             if (a->mask && !a->mask[c]) continue;
             heat_column(a, lev, c, dtime, cfg, &S);
         }
-        free(S.imelt);
-        free(buf);
+        heat_scr_free(&S);
     }
     return CLMSOIL_OK;
 }

@@ -80,6 +80,22 @@ int orc_soilwater_moisture_form(const clmsoil_mf_args *a, const clmsoil_levels *
 /* Non-urban soil_temperature!, soil_temperature.jl:358-593. */
 int orc_soil_temperature(const clmsoil_heat_args *a, const clmsoil_levels *lev,
                          double dtime, const clmsoil_heat_config *cfg);
+/* Stages of one heat column (0-based column c), as the reference's tests call
+ * them. orc_heat_assemble_probe runs soil_therm_prop! through the solve and
+ * fn1 (t_soisno of column c is updated) and copies out the interface
+ * conductivity tk and heat capacity cv [nlevsno+nlevgrnd], the band rows
+ * bmatrix[(b-1) + 5*(row-1)] and rvector [nlevsno+1+nlevgrnd], and tk_h2osfc.
+ * orc_phase_change_h2osfc / orc_phase_change_beta run phase_change_h2osfc!
+ * (:1757-1892) / phase_change_beta! (:1902-2289) alone, with the given
+ * c_h2osfc and top-snow fact / the column's fact [nlevsno+nlevgrnd]. */
+int orc_heat_assemble_probe(const clmsoil_heat_args *a, const clmsoil_levels *lev,
+                            double dtime, const clmsoil_heat_config *cfg, int64_t c,
+                            double *tk, double *cv, double *bmatrix, double *rvector,
+                            double *tk_h2osfc);
+int orc_phase_change_h2osfc(const clmsoil_heat_args *a, const clmsoil_levels *lev,
+                            double dtime, int64_t c, double c_h2osfc, double fact_top);
+int orc_phase_change_beta(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime,
+                          int64_t c, const double *fact);
 
 /* Julia's x^4 for Float64 (Base.Math.pow_body: compensated power by squaring). */
 double orc_jl_pow4(double x);
@@ -144,7 +160,15 @@ int orc_soilwater_zengdecker2009(const clmsoil_zd_args *a, const clmsoil_levels 
                                  double dtime, const clmsoil_swm_config *cfg);
 
 /* Number of OpenMP threads the batched functions use. */
+/* op 0: clm_pow(x, y), 1: clm_log(x), 2: clm_log10(x), 3: clm_exp(x),
+ * 4: Julia max(x, y), 5: Julia min(x, y) */
+void orc_clm_math(int op, const double *x, const double *y, double *out, int64_t n);
 int orc_num_threads(void);
+/* Test hook of the moisture-form step: each substep iteration of column `col`
+ * (0-based; -1 = off) appends (dtsub, errorMax, accepted) to buf[3*k..3*k+2],
+ * at most `cap` rows; orc_mf_trace_count returns how many were written. */
+void orc_set_mf_trace(int64_t col, double *buf, int cap);
+int orc_mf_trace_count(void);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,15 @@ def lib() -> C.CDLL:
         L.orc_compute_lhs_moisture_form.argtypes = [C.c_int] + [C.c_void_p] * 8
         L.orc_tridiagonal_solve.argtypes = [C.c_void_p] * 5 + [C.c_int] * 2
         L.orc_num_threads.restype = C.c_int
+        L.orc_clm_math.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_set_mf_trace.argtypes = [C.c_int64, C.c_void_p, C.c_int]
+        L.orc_mf_trace_count.restype = C.c_int
+        L.orc_heat_assemble_probe.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                              C.c_int64] + [C.c_void_p] * 5
+        L.orc_phase_change_h2osfc.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int64,
+                                              C.c_double, C.c_double]
+        L.orc_phase_change_beta.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int64,
+                                            C.c_void_p]
         _lib = L
     return _lib
 
@@ -267,3 +276,62 @@ def jl_pow5(x: float) -> float:
     L.orc_jl_pow5.argtypes = [C.c_double]
     L.orc_jl_pow5.restype = C.c_double
     return float(L.orc_jl_pow5(float(x)))
+
+
+_MATH_OPS = {"pow": 0, "log": 1, "log10": 2, "exp": 3, "jl_max": 4, "jl_min": 5}
+
+
+def clm_math(op: str, x, y=None):
+    """The library's FP64 pow/log/log10/exp (clm_math.h, compiled into the
+    oracle) and Julia's max/min element-wise over float64 arrays."""
+    import numpy as np
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(x if y is None else y, dtype=np.float64)
+    out = np.empty_like(x)
+    lib().orc_clm_math(_MATH_OPS[op], x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
+    return out
+
+
+def mf_trace(d: dict, cfg, col: int, cap: int = 4096):
+    """Run the moisture-form step on `d` (in place) and return the substep
+    iterations of column `col` as rows (dtsub, errorMax, accepted)."""
+    import numpy as np
+    buf = np.zeros((cap, 3))
+    L = lib()
+    L.orc_set_mf_trace(int(col), buf.ctypes.data, int(cap))
+    try:
+        soilwm_moisture_form_solve(d, cfg)
+        n = L.orc_mf_trace_count()
+    finally:
+        L.orc_set_mf_trace(-1, None, 0)
+    return buf[:n].copy()
+
+
+def heat_assemble_probe(d: dict, hcfg, col: int):
+    """soil_therm_prop! through the banded solve for column `col` of `d`:
+    returns tk, cv [nlev], bmatrix [nlev_total, 5] (row, band), rvector and
+    tk_h2osfc. t_soisno of the column is updated in place."""
+    import numpy as np
+    nlev = d["nlevsno"] + d["nlevgrnd"]
+    tk, cv = np.zeros(nlev), np.zeros(nlev)
+    bm, rv = np.zeros((nlev + 1, 5)), np.zeros(nlev + 1)
+    tkh = C.c_double(0.0)
+    a, lev = heat_args(d), levels(d)
+    check(lib().orc_heat_assemble_probe(byref(a), byref(lev), float(d["dtime"]), byref(hcfg),
+                                        int(col), tk.ctypes.data, cv.ctypes.data, bm.ctypes.data,
+                                        rv.ctypes.data, C.addressof(tkh)), "heat_assemble_probe")
+    return dict(tk=tk, cv=cv, bmatrix=bm, rvector=rv, tk_h2osfc=tkh.value)
+
+
+def phase_change_h2osfc(d: dict, col: int, c_h2osfc: float, fact_top: float) -> None:
+    a, lev = heat_args(d), levels(d)
+    check(lib().orc_phase_change_h2osfc(byref(a), byref(lev), float(d["dtime"]), int(col),
+                                        float(c_h2osfc), float(fact_top)), "phase_change_h2osfc")
+
+
+def phase_change_beta(d: dict, col: int, fact) -> None:
+    import numpy as np
+    f = np.ascontiguousarray(fact, dtype=np.float64)
+    a, lev = heat_args(d), levels(d)
+    check(lib().orc_phase_change_beta(byref(a), byref(lev), float(d["dtime"]), int(col),
+                                      f.ctypes.data), "phase_change_beta")

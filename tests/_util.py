@@ -24,5 +24,29 @@ def rel_err(a, b, floor=0.0):
     return float(r.max())
 
 
+def bitwise_mismatch(a, b):
+    """Number of elements whose float64 bits differ (NaN payloads aside: any
+    NaN equals any NaN) between two arrays of the same shape."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.shape != b.shape:
+        return -1
+    same = (a.view(np.int64) == b.view(np.int64)) | (np.isnan(a) & np.isnan(b))
+    return int(np.count_nonzero(~same))
+
+
+def assert_bitwise(got, ref, keys):
+    """Every listed output equal bit for bit; the message names the fields that
+    differ with their count of differing elements and largest relative error."""
+    bad = {}
+    for k in keys:
+        if k not in ref:
+            continue
+        n = bitwise_mismatch(got[k], ref[k])
+        if n:
+            bad[k] = (n, rel_err(got[k], ref[k]))
+    assert not bad, bad
+
+
 def clone(d):
     return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in d.items()}

@@ -31,13 +31,15 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_strerror():
     lib = clmsoil.load_library()
-    assert lib.clmsoil_abi_version() == 1
+    assert lib.clmsoil_abi_version() == 2
     assert lib.clmsoil_strerror(2) == b"unsupported boundary condition"
+    assert lib.clmsoil_strerror(_abi.ERR_INIT) == b"clmsoil_init() has not set up this device"
+    assert lib.clmsoil_richards_workspace_bytes() == 256
 
 
 def test_struct_sizes_match_header():
     # 2 int64 + pointer fields, 8-byte aligned
-    assert C.sizeof(_abi.MfArgs) == 16 + 8 * len(_abi.MF_FIELDS)
+    assert C.sizeof(_abi.MfArgs) == 16 + 8 * len(_abi.MF_FIELDS) + 16
     assert C.sizeof(_abi.HeatArgs) == 16 + 8 * len(_abi.HEAT_FIELDS)
     assert C.sizeof(_abi.SwmConfig) == 16 + 5 * 8
     assert C.sizeof(_abi.Levels) == 16

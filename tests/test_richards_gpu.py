@@ -1,15 +1,17 @@
 """HIP moisture-form Richards step vs the oracle (same seeded inputs).
 
-Bar: <= 1e-11 relative on h2osoi_liq, smp_l, hk_l, qcharge (|a-b|/max(|a|,|b|));
-nsubsteps identical (a differing accept/reject decision would show here)."""
+Bar: h2osoi_liq, smp_l, hk_l and qcharge equal to the oracle's bit for bit
+(both sides evaluate pow with clm_math.h, pinned in test_clm_math.py, and
+neither fuses a multiply-add), so the north star's 1e-11 relative holds with
+nothing to spare; nsubsteps identical (a differing accept/reject decision would
+show here)."""
 import numpy as np
 import pytest
 
 import oracle
-from _util import clone, rel_err
+from _util import assert_bitwise, clone
 from clmsoil import api, bind, inputs
 
-TOL = 1e-11
 NS = 12
 
 
@@ -33,11 +35,7 @@ def run_gpu(d, cfg: api.SoilWaterMovementConfig, aquifer=False):
 
 
 def compare(got, ref, d):
-    ns, n = d["nlevsno"], d["nlevsoi"]
-    assert rel_err(got["h2osoi_liq"], ref["h2osoi_liq"]) <= TOL
-    assert rel_err(got["smp_l"], ref["smp_l"]) <= TOL
-    assert rel_err(got["hk_l"], ref["hk_l"]) <= TOL
-    assert rel_err(got["qcharge"], ref["qcharge"]) <= TOL
+    assert_bitwise(got, ref, ("h2osoi_liq", "smp_l", "hk_l", "qcharge"))
     assert np.array_equal(got["nsubsteps"], ref["nsubsteps"], equal_nan=True)
 
 
@@ -164,3 +162,92 @@ def test_water_balance_1m_columns(gpu, bc):
     oracle.soilwm_moisture_form_solve(ref, cfg.to_abi())
     compare({k: got[k][..., idx] for k in ("h2osoi_liq", "smp_l", "hk_l", "qcharge", "nsubsteps")},
             ref, d)
+
+
+@pytest.mark.gpu
+def test_nan_and_signed_zero_inputs(gpu):
+    """NaN inputs propagate as in the reference (its kernels never raise) and
+    -0.0 is an ordinary value: columns with a NaN layer, a NaN infiltration, a
+    NaN Clapp-Hornberger exponent and -0.0 liquid water come out of the HIP step
+    exactly as out of the oracle (NaN where the oracle has NaN)."""
+    d = inputs.richards_config0(nc=1024, seed=25)
+    ns = d["nlevsno"]
+    d["h2osoi_liq"][ns + 2, 5] = np.nan
+    d["qflx_infl"][7] = np.nan
+    d["bsw"][4, 9] = np.nan
+    d["h2osoi_liq"][ns + 1, 11] = -0.0
+    d["h2osoi_liq"][ns:, 13] = -0.0
+    d["nrejected"] = np.full(d["nc"], np.nan)
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_FLUX)
+    ref = clone(d)
+    oracle.soilwm_moisture_form_solve(ref, cfg.to_abi())
+    assert np.isnan(ref["h2osoi_liq"][ns:, 5]).any() and np.isnan(ref["qcharge"][7])
+    for G in (1, 4):
+        api.set_tuning(G, 256, 4, 256)
+        got = run_gpu(clone(d), cfg)
+        compare(got, ref, d)
+        assert np.array_equal(got["nrejected"], ref["nrejected"], equal_nan=True)
+
+
+def column_block_args(a_full, h, ncb):
+    """clmsoil_mf_args of columns h*ncb .. (h+1)*ncb-1 of the arrays a_full
+    points to: pointers moved by the block's first column, leading dimension
+    kept (the arrays are column-major [c, j])."""
+    import ctypes as C
+    from clmsoil import _abi
+    b = _abi.MfArgs()
+    C.memmove(C.byref(b), C.byref(a_full), C.sizeof(_abi.MfArgs))
+    b.nc = ncb
+    for f in _abi.MF_FIELDS:
+        p = getattr(a_full, f)
+        if p:
+            setattr(b, f, p + h * ncb * (1 if f == "mask" else 8))
+    return b
+
+
+@pytest.mark.gpu
+def test_two_streams_disjoint_column_blocks(gpu):
+    """Re-entrancy: two streams of one GPU run the resident-grid Richards step at
+    the same time on disjoint column blocks of the same arrays, each call with its
+    own workspace; both blocks match the oracle bit for bit and neither call
+    reports an error."""
+    import ctypes as C
+    import torch
+    from clmsoil import _abi
+    nc = 1 << 17
+    d = inputs.richards_config0(nc=nc, seed=26, nbedrock="random", lower_bc=api.BC_WATERTABLE)
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_WATERTABLE)
+    ref = clone(d)
+    oracle.soilwm_moisture_form_solve(ref, cfg.to_abi())
+    t = api.to_device(d)
+    api.set_tuning(4, 256, 4, 256)
+    api.set_richards_schedule(True)
+    L = clmsoil_lib()
+    full = bind.mf_args(t)
+    lev = bind.levels(d)
+    c = cfg.to_abi()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    ws = [api.richards_workspace(), api.richards_workspace()]
+    torch.cuda.synchronize()
+    half = nc // 2
+    for h in range(2):
+        b = column_block_args(full, h, half)
+        b.workspace = ws[h].data_ptr()
+        b.workspace_bytes = ws[h].numel()
+        st = L.clmsoil_soilwm_moisture_form_solve(C.byref(b), C.byref(lev), d["dtime"], C.byref(c),
+                                                  C.c_void_p(streams[h].cuda_stream))
+        assert st == _abi.OK
+    torch.cuda.synchronize()
+    for h in range(2):
+        with torch.cuda.stream(streams[h]):
+            assert api.workspace_errors(ws[h]) == 0
+    assert api.device_errors() == 0
+    got = api.to_host(t)
+    compare(got, ref, d)
+
+
+def clmsoil_lib():
+    import clmsoil
+    return clmsoil.load_library()
