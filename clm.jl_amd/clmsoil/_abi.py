@@ -328,6 +328,9 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
     lib.clmsoil_soilwm_moisture_form_solve.argtypes = [C.POINTER(MfArgs), C.POINTER(Levels),
                                                        C.c_double, C.POINTER(SwmConfig), C.c_void_p]
     lib.clmsoil_soilwm_moisture_form_solve.restype = C.c_int
+    lib.clmsoil_soilwm_moisture_form_solve_static.argtypes = [
+        C.POINTER(MfArgs), C.POINTER(Levels), C.c_double, C.POINTER(SwmConfig), C.c_void_p]
+    lib.clmsoil_soilwm_moisture_form_solve_static.restype = C.c_int
     _declare_extended(lib)
     # the library's one-time device set-up (its only allocation); no-op without a GPU
     check(lib, lib.clmsoil_init(), "init")

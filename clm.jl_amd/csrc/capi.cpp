@@ -42,6 +42,14 @@ namespace clmsoil {
 
 Tuning g_tuning;   // set by clmsoil_set_tuning (the autotuner), read at launch
 
+// Scratch of clmsoil_soilwm_moisture_form_solve_static, the entry point for
+// callers that cannot allocate: one Richards workspace (the call's error word
+// and the per-XCD work counters) per GPU, held until the process ends. Every
+// call on a GPU uses the same array, so calls that overlap on one GPU race.
+// Host code must not pass this name to a kernel (it is a host-side stand-in):
+// clmsoil_init() resolves each device's copy with hipGetSymbolAddress.
+static __device__ __align__(256) unsigned scratch[RICHARDS_WORKSPACE_BYTES / sizeof(unsigned)];
+
 namespace {
 std::mutex g_mu;                     // serialises clmsoil_init
 std::vector<DeviceState> g_dev;      // one entry per device, filled by clmsoil_init
@@ -68,6 +76,12 @@ int init_devices() {
             break;
         }
         dev[(size_t)d].err = static_cast<unsigned *>(p);
+        void *sp = nullptr;
+        if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) != hipSuccess || !sp) {
+            st = CLMSOIL_ERR_HIP;
+            break;
+        }
+        dev[(size_t)d].scratch = static_cast<unsigned *>(sp);
     }
     if (n > 0 && hipSetDevice(cur) != hipSuccess) st = CLMSOIL_ERR_HIP;
     if (st != CLMSOIL_OK) {
@@ -317,6 +331,21 @@ int clmsoil_soilwm_moisture_form_solve(const clmsoil_mf_args *a, const clmsoil_l
     DeviceState *s = nullptr;
     if ((st = current_device_state(&s))) return st;
     RichardsParams P = richards_params(a, lev, dtime, cfg, false, s->err);
+    return run_richards(P, (hipStream_t)stream);
+}
+
+int clmsoil_soilwm_moisture_form_solve_static(const clmsoil_mf_args *a,
+                                              const clmsoil_levels *lev, double dtime,
+                                              const clmsoil_swm_config *cfg, void *stream) {
+    int st = validate_mf(a, lev, dtime, cfg);
+    if (st || a->nc == 0) return st;
+    DeviceState *s = nullptr;
+    if ((st = current_device_state(&s))) return st;
+    // the static scratch stands in for the caller's workspace
+    clmsoil_mf_args m = *a;
+    m.workspace = s->scratch;
+    m.workspace_bytes = RICHARDS_WORKSPACE_BYTES;
+    RichardsParams P = richards_params(&m, lev, dtime, cfg, false, s->err);
     return run_richards(P, (hipStream_t)stream);
 }
 

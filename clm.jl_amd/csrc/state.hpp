@@ -24,9 +24,11 @@ struct Tuning {
 extern Tuning g_tuning;
 
 // Set up once per device by clmsoil_init(): the 256-byte status block whose
-// word 0 holds the device's error bits.
+// word 0 holds the device's error bits, and the device address of the static
+// scratch of clmsoil_soilwm_moisture_form_solve_static.
 struct DeviceState {
     unsigned *err = nullptr;
+    unsigned *scratch = nullptr;
 };
 
 // Bytes of a Richards call's workspace: word 0 the call's error bits, words

@@ -213,6 +213,19 @@ int clmsoil_soilwm_moisture_form_solve(const clmsoil_mf_args *a,
                                        const clmsoil_levels *lev, double dtime,
                                        const clmsoil_swm_config *cfg, void *stream);
 
+/* The same step for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc): no counterpart in
+ * the reference. Results equal clmsoil_soilwm_moisture_form_solve's. Instead of
+ * a->workspace (ignored) it uses a 256-byte scratch the library keeps in
+ * device memory, one per GPU, set up by clmsoil_init(): the resident-grid work
+ * counters (32-bit, so nc <= 2^34; larger nc returns CLMSOIL_ERR_SIZE) and the
+ * call's error word. It allocates nothing. All calls on one GPU share that
+ * scratch: calls that overlap on a GPU race (serialise them, e.g. on one
+ * stream). */
+int clmsoil_soilwm_moisture_form_solve_static(const clmsoil_mf_args *a,
+                                              const clmsoil_levels *lev, double dtime,
+                                              const clmsoil_swm_config *cfg, void *stream);
+
 /* Replaces soilwater_moisture_form! (soil_water_movement.jl:1532-1601): the
  * solve above followed, when the lower BC uses an aquifer layer (WATERTABLE or
  * AQUIFER, :700-703), by the recharge pass `compute_qcharge!` (:1153-1217) with
