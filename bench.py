@@ -23,8 +23,15 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch at 1,048,576 columns (seed 2), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE counters, each scaled by what the counter reads for a plain 2 GiB
 # device copy (fetch x2.0, write x1.0); recorded in
-# profiles/hbm_traffic_production_day_1M.md.
-MEASURED_TRAFFIC_1M = {"heat": 7.956e9, "richards": 4.243e9}
+# profiles/hbm_traffic_production_day_1M.md. FP64 FLOP per launch (fma = 2) from
+# the SQ_INSTS_VALU_*_F64 counters of the same kernels, recorded in
+# profiles/sq_counters_production_day_1M.md. Both belong to the kernel sources of
+# commit PROFILED_AT (the bench line names it); a later kernel change needs a new
+# profile.
+MEASURED_TRAFFIC_1M = {"heat": 8.494e9, "richards": 4.188e9}
+FP64_FLOP_1M = {"heat": 3.42e10, "richards": 4.19e10}
+PROFILED_AT = "866cf32"
+FP64_PEAK_TFLOPS = 78.6
 METRIC = "column-steps/s @1M cols FP64 soil solves; HBM GB/s frac; learned vs plain dt"
 
 
@@ -78,9 +85,25 @@ HEAT_KEYS = ["snl", "nbedrock", "z", "dz", "zi", "t_soisno", "h2osoi_liq", "h2os
              "qflx_snofrz_lyr"]
 
 
-def cpu_baseline(host, sample, steps):
-    """The oracle (a C restatement of CLM.jl's CPU path, OpenMP over columns) on
-    the first `sample` columns for `steps` steps, best of 2."""
+def host_cpus():
+    """CPUs this process may run on (nproc) and the CPU model (lscpu's)."""
+    n = len(os.sched_getaffinity(0))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, model
+
+
+def cpu_baseline(host, sample, steps, reps=5):
+    """The oracle (a C restatement of CLM.jl's CPU path, OpenMP over columns on
+    OMP_NUM_THREADS threads, -O3) on the first `sample` columns for `steps`
+    steps of the production day: one warm-up pass, then the best of `reps`."""
     import numpy as np
     sys.path.insert(0, str(REPO / "oracle"))
     import oracle
@@ -95,14 +118,56 @@ def cpu_baseline(host, sample, steps):
     cfg = bind.swm_config(lower_bc=3)
     hcfg = bind.heat_config()
     best = float("inf")
-    for _ in range(2):
+    for rep in range(reps + 1):
         s = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in sub.items()}
         t0 = time.perf_counter()
         for _ in range(steps):
             oracle.soil_temperature(s, hcfg)
             oracle.soilwater_moisture_form(s, cfg)
-        best = min(best, time.perf_counter() - t0)
+        if rep:                                  # pass 0 is the warm-up
+            best = min(best, time.perf_counter() - t0)
     return sample * steps / best, oracle.num_threads(), best
+
+
+def zero_flux_day(dev, snap, nc, steps):
+    """The same production day with CLM.jl's live lower boundary (moisture form
+    with BC_ZERO_FLUX, clm_driver.jl:2043-2047, which accepts ragged bedrock):
+    from the same warmed-up state, the day as one graph launch, then the same
+    steps launched one by one with HIP events around each kernel."""
+    import torch
+    from clmsoil import api, day
+    for k, v in snap.items():
+        dev[k].copy_(v)
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_ZERO_FLUX, use_bedrock=True)
+    st = api.MfState(dev["h2osoi_liq"], dev["smp_l"], dev["hk_l"], dev["qcharge"], dev["nsubsteps"])
+    mi = api.MfIn(dev["z"], dev["zi"], dev["dz"], dev["qflx_rootsoi"], dev["watsat"], dev["hksat"],
+                  dev["bsw"], dev["sucsat"], dev["icefrac"], dev["zwt"], dev["qflx_infl"],
+                  dev["smpmin"])
+    plan = day.DayPlan({k: dev[k] for k in HEAT_KEYS} | {"nc": nc}, st, mi, None, dev["nbedrock"],
+                       12, 20, 25, 1800.0, cfg, steps)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    plan.launch()
+    b.record()
+    torch.cuda.synchronize()
+    plan.close()
+    day_ms = a.elapsed_time(b)
+    for k, v in snap.items():
+        dev[k].copy_(v)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    for e in ev:
+        gpu_step(dev, nc, cfg, HEAT_KEYS, e)
+    torch.cuda.synchronize()
+    assert api.device_errors() == 0
+    assert bool((dev["qcharge"] == 0.0).all())
+    return {"workload": f"{nc} columns, the production day's columns with BC_ZERO_FLUX "
+                        f"(use_bedrock), {steps} x 1800 s steps of heat + Richards",
+            "column_steps_per_s": nc * steps / (day_ms / 1e3), "ms_per_step": day_ms / steps,
+            "kernels_ms": {"heat": sum(e[0].elapsed_time(e[1]) for e in ev) / steps,
+                           "richards": sum(e[1].elapsed_time(e[2]) for e in ev) / steps},
+            "mean_substep_iterations": dev["nsubsteps"].float().mean().item()}
 
 
 def learned_vs_plain(nc, steps):
@@ -113,10 +178,43 @@ def learned_vs_plain(nc, steps):
     cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
                                       lower_boundary_condition=api.BC_ZERO_FLUX)
     pred = learned.DtPredictor.load()
-    r = learned.compare_controllers(pred, nc, steps, 2024, cfg)
+    r = learned.compare_controllers(pred, nc, steps, 2024, cfg, hindsight=True)
     r["workload"] = (f"{nc} config-0 columns x {steps} steps, Richards only (BC_ZERO_FLUX), "
-                     f"prediction kernel included in the learned timing")
+                     f"prediction kernel included in the learned timing; 'hindsight' runs each "
+                     f"column-step with the best of the 12 candidate first substeps, found by "
+                     f"trying them all first (untimed): the ceiling of any first-substep "
+                     f"predictor under the unchanged controller")
     return r
+
+
+def calibration_section(nc=262144, days=30, fd_columns=64, fd_days=30):
+    """The differentiable-calibration workload at its size: 262,144 columns x 30
+    days (1,440 steps) of heat + Richards (BC_FLUX), the loss of daily drainage
+    against a target from perturbed parameters, and its gradient w.r.t. per-column
+    multipliers of b, ksat and porosity through the hand-written adjoint with one
+    checkpoint per day; plus the adjoint against central differences (relative
+    step 1e-6) on 64 columns over the same 30 days."""
+    import torch
+    from clmsoil import calibration
+    r = calibration.fd_check(fd_columns, fd_days)
+    out = {"fd_check": {k: r[k] for k in ("columns", "days", "rel_step", "max_rel_err",
+                                          "median_rel_err", "excluded_columns")}}
+    cal = calibration.Calibration(nc, days)
+    _, T = cal.make_target()
+    m0 = torch.ones((3, nc), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cal.forward(m0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    Lc, g = cal.loss_and_gradient(m0, T)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    assert bool(torch.isfinite(g).all())
+    out["gradient"] = {"columns": nc, "steps": days * 48, "forward_s": t1 - t0,
+                       "loss_and_gradient_s": t2 - t1, "loss": float(Lc.sum()),
+                       "grad_norm": float(g.norm())}
+    return out
 
 
 def zd09_bytes_per_column(n):
@@ -204,15 +302,17 @@ def zengdecker_2009(nc, steps, cpu_sample):
                for k, v in d.items()}
         sub["nc"] = m
         best = float("inf")
-        for _ in range(2):
+        for rep in range(6):
             s = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in sub.items()}
             t0 = time.perf_counter()
             for _ in range(2):
                 oracle.soilwater_zengdecker2009(s, cfg.to_abi())
-            best = min(best, time.perf_counter() - t0)
+            if rep:                              # pass 0 is the warm-up
+                best = min(best, time.perf_counter() - t0)
         out["cpu_baseline"] = {"value": 2 * m / best, "unit": "column-steps/s",
                                "cores": oracle.num_threads(), "kind": "port",
-                               "sample": f"first {m} columns x 2 steps, best of 2 ({best:.2f} s)"}
+                               "sample": f"first {m} columns x 2 steps, one warm-up pass then "
+                                         f"best of 5 ({best:.2f} s)"}
     return out
 
 
@@ -435,7 +535,8 @@ def _first_columns(d, nc, m):
 
 
 def _oracle_baseline(d, nc, m, run):
-    """The oracle (one thread) timed on one step of the first m columns, best of 2."""
+    """The oracle (one thread) timed on one step of the first m columns: one
+    warm-up pass, then the best of 5."""
     import numpy as np
     sys.path.insert(0, str(REPO / "oracle"))
     sub = _first_columns(d, nc, min(m, nc))
@@ -446,13 +547,15 @@ def _oracle_baseline(d, nc, m, run):
                     if isinstance(v, list) else v)
                 for k, v in sub.items()}
     best = float("inf")
-    for _ in range(2):
+    for rep in range(6):
         s = fresh()
         t0 = time.perf_counter()
         run(s)
-        best = min(best, time.perf_counter() - t0)
+        if rep:
+            best = min(best, time.perf_counter() - t0)
     return {"value": sub["nc"] / best, "unit": "column-steps/s", "cores": 1, "kind": "port",
-            "sample": f"first {sub['nc']} columns, one step, best of 2 ({best:.2f} s)"}
+            "sample": f"first {sub['nc']} columns, one step, one warm-up pass then best of 5 "
+                      f"({best:.2f} s)"}
 
 
 def ch4_transport_step(nc, steps, cpu_sample=0):
@@ -584,11 +687,15 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--nc", type=int, default=1 << 20)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 20)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-plain first-substep comparison")
+    ap.add_argument("--no-calibration", action="store_true",
+                    help="skip the calibration-gradient section (262,144 columns x 30 days)")
+    ap.add_argument("--no-zero-flux", action="store_true",
+                    help="skip the production day with the BC_ZERO_FLUX lower boundary")
     ap.add_argument("--no-zd", action="store_true",
                     help="skip the Zeng-Decker 2009 step section")
     ap.add_argument("--no-surface", action="store_true",
@@ -716,15 +823,26 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC_1M[dom] if (nc == 1 << 20 and args.seed == 2)
-                     else None},
+                     else None,
+                     "traffic_profiled_at": PROFILED_AT},
+        # the same kernel against the FP64 vector pipe (FLOP from its counters)
+        "fp64_view": ({"kernel": dom, "achieved": FP64_FLOP_1M[dom] / (ms / 1e3) / 1e12,
+                       "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": FP64_FLOP_1M[dom] / (ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
+                       "profiled_at": PROFILED_AT}
+                      if (nc == 1 << 20 and args.seed == 2) else None),
         "kernels_ms": {"heat": heat_ms, "richards": rich_ms},
         "eager_column_steps_per_s": nc * args.steps / (eager_ms / 1e3),
         "fused_step_hbm_frac": (b_r + b_h) / (total_ms / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS,
         "mean_substep_iterations": nsub,
         "wall_s": wall,
     }
+    if not args.no_zero_flux:
+        out["zero_flux_day"] = zero_flux_day(dev, snap, nc, args.steps)
     if not args.no_learned:
         out["learned_vs_plain"] = learned_vs_plain(nc, args.steps)
+    if not args.no_calibration and nc == 1 << 20:
+        out["calibration"] = calibration_section()
     if not args.no_surface:
         out["surface_energy"] = surface_energy(nc, args.steps)
     if not args.no_zd:
@@ -738,10 +856,13 @@ def main(argv=None):
         out["ch4_transport"] = ch4_transport_step(nc, args.steps, sec_cpu)
     if not args.no_cpu_baseline:
         v, cores, secs = cpu_baseline(host, min(args.cpu_sample, nc), args.cpu_steps)
+        ncpu, model = host_cpus()
         out["cpu_baseline"] = {"value": v, "unit": "column-steps/s", "cores": cores,
-                               "kind": "port",
+                               "kind": "port", "host_cpus": ncpu, "cpu_model": model,
                                "sample": f"first {min(args.cpu_sample, nc)} columns x {args.cpu_steps} "
-                                         f"steps of the same workload, best of 2 ({secs:.2f} s)"}
+                                         f"steps of the same workload (heat + Richards, BC_WATERTABLE), "
+                                         f"oracle -O3 -march=x86-64-v3 on {cores} OpenMP threads, one "
+                                         f"warm-up pass then best of 5 ({secs:.2f} s)"}
     print(json.dumps(out), flush=True)
     return 0
 

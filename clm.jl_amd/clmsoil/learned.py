@@ -289,20 +289,48 @@ def column_water(dev):
     return dev["h2osoi_liq"][ns:ns + n].sum(dim=0)
 
 
+def best_candidate_dtsub0(dev, cfg: SoilWaterMovementConfig, work=None):
+    """Per column, the candidate first substep dtime 2^(-k/2) (k = 0..11) that
+    completes the step from the current state in the fewest substep iterations,
+    found by running the step once per candidate on a copy (the labels the
+    network is trained on)."""
+    import torch
+    nc = int(dev["nc"]) if "nc" in dev else dev["nbedrock"].shape[0]
+    dtime = float(dev["dtime"])
+    device = dev["nbedrock"].device
+    work = dict(dev) if work is None else work
+    best_n = torch.full((nc,), float("inf"), dtype=torch.float64, device=device)
+    best_dt = torch.full((nc,), dtime, dtype=torch.float64, device=device)
+    for k in range(N_CANDIDATES):
+        dt0 = candidate_dtsub0(dtime, k)
+        for key in ("h2osoi_liq", "smp_l", "hk_l", "qcharge", "nsubsteps", "nrejected"):
+            work[key] = dev[key].clone()
+        run_step(work, cfg, torch.full((nc,), dt0, dtype=torch.float64, device=device))
+        better = work["nsubsteps"] < best_n
+        best_n = torch.where(better, work["nsubsteps"], best_n)
+        best_dt = torch.where(better, torch.full_like(best_dt, max(dt0, cfg.dtmin)), best_dt)
+    return best_dt, best_n
+
+
 def compare_controllers(pred: DtPredictor | None, nc: int, steps: int, seed: int,
-                        cfg: SoilWaterMovementConfig, device="cuda") -> dict:
+                        cfg: SoilWaterMovementConfig, device="cuda", hindsight=False) -> dict:
     """Advance the same config-0 columns through `steps` steps with the plain
-    controller and with the learned first substep (when `pred` is given).
-    Returns, per controller: column-steps/s of the timed GPU work (prediction
-    included), mean substep iterations, rejected-substep rate, and the largest
-    water-balance residual |dW - (infl - qcharge - roots) dt| in kg/m2."""
+    controller and with the learned first substep (when `pred` is given), and,
+    with `hindsight`, with each column's best candidate first substep chosen by
+    trying them all first (untimed): the ceiling any first-substep predictor can
+    reach under the unchanged controller. Returns, per controller: column-steps/s
+    of the timed GPU work (prediction included), mean substep iterations,
+    rejected-substep rate, and the largest water-balance residual
+    |dW - (infl - qcharge - roots) dt| in kg/m2."""
     import torch
     out = {}
-    modes = ["plain"] + (["learned"] if pred is not None else [])
+    modes = (["plain"] + (["learned"] if pred is not None else []) +
+             (["hindsight"] if hindsight else []))
     for mode in modes:
         dev, infl = config0_day(nc, seed, device)
         dtime = float(dev["dtime"])
         dt0 = torch.empty(nc, dtype=torch.float64, device=device) if mode == "learned" else None
+        work = dict(dev)
         evs = []
         iters = torch.zeros((), dtype=torch.float64, device=device)
         rej = torch.zeros((), dtype=torch.float64, device=device)
@@ -310,6 +338,9 @@ def compare_controllers(pred: DtPredictor | None, nc: int, steps: int, seed: int
         for _ in range(steps):
             dev["qflx_infl"] = infl()
             w0 = column_water(dev)
+            if mode == "hindsight":
+                work["qflx_infl"] = dev["qflx_infl"]
+                dt0, _ = best_candidate_dtsub0(dev, cfg, work)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
             if mode == "learned":
@@ -332,6 +363,9 @@ def compare_controllers(pred: DtPredictor | None, nc: int, steps: int, seed: int
                      "max_balance_residual": float(bal)}
     if pred is not None:
         out["speedup"] = out["learned"]["column_steps_per_s"] / out["plain"]["column_steps_per_s"]
+    if hindsight:
+        out["hindsight_speedup"] = (out["hindsight"]["column_steps_per_s"] /
+                                    out["plain"]["column_steps_per_s"])
     return out
 
 

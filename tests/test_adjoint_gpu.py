@@ -217,3 +217,12 @@ def test_calibration_gradient_vs_central_differences(gpu):
     assert r["max_rel_err"] is not None and r["max_rel_err"] <= 1e-6, r["max_rel_err"]
     assert r["excluded_columns"] <= 3
     assert np.all(np.isfinite(r["grad"]))
+
+
+def test_calibration_gradient_30_days(gpu):
+    """The calibration workload's full horizon: 30 days (1,440 steps) on 64
+    columns, adjoint through one checkpoint per day against central differences."""
+    r = calibration.fd_check(nc=64, days=30)
+    assert r["max_rel_err"] is not None and r["max_rel_err"] <= 1e-6, r["max_rel_err"]
+    assert r["excluded_columns"] <= 4
+    assert np.all(np.isfinite(r["grad"]))
