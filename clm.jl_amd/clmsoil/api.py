@@ -499,13 +499,15 @@ def get_tuning() -> tuple:
     return tuple(int(x.value) for x in v)
 
 
-_MATH_OPS = {"pow": 0, "log": 1, "log10": 2, "exp": 3, "jl_max": 4, "jl_min": 5}
+_MATH_OPS = {"pow": 0, "log": 1, "log10": 2, "exp": 3, "jl_max": 4, "jl_min": 5,
+             "pow_w": 6, "pow_base_w": 7, "log10_w": 8}
 
 
 def math_eval(op: str, x, y=None):
-    """The kernels' FP64 pow/log/log10/exp (clm_math.h) and Julia max/min
-    evaluated on the device, element-wise over float64 CUDA tensors; returns a
-    new tensor."""
+    """The kernels' FP64 pow/log/log10/exp (clm_math.h), their wavefront-
+    dispatched fast-path forms (pow_w, pow_base_w with x the fixed base, log10_w)
+    and Julia max/min evaluated on the device, element-wise over float64 CUDA
+    tensors; returns a new tensor."""
     import torch
     x = x.contiguous()
     if y is not None:

@@ -94,14 +94,14 @@ CLM_MATH_FN double clm_fast_two_sum(double a, double b, double *e) {
 }
 
 /* log x as hi + lo for positive finite x (|error| < 2^-72 |log x| away from
- * x = 1, relative 2^-100 in the subinterval around 1). Subnormal x is scaled
- * by 2^52 first (kadj = -52). Straight-line code: no branches. */
-CLM_MATH_FN clm_dd clm_log_ext(double x) {
-    const int sub = clm_bits(x) < 0x0010000000000000ULL;
-    const uint64_t ix = clm_bits(sub ? x * 0x1p52 : x);
+ * x = 1, relative 2^-100 in the subinterval around 1): clm_log_ext below.
+ * Subnormal x is scaled by 2^52 first (kadj = -52). Straight-line code: no
+ * branches. */
+/* Core: ix = the bits of a positive normal double, kadj added to its exponent. */
+CLM_MATH_FN clm_dd clm_log_ext_bits(uint64_t ix, int kadj) {
     const uint64_t tmp = ix - CLM_LOG_OFF;
     const int i = (int)((tmp >> 44) & 255u);
-    const int k = (int)((int64_t)tmp >> 52) - (sub ? 52 : 0);
+    const int k = (int)((int64_t)tmp >> 52) + kadj;
     const uint64_t iz = ix - (tmp & 0xfff0000000000000ULL);
     const double z = clm_from_bits(iz);
     const double invc = CLM_LOG_TAB(i, 0);
@@ -137,6 +137,14 @@ CLM_MATH_FN clm_dd clm_log_ext(double x) {
     out.hi = clm_fast_two_sum(s3, lo, &out.lo);
     return out;
 }
+
+CLM_MATH_FN clm_dd clm_log_ext(double x) {
+    const int sub = clm_bits(x) < 0x0010000000000000ULL;
+    return clm_log_ext_bits(clm_bits(sub ? x * 0x1p52 : x), sub ? -52 : 0);
+}
+
+/* clm_log_ext for x >= 2^-1022 (no subnormal scaling): the same value. */
+CLM_MATH_FN clm_dd clm_log_ext_normal(double x) { return clm_log_ext_bits(clm_bits(x), 0); }
 
 /* exp(hi + lo) with |lo| <= 2^-52 |hi|, rounded to double. Straight-line
  * code: out-of-range and NaN arguments are clamped for the evaluation and the
@@ -193,15 +201,19 @@ CLM_MATH_FN double clm_log(double x) {
     return clm_log_special(x, clm_log_ext(ok ? x : 1.0).hi);
 }
 
-CLM_MATH_FN double clm_log10(double x) {
-    const int ok = (x > 0.0) & (x <= 0x1.fffffffffffffp+1023);
-    const clm_dd d = clm_log_ext(ok ? x : 1.0);
+/* log10 from log x = d.hi + d.lo */
+CLM_MATH_FN double clm_log10_dd(clm_dd d) {
     const double p = d.hi * CLM_INVLN10_HI;
     const double plo = fma(d.hi, CLM_INVLN10_HI, -p);
     double t = d.hi * CLM_INVLN10_LO;
     t = t + d.lo * CLM_INVLN10_HI;
     t = t + plo;
-    return clm_log_special(x, p + t);
+    return p + t;
+}
+
+CLM_MATH_FN double clm_log10(double x) {
+    const int ok = (x > 0.0) & (x <= 0x1.fffffffffffffp+1023);
+    return clm_log_special(x, clm_log10_dd(clm_log_ext(ok ? x : 1.0)));
 }
 
 /* exp(y * L) for L = log x given as a double-double (x > 0 finite). */
@@ -239,5 +251,29 @@ CLM_MATH_FN double clm_pow(double x, double y) {
     v = (x != x || y != y) ? x + y : v;
     return (y == 0.0 || x == 1.0) ? 1.0 : v;
 }
+
+/* Fast paths for the arguments the soil physics produces. Each returns exactly
+ * what the general function returns on its domain, so a caller may take it for
+ * a batch of arguments that all satisfy the domain test and the general
+ * function otherwise, and get the same bits either way. */
+
+/* Domain of clm_pow_pos: x positive, normal and finite; y finite. */
+CLM_MATH_FN int clm_pow_pos_ok(double x, double y) {
+    return (x >= 0x1p-1022) & (x <= 0x1.fffffffffffffp+1023) & (fabs(y) <= 0x1.fffffffffffffp+1023);
+}
+/* clm_pow(x, y) where clm_pow_pos_ok(x, y): none of the special cases but
+ * y == 0 and x == 1 can occur there. */
+CLM_MATH_FN double clm_pow_pos(double x, double y) {
+    const double r = clm_pow_log(clm_log_ext_normal(x), y);
+    return (y == 0.0 || x == 1.0) ? 1.0 : r;
+}
+/* clm_pow(b, y) for a fixed base b (positive, normal, finite, b != 1) given as
+ * Lb = clm_log_ext(b), and finite y (the domain test is fabs(y) <= DBL_MAX). */
+CLM_MATH_FN double clm_pow_base(clm_dd Lb, double y) {
+    const double r = clm_pow_log(Lb, y);
+    return (y == 0.0) ? 1.0 : r;
+}
+/* clm_log10(x) for positive, normal, finite x (clm_pow_pos_ok(x, 0)). */
+CLM_MATH_FN double clm_log10_pos(double x) { return clm_log10_dd(clm_log_ext_normal(x)); }
 
 #endif /* CLM_MATH_H */

@@ -52,6 +52,25 @@ __host__ __device__ __forceinline__ double jl_min(double x, double y) {
     return (x != x || y != y) ? d : am;
 }
 
+// Transcendentals through the fast paths of clm_math.h when every active lane
+// of the wavefront has an argument in their domain (wavefront-uniform branch),
+// through the general functions otherwise: the same bits either way.
+__device__ __forceinline__ bool wave_all(bool p) { return __ballot(!p) == 0ull; }
+__device__ __forceinline__ double pow_w(double x, double y) {
+    if (__builtin_expect(wave_all(clm_pow_pos_ok(x, y)), 1)) return clm_pow_pos(x, y);
+    return clm_pow(x, y);
+}
+// b^y for a fixed base b (positive, normal, finite, b != 1) with Lb = clm_log_ext(b)
+__device__ __forceinline__ double pow_base_w(clm_dd Lb, double b, double y) {
+    if (__builtin_expect(wave_all(fabs(y) <= 0x1.fffffffffffffp+1023), 1))
+        return clm_pow_base(Lb, y);
+    return clm_pow(b, y);
+}
+__device__ __forceinline__ double log10_w(double x) {
+    if (__builtin_expect(wave_all(clm_pow_pos_ok(x, 0.0)), 1)) return clm_log10_pos(x);
+    return clm_log10(x);
+}
+
 // Column-major [c, j] access with 1-based level j (the reference's index).
 struct Mat {
     double *p;

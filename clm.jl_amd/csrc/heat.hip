@@ -129,6 +129,10 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     double int_snow = a.int_snow ? a.int_snow[c] : 0.0;
     double snomelt_accum = a.snomelt_accum ? a.snomelt_accum[c] : 0.0;
 
+    // saturated conductivity tkmg * TKWAT^(fl*watsat) * TKICE^((1-fl)*watsat): the
+    // logarithms of the two fixed bases once per column
+    const clm_dd L_tkwat = clm_log_ext_normal(TKWAT), L_tkice = clm_log_ext_normal(TKICE);
+
     // ---- per-level loads, thermal conductivity thk (:647-716) and heat capacity
     // cv (:781-872), one level at a time so that only t, z, zi, dz, sabg, thk and
     // cv stay live afterwards ----
@@ -169,12 +173,13 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             satw = jl_min(1.0, satw);
             if (satw > 1.0e-7) {
                 double dke;
-                if (t[i] >= TFRZ) dke = jl_max(0.0, clm_log10(satw) + 1.0);
+                if (t[i] >= TFRZ) dke = jl_max(0.0, log10_w(satw) + 1.0);
                 else dke = satw;
                 const double fl_denom = liq / (DENH2O * dzi) + ice / (DENICE * dzi) +
                                         exc / (DENICE * dzi);
                 const double fl = (liq / (DENH2O * dzi)) / fl_denom;
-                const double dksat = tkmg * clm_pow(TKWAT, fl * wsat) * clm_pow(TKICE, (1.0 - fl) * wsat);
+                const double dksat = tkmg * pow_base_w(L_tkwat, TKWAT, fl * wsat) *
+                                     pow_base_w(L_tkice, TKICE, (1.0 - fl) * wsat);
                 th = dke * dksat + (1.0 - dke) * tkdry;
             } else {
                 th = tkdry;
@@ -694,7 +699,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             if (exc_i > 0.0 && tt > tfrz) { imelt[i] = 1; tinc = tfrz - tt; tt = tfrz; }
             if (tt < tfrz) {
                 const double smp = hfus * (tfrz - tt) / (GRAV * tt) * 1000.0;
-                supercool = wsat_i * clm_pow(smp / sucsat_i, -1.0 / bsw_i);
+                supercool = wsat_i * pow_w(smp / sucsat_i, -1.0 / bsw_i);
                 supercool *= dz_i * 1000.0;
             }
             if (liq[i] > supercool && tt < tfrz) { imelt[i] = 2; tinc = tfrz - tt; tt = tfrz; }

@@ -112,6 +112,7 @@ struct RColumn {
             if (k == 0) P.flag((unsigned)DEV_ERR_NBEDROCK);
             return false;
         }
+        const clm_dd L10 = clm_log_ext_normal(10.0);   // impedance 10^(-e_ice * icefrac)
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             const int j = i * G + k + 1;
@@ -144,10 +145,10 @@ struct RColumn {
                 watsat[i] = ws;
                 double imp;
                 if (j == n) {
-                    imp = clm_pow(10.0, -P.e_ice * icf);
+                    imp = pow_base_w(L10, 10.0, -P.e_ice * icf);
                     dn(i) = M_TO_MM * (zwt - zj);                      // :1357 (WATERTABLE)
                 } else {
-                    imp = clm_pow(10.0, -P.e_ice * (0.5 * (icf + icf1)));
+                    imp = pow_base_w(L10, 10.0, -P.e_ice * (0.5 * (icf + icf1)));
                     dn(i) = M_TO_MM * (zj1 - zj);                      // :1338
                 }
                 kh(i) = imp * hks;   // (imp * hksat) * s1^(2b+3), the reference's grouping
@@ -210,7 +211,7 @@ struct RColumn {
             }
             bsw[i] = b;
             const double sn = s2[i];
-            const double smpv = -suc * clm_pow(sn, -b);
+            const double smpv = -suc * pow_w(sn, -b);
             const double dsmpds = -b * smpv / sn;
             smp[i] = smpv;
             dsmpdw[i] = dsmpds / watsat[i];
@@ -227,7 +228,7 @@ struct RColumn {
             s1 = jl_min(s1, 1.0);
             s1 = jl_max(0.01, s1);
             const double b = bsw[i];
-            const double hkv = kh(i) * clm_pow(s1, 2.0 * b + 3.0);
+            const double hkv = kh(i) * pow_w(s1, 2.0 * b + 3.0);
             const double dhkdw = (2.0 * b + 3.0) * hkv / s1;
             if constexpr (LC) cache(LC_HK, i) = hkv;
             else if (j <= n && P.hk_l) P.hk_l[soil(j)] = hkv;

@@ -62,6 +62,14 @@ int clmsoil_set_zd_block(int block);
 #define CLMSOIL_MATH_EXP    3
 #define CLMSOIL_MATH_JL_MAX 4
 #define CLMSOIL_MATH_JL_MIN 5
+/* The kernels' forms: the fast path of pow / log10 for a wavefront whose
+ * arguments all lie in its domain (positive normal x, finite y), the general
+ * function otherwise — the same bits either way. POW_BASE_W takes x[i] as a
+ * fixed base (its logarithm computed once, as the kernels do for constant
+ * bases). */
+#define CLMSOIL_MATH_POW_W      6
+#define CLMSOIL_MATH_POW_BASE_W 7
+#define CLMSOIL_MATH_LOG10_W    8
 int clmsoil_math_eval(int op, const double *x, const double *y, double *out, int64_t n,
                       void *stream);
 

@@ -42,7 +42,10 @@ void orc_clm_math(int op, const double *x, const double *y, double *out, int64_t
             case 2: out[i] = clm_log10(x[i]); break;
             case 3: out[i] = clm_exp(x[i]); break;
             case 4: out[i] = orc_jl_max(x[i], y[i]); break;
-            default: out[i] = orc_jl_min(x[i], y[i]); break;
+            case 5: out[i] = orc_jl_min(x[i], y[i]); break;
+            case 6: out[i] = clm_pow_pos(x[i], y[i]); break;
+            case 7: out[i] = clm_pow_base(clm_log_ext(x[i]), y[i]); break;
+            default: out[i] = clm_log10_pos(x[i]); break;
         }
     }
 }

@@ -159,10 +159,12 @@ void orc_set_zd_pow_jitter(uint64_t seed);
 int orc_soilwater_zengdecker2009(const clmsoil_zd_args *a, const clmsoil_levels *lev,
                                  double dtime, const clmsoil_swm_config *cfg);
 
-/* Number of OpenMP threads the batched functions use. */
 /* op 0: clm_pow(x, y), 1: clm_log(x), 2: clm_log10(x), 3: clm_exp(x),
- * 4: Julia max(x, y), 5: Julia min(x, y) */
+ * 4: Julia max(x, y), 5: Julia min(x, y), and the fast paths on their domains:
+ * 6: clm_pow_pos(x, y), 7: clm_pow_base(clm_log_ext(x), y) (x the base),
+ * 8: clm_log10_pos(x) */
 void orc_clm_math(int op, const double *x, const double *y, double *out, int64_t n);
+/* Number of OpenMP threads the batched functions use. */
 int orc_num_threads(void);
 /* Test hook of the moisture-form step: each substep iteration of column `col`
  * (0-based; -1 = off) appends (dtsub, errorMax, accepted) to buf[3*k..3*k+2],

@@ -278,7 +278,8 @@ def jl_pow5(x: float) -> float:
     return float(L.orc_jl_pow5(float(x)))
 
 
-_MATH_OPS = {"pow": 0, "log": 1, "log10": 2, "exp": 3, "jl_max": 4, "jl_min": 5}
+_MATH_OPS = {"pow": 0, "log": 1, "log10": 2, "exp": 3, "jl_max": 4, "jl_min": 5,
+             "pow_pos": 6, "pow_base": 7, "log10_pos": 8}
 
 
 def clm_math(op: str, x, y=None):

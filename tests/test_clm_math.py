@@ -174,3 +174,31 @@ def test_bits_helper_roundtrip():
     assert (ulp_dist(a, b) == 1).all()
     assert ulp_dist(np.array([0.0]), np.array([-0.0]))[0] == 0
     assert struct.pack("<d", 1.0) == np.float64(1.0).tobytes()
+
+
+def test_fast_paths_equal_general_functions():
+    """clm_pow_pos, clm_pow_base and clm_log10_pos return the bits of clm_pow /
+    clm_log10 on their domains (positive normal x, finite y): the kernels take
+    them for a batch whose arguments all qualify, the general functions else."""
+    rng = np.random.default_rng(11)
+    n = 400_000
+    x = np.concatenate([rng.uniform(0.01, 1.0, n),                     # saturations
+                        np.exp(rng.uniform(-700.0, 700.0, n)),         # wide range
+                        [2.0 ** -1022, 1.0, 1.0 + 2.0 ** -52, np.finfo(float).max]])
+    y = np.concatenate([rng.uniform(-20.0, 30.0, n), rng.uniform(-2.0, 2.0, n),
+                        [0.5, 7.0, 0.0, -0.0]])
+    y[::97] = 0.0
+    y[1::101] = -0.0
+    a = oracle.clm_math("pow", x, y)
+    b = oracle.clm_math("pow_pos", x, y)
+    assert np.array_equal(a.view(np.int64), b.view(np.int64))
+    for base in (0.57, 2.29, 10.0, 0.01, 1.5):
+        yb = np.concatenate([rng.uniform(-300.0, 300.0, n // 4), [0.0, -0.0, 1.0, -1.0]])
+        g = oracle.clm_math("pow", np.full_like(yb, base), yb)
+        f = oracle.clm_math("pow_base", np.full_like(yb, base), yb)
+        assert np.array_equal(g.view(np.int64), f.view(np.int64)), base
+    g = oracle.clm_math("log10", x)
+    f = oracle.clm_math("log10_pos", x)
+    assert np.array_equal(g.view(np.int64), f.view(np.int64))
+    # outside the domain they differ: the domain test is needed
+    assert oracle.clm_math("pow_pos", np.array([-2.0]), np.array([2.0]))[0] != 4.0

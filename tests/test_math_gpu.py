@@ -39,6 +39,25 @@ def test_device_equals_host(gpu, op):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("op,general", [("pow_w", "pow"), ("pow_base_w", "pow"),
+                                        ("log10_w", "log10")])
+def test_wavefront_fast_paths_equal_general(gpu, op, general):
+    """The kernels' fast-path forms against the host's general functions: whole
+    wavefronts in the fast path's domain, wavefronts with one lane outside it
+    (the general function then runs for the wavefront) and special values."""
+    import torch
+    rng = np.random.default_rng(8)
+    x, y = _args(rng, 64 * 3000)
+    if op == "pow_base_w":   # fixed bases, one per wavefront of 64 lanes
+        x = np.repeat(rng.choice([0.57, 2.29, 10.0, 0.3, 1.0, -2.0, np.nan], x.size // 64 + 1),
+                      64)[:x.size]
+    host = oracle.clm_math(general, x, y)
+    dev = api.math_eval(op, torch.from_numpy(x).cuda(),
+                        None if op == "log10_w" else torch.from_numpy(y).cuda()).cpu().numpy()
+    assert bitwise_mismatch(dev, host) == 0
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("op", ["jl_max", "jl_min"])
 def test_julia_max_min_device(gpu, op):
     """Julia's Float64 max/min as the kernels evaluate them: NaN from either
