@@ -288,38 +288,51 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         if (act) {
             const double dzp = zn - z[i];
             const double dzm = z[i] - zp;
-            if (j <= 0) {                                     // snow (_rhs_snow, _mat_snow)
-                if (jj == jtop) {
+            const bool snow = j <= 0, top = jj == jtop;
+            const bool s1u = !snow && !top && j == 1;         // soil 1 under snow
+            // The rows below need at most four quotients per level. Each is the
+            // reference's own subexpression with its own rounding:
+            //   q1 = omc*fact*tk/dzp (band 2 is -q1; -omc*fact*tk/dzp == -q1 exactly)
+            //   q2 = omc*fact*tkp/dzm, or frac_sno*omc*fact*tkp/dzm for soil 1 under
+            //        snow (the lower band is -q2)
+            //   q3 = tk/dzp,  q4 = tkp/dzm, or frac_sno*tkp/dzm for soil 1 under snow
+            // so that every lane divides four times whatever its case.
+            const double of = omc * fact[i];
+            const double q1 = of * tk[i] / dzp;
+            const double q2 = (s1u ? frac_sno * omc * fact[i] : of) * tkp / dzm;
+            const double q3 = tk[i] / dzp;
+            const double q4 = (s1u ? frac_sno * tkp : tkp) / dzm;
+            if (snow) {                                       // snow (_rhs_snow, _mat_snow)
+                if (top) {
                     r = t[i] + fact[i] * (hs_top_snow - dhsdT * t[i] + cnfac * fn[i]);
-                    d = 1.0 + omc * fact[i] * tk[i] / dzp - fact[i] * dhsdT;
+                    d = 1.0 + q1 - fact[i] * dhsdT;
                 } else {
                     r = t[i] + cnfac * fact[i] * (fn[i] - fnp);
                     r += fact[i] * sabg[i];
-                    lo = -omc * fact[i] * tkp / dzm;
-                    d = 1.0 + omc * fact[i] * (tk[i] / dzp + tkp / dzm);
+                    lo = -q2;
+                    d = 1.0 + of * (q3 + q4);
                 }
-                up = -omc * fact[i] * tk[i] / dzp;            // band 2, or band 1 at j == 0
-            } else if (jj == jtop) {                          // soil 1, no snow
+                up = -q1;                                     // band 2, or band 1 at j == 0
+            } else if (top) {                                 // soil 1, no snow
                 r = t[i] + fact[i] * (hs_top_snow - dhsdT * t[i] + cnfac * fn[i]);
-                d = 1.0 + omc * fact[i] * tk[i] / dzp - fact[i] * dhsdT;
-                up = -omc * fact[i] * tk[i] / dzp;
-            } else if (j == 1) {                              // soil 1 under snow
+                d = 1.0 + q1 - fact[i] * dhsdT;
+                up = -q1;
+            } else if (s1u) {                                 // soil 1 under snow
                 r = t[i] + fact[i] * ((1.0 - frac_sno) * (hs_soil - dhsdT * t[i]) +
                                       cnfac * (fn[i] - frac_sno * fnp));
                 r += frac_sno * fact[i] * sabg[i];
-                up = -omc * fact[i] * tk[i] / dzp;
-                d = 1.0 + omc * fact[i] * (tk[i] / dzp + frac_sno * tkp / dzm) -
-                    (1.0 - frac_sno) * fact[i] * dhsdT;
-                lo = -frac_sno * omc * fact[i] * tkp / dzm;   // band 5 (snow 0)
+                up = -q1;
+                d = 1.0 + of * (q3 + q4) - (1.0 - frac_sno) * fact[i] * dhsdT;
+                lo = -q2;                                     // band 5 (snow 0)
             } else if (j <= nlevgrnd - 1) {
                 r = t[i] + cnfac * fact[i] * (fn[i] - fnp);
-                up = -omc * fact[i] * tk[i] / dzp;
-                d = 1.0 + omc * fact[i] * (tk[i] / dzp + tkp / dzm);
-                lo = -omc * fact[i] * tkp / dzm;
+                up = -q1;
+                d = 1.0 + of * (q3 + q4);
+                lo = -q2;
             } else {
                 r = t[i] - cnfac * fact[i] * fnp + fact[i] * fn[i];
-                d = 1.0 + omc * fact[i] * tkp / dzm;
-                lo = -omc * fact[i] * tkp / dzm;
+                d = 1.0 + q2;
+                lo = -q2;
             }
             if (j == 1 && frac_h2osfc != 0.0) {               // h2osfc corrections
                 r -= frac_h2osfc * fact[i] * ((hs_soil - dhsdT * t[i]) + cnfac * fn_h2osfc);
@@ -711,15 +724,18 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         double hm = 0.0;
         const double fct = fact[i];
         if (imelt[i] > 0) {
+            // one quotient per lane: tinc/fct, or frac_sno*tinc/fct below the top
+            // snow layer (-frac_sno*tinc/fct == -(frac_sno*tinc/fct) exactly)
+            const bool sni = jj != jtop && j < 1;
+            const double qh = (sni ? frac_sno * tinc : tinc) / fct;
             if (jj == jtop) {
-                if (j > 0) hm = dhsdT * tinc - tinc / fct;
-                else hm = frac_sno * (dhsdT * tinc - tinc / fct);
+                if (j > 0) hm = dhsdT * tinc - qh;
+                else hm = frac_sno * (dhsdT * tinc - qh);
                 if (j == 1 && frac_h2osfc != 0.0) hm -= frac_h2osfc * dhsdT * tinc;
             } else if (j == 1) {
-                hm = (1.0 - frac_sno - frac_h2osfc) * dhsdT * tinc - tinc / fct;
+                hm = (1.0 - frac_sno - frac_h2osfc) * dhsdT * tinc - qh;
             } else {
-                if (j < 1) hm = -frac_sno * tinc / fct;
-                else hm = -tinc / fct;
+                hm = -qh;
             }
         }
         if (imelt[i] == 1 && hm < 0.0) { hm = 0.0; imelt[i] = 0; }
@@ -738,16 +754,16 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
                 qsm_set = jl_max(0.0, temp1 - h2osno_nl_1) / dtime;
                 unresolved = true;
             }
-            double heatr = 0.0;
+            // new ice (and excess ice) first; the latent heat they release is then
+            // hfus*(wice0 - icev)/dtime, or hfus*(wexice0 - excv + wice0 - icev)/dtime
+            // when excess ice melts too
+            bool exc_melt = false;
             if (xm > 0.0) {
                 icev = jl_max(0.0, wice0 - xm);
-                heatr = hm - hfus * (wice0 - icev) / dtime;
                 const double xm2 = xm - icev;
-                if (icev == 0.0) {
-                    if (wexice0 >= 0.0 && xm2 > 0.0 && j >= 2) {
-                        excv = jl_max(0.0, wexice0 - xm2);
-                        heatr = hm - hfus * (wexice0 - excv + wice0 - icev) / dtime;
-                    }
+                if (icev == 0.0 && wexice0 >= 0.0 && xm2 > 0.0 && j >= 2) {
+                    excv = jl_max(0.0, wexice0 - xm2);
+                    exc_melt = true;
                 }
             } else if (xm < 0.0) {
                 if (j <= 0) {
@@ -756,33 +772,32 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
                     if (wmass0 - wexice0 < supercool) icev = 0.0;
                     else icev = jl_min(wmass0 - wexice0 - supercool, wice0 - xm);
                 }
-                heatr = hm - hfus * (wice0 - icev) / dtime;
             }
+            const double qi = hfus * (wice0 - icev) / dtime;
+            const double qx = exc_melt ? hfus * (wexice0 - excv + wice0 - icev) / dtime : qi;
+            const double heatr = (xm > 0.0 || xm < 0.0) ? hm - qx : 0.0;
             const double ei_val = j >= 1 ? excv : 0.0;
             liqv = jl_max(0.0, wmass0 - icev - ei_val);
             if (fabs(heatr) > 0.0) {
-                if (jj == jtop) {
-                    if (j == 1) {
-                        tt += fct * heatr / (1.0 - (1.0 - frac_h2osfc) * fct * dhsdT);
-                    } else {
-                        if (frac_sno > 0.0) tt += (fct / frac_sno) * heatr / (1.0 - fct * dhsdT);
-                    }
-                } else if (j == 1) {
-                    tt += fct * heatr / (1.0 - (1.0 - frac_sno - frac_h2osfc) * fct * dhsdT);
-                } else {
-                    if (j > 0) tt += fct * heatr;
-                    else if (frac_sno > 0.0) tt += (fct / frac_sno) * heatr;
-                }
+                // tt += (fct, or fct/frac_sno for snow) * heatr, divided by the
+                // implicit factor of the top layer and of soil layer 1 (by 1.0,
+                // which is exact, elsewhere); snow layers only when frac_sno > 0
+                const bool snowl = j <= 0;
+                const double fa = snowl ? fct / frac_sno : fct;
+                double den = 1.0;
+                if (jj == jtop && j == 1) den = 1.0 - (1.0 - frac_h2osfc) * fct * dhsdT;
+                else if (jj == jtop) den = 1.0 - fct * dhsdT;
+                else if (j == 1) den = 1.0 - (1.0 - frac_sno - frac_h2osfc) * fct * dhsdT;
+                if (!snowl || frac_sno > 0.0) tt += fa * heatr / den;
                 if (j <= 0)
                     if (liqv * icev > 0.0) tt = tfrz;
             }
-            if (j >= 1) {
-                dxmf[i] = hfus * (wice0 - icev) / dtime + hfus * (wexice0 - excv) / dtime;
-            } else {
-                dxmf[i] = hfus * (wice0 - icev) / dtime;
+            dxmf[i] = (j >= 1) ? qi + hfus * (wexice0 - excv) / dtime : qi;
+            if (j < 1 && (imelt[i] == 1 || imelt[i] == 2)) {
+                const double qm = jl_max(0.0, imelt[i] == 1 ? wice0 - icev : icev - wice0) / dtime;
+                if (imelt[i] == 1) qmelt_l[i] = qm;
+                else qfrz_l[i] = qm;
             }
-            if (imelt[i] == 1 && j < 1) qmelt_l[i] = jl_max(0.0, wice0 - icev) / dtime;
-            if (imelt[i] == 2 && j < 1) qfrz_l[i] = jl_max(0.0, icev - wice0) / dtime;
             tgt_written[i] = true;
         }
         tnew[i] = tt;
