@@ -191,14 +191,20 @@ def calibration_section(nc=262144, days=30, fd_columns=64, fd_days=30):
     """The differentiable-calibration workload at its size: 262,144 columns x 30
     days (1,440 steps) of heat + Richards (BC_FLUX), the loss of daily drainage
     against a target from perturbed parameters, and its gradient w.r.t. per-column
-    multipliers of b, ksat and porosity through the hand-written adjoint with one
-    checkpoint per day; plus the adjoint against central differences (relative
-    step 1e-6) on 64 columns over the same 30 days."""
+    multipliers of b, ksat and porosity through the hand-written adjoints of both
+    steps with one checkpoint per day; plus the adjoint against central
+    differences (relative step 1e-6) on 64 columns over the same 30 days, and on
+    64 freezing columns over 2 days."""
     import torch
     from clmsoil import calibration
+    keys = ("columns", "days", "rel_step", "max_rel_err", "median_rel_err", "excluded_columns")
     r = calibration.fd_check(fd_columns, fd_days)
-    out = {"fd_check": {k: r[k] for k in ("columns", "days", "rel_step", "max_rel_err",
-                                          "median_rel_err", "excluded_columns")}}
+    out = {"fd_check": {k: r[k] for k in keys}}
+    # columns around the freezing point (the heat-step adjoint with phase change);
+    # two days: gradients through layers held at their supercooled-water limit grow
+    # by orders of magnitude per day, so longer horizons are not a useful check
+    r = calibration.fd_check(fd_columns, 2, freezing=True)
+    out["fd_check_freezing"] = {k: r[k] for k in keys}
     cal = calibration.Calibration(nc, days)
     _, T = cal.make_target()
     m0 = torch.ones((3, nc), dtype=torch.float64, device="cuda")

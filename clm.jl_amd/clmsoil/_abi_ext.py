@@ -52,6 +52,8 @@ def _declare_adjoint(lib) -> None:
     lib.clmsoil_adjoint_workspace_bytes.restype = L
     lib.clmsoil_soilwm_moisture_form_adjoint.argtypes = [P, P, D, P, P, P, P, P, P, P, P, L, P]
     lib.clmsoil_soilwm_moisture_form_adjoint.restype = I
+    lib.clmsoil_soil_temperature_adjoint.argtypes = [P, P, D] + [P] * 7
+    lib.clmsoil_soil_temperature_adjoint.restype = I
     lib.clmsoil_tridiagonal_multi_vjp.argtypes = [P] * 7 + [L, L, L] + [P] * 5
     lib.clmsoil_tridiagonal_multi_vjp.restype = I
     lib.clmsoil_batched_band_solve_vjp.argtypes = [P] * 6 + [I, I, I, L, L, L, P, P, P]

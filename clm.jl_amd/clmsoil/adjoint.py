@@ -12,7 +12,7 @@ import ctypes as C
 
 from . import _abi
 from .api import MfIn, MfState, SoilWaterMovementConfig, _mf_dict, _stream, MOISTURE_FORM, CLMSoilError
-from .bind import byref, levels, mf_args
+from .bind import byref, heat_args, levels, mf_args
 
 MAX_SUBSTEPS = 64
 
@@ -58,6 +58,27 @@ def soilwm_moisture_form_adjoint(st: MfState, mfin: MfIn, mask, nbedrock, nlevsn
         byref(a), byref(lev), float(dtime), byref(c), _p(liq_bar), _p(qcharge_bar),
         _p(grad_bsw), _p(grad_hksat), _p(grad_watsat), _p(grad_scale), _p(workspace), nbytes,
         _stream()), "soilwm_moisture_form_adjoint")
+
+
+def soil_temperature_adjoint(state: dict, dtime: float, t_bar, liq_bar, ice_bar,
+                             nlevsno: int = 12, nlevgrnd: int = 25, nlevsoi: int = 20,
+                             grad_bsw=None, grad_watsat=None, grad_scale=None) -> None:
+    """Reverse of one heat step (api.soil_temperature) from the state at its
+    start, on soil columns without snow, surface water or excess ice.
+
+    t_bar, liq_bar, ice_bar (shape of t_soisno) hold dL/d(t_soisno, h2osoi_liq,
+    h2osoi_ice) after the step on entry and before it on return. grad_scale
+    [3, nc] accumulates sum_j p_j dL/dp_j in rows 0 (bsw) and 2 (watsat)."""
+    d = dict(state)
+    d.setdefault("nc", d["snl"].shape[0])
+    d.update(nlevsno=nlevsno, nlevgrnd=nlevgrnd, nlevsoi=nlevsoi)
+    if d["nc"] == 0:
+        return
+    L = _lib()
+    a, lev = heat_args(d), levels(d)
+    _abi.check(L, L.clmsoil_soil_temperature_adjoint(
+        byref(a), byref(lev), float(dtime), _p(t_bar), _p(liq_bar), _p(ice_bar), _p(grad_bsw),
+        _p(grad_watsat), _p(grad_scale), _stream()), "soil_temperature_adjoint")
 
 
 def tridiagonal_multi_vjp(ubar, u, a, b, c, jtop, mask, ncols: int, nlevs: int,

@@ -1,6 +1,6 @@
 """Differentiable calibration of per-column soil hydraulic parameters.
 
-Workload: `nc` snow-free, unfrozen columns (inputs.calibration_columns) advanced
+Workload: `nc` snow-free columns (inputs.calibration_columns) advanced
 `days` x 48 steps of 1800 s. Each step runs the heat step and then the
 moisture-form Richards step with a free-drainage (BC_FLUX) lower boundary. The
 parameters are per-column multipliers m = (m_b, m_ksat, m_porosity) on bsw,
@@ -11,27 +11,32 @@ hksat and watsat (all layers). The loss is
 the squared error of daily drainage (mm) against a target T made by the same
 model from perturbed multipliers.
 
-Gradient: reverse mode through the library's hand-written Richards adjoint
-(clmsoil_soilwm_moisture_form_adjoint). The forward pass keeps one checkpoint
-of the soil water per day. The reverse pass recomputes each day's 48 states
-from its checkpoint and applies the step adjoints in reverse order; each step
-adjoint replays its substeps from a tape. The heat step is part of the
-forward model, but on these columns nothing freezes, so it never changes
-h2osoi_liq: every forward pass checks that on the device. The water state,
-and with it the loss, therefore depends on the parameters only through the
-Richards steps, and the heat step's adjoint contributes exactly zero.
+Gradient: reverse mode through the library's hand-written adjoints of both
+steps. The forward pass keeps one checkpoint of the heat and water state per
+day. The reverse pass recomputes each day's 48 steps from its checkpoint
+(keeping the state at the start of every heat and Richards step) and then
+applies, step by step in reverse order, the Richards adjoint
+(clmsoil_soilwm_moisture_form_adjoint, which replays its substeps from a tape)
+and the heat-step adjoint (clmsoil_soil_temperature_adjoint). Layers that
+freeze or thaw move water between h2osoi_liq and h2osoi_ice in the heat step,
+so the temperature, liquid water and ice all carry adjoints from step to step;
+watsat and bsw reach the loss through both steps (conductivity, heat capacity
+and the supercooled-water limit in the heat step). The ice fraction the
+Richards step reads (icefrac) is an input of the workload, held at its initial
+value.
 
 The reference computes this kind of gradient with Enzyme over its driver
-(src/biogeophys/driver_reverse.jl:598-615 soil water phase, multi-step
-checkpointing in canopy_fluxes_reverse.jl:488-542).
+(src/biogeophys/driver_reverse.jl:169-190 soil temperature, :598-615 soil
+water, multi-step checkpointing in canopy_fluxes_reverse.jl:488-542).
 """
 from __future__ import annotations
 
 import numpy as np
 
 from . import adjoint, inputs
-from .api import (BC_FLUX, MOISTURE_FORM, MfIn, MfState, init_soilwater_movement,
-                  soil_temperature, soilwm_moisture_form_solve, to_device)
+from .api import (BC_FLUX, MOISTURE_FORM, MfIn, MfState, device_errors,
+                  init_soilwater_movement, soil_temperature, soilwm_moisture_form_solve,
+                  to_device)
 
 HEAT_STATE = ["t_soisno", "h2osoi_liq", "h2osoi_ice", "excess_ice", "t_h2osfc", "h2osfc",
               "h2osno_no_layers", "snow_depth", "int_snow"]
@@ -42,14 +47,17 @@ HEAT_INPUTS = ["snl", "nbedrock", "z", "dz", "zi", "tkmg", "tkdry", "csol", "suc
 
 class Calibration:
     def __init__(self, nc: int, days: int, steps_per_day: int = 48, seed: int = 3,
-                 device="cuda"):
+                 device="cuda", freezing: bool = False):
         import torch
         self.nc, self.days, self.spd, self.seed, self.device = nc, days, steps_per_day, seed, device
-        host = inputs.calibration_columns(nc, seed)
+        host = inputs.calibration_columns(nc, seed, freezing=freezing)
         self.nlevsno, self.nlevsoi, self.nlevgrnd = host["nlevsno"], host["nlevsoi"], host["nlevgrnd"]
         self.dtime = float(host["dtime"])
         self.base = to_device(host, device)
         self.heat_out = to_device(inputs.heat_outputs(host, diagnostics=False, fill=0.0), device)
+        nlev = self.nlevsno + self.nlevgrnd
+        # melt/freeze labels of the last heat step: part of the path signature
+        self.heat_out["imelt"] = torch.zeros((nlev, nc), dtype=torch.int64, device=device)
         self.cfg = init_soilwater_movement(soilwater_movement_method=MOISTURE_FORM,
                                            lower_boundary_condition=BC_FLUX)
         self.workspace = adjoint.new_workspace(nc, device)
@@ -79,34 +87,40 @@ class Calibration:
                                    self.nlevsoi, self.dtime, self.cfg, self.nlevgrnd)
 
     # ---- forward ----
+    def _heat_dict(self, state, p):
+        b = self.base
+        return ({k: b[k] for k in HEAT_INPUTS} | dict(state) | dict(self.heat_out) | p |
+                {"nc": self.nc})
+
+    def _heat(self, heat):
+        soil_temperature(heat, self.dtime, self.nlevsno, self.nlevgrnd)
+
     def forward(self, m, keep_checkpoints: bool = False):
-        """Daily drainage D [days, nc], the per-day checkpoints of h2osoi_liq and a
-        per-column signature of the substep path (weighted iteration counts)."""
+        """Daily drainage D [days, nc], the per-day checkpoints of the heat and
+        water state and a per-column signature of the path (substep iterations
+        and melt/freeze labels, weighted by step)."""
         import torch
         b = self.base
         state = {k: b[k].clone() for k in HEAT_STATE}
         p = self.params(m)
-        heat = {k: b[k] for k in HEAT_INPUTS} | {k: state[k] for k in HEAT_STATE} | \
-            dict(self.heat_out) | p | {"nc": self.nc}
+        heat = self._heat_dict(state, p)
         D = torch.zeros((self.days, self.nc), dtype=torch.float64, device=self.device)
         ck = [] if keep_checkpoints else None
         sig = torch.zeros(self.nc, dtype=torch.float64, device=self.device)
-        heat_changed_water = torch.zeros((), dtype=torch.bool, device=self.device)
+        imelt = self.heat_out["imelt"]
+        levw = torch.arange(1, imelt.shape[0] + 1, dtype=torch.float64,
+                            device=self.device)[:, None]
         liq = state["h2osoi_liq"]
         for d in range(self.days):
             if keep_checkpoints:
-                ck.append(liq.clone())
+                ck.append({k: state[k].clone() for k in HEAT_STATE})
             for k in range(self.spd):
                 step = d * self.spd + k
-                before = liq.clone()
-                soil_temperature(heat, self.dtime, self.nlevsno, self.nlevgrnd)
-                heat_changed_water |= (liq != before).any()
+                self._heat(heat)
+                sig += (imelt.to(torch.float64) * levw).sum(dim=0) * (1.0e4 * (step + 1))
                 self._richards(liq, p, self.infl(step))
                 D[d] += self.scratch["qcharge"] * self.dtime
                 sig += self.scratch["nsubsteps"] * (step + 1) + self.scratch["nrejected"]
-        if bool(heat_changed_water):
-            raise RuntimeError("the heat step changed h2osoi_liq: a layer froze or melted, so "
-                               "the Richards-only adjoint does not cover this run")
         return D, ck, sig
 
     @staticmethod
@@ -121,25 +135,44 @@ class Calibration:
         Lc = self.column_loss(D, T)
         p = self.params(m)
         b = self.base
-        liq_bar = torch.zeros_like(b["h2osoi_liq"])
+        shape = tuple(b["h2osoi_liq"].shape)
+        t_bar = torch.zeros(shape, dtype=torch.float64, device=self.device)
+        liq_bar = torch.zeros_like(t_bar)
+        ice_bar = torch.zeros_like(t_bar)
         g = torch.zeros((3, self.nc), dtype=torch.float64, device=self.device)
-        states = torch.empty((self.spd,) + tuple(b["h2osoi_liq"].shape), dtype=torch.float64,
-                             device=self.device)
+        empty = lambda: torch.empty((self.spd,) + shape, dtype=torch.float64, device=self.device)
+        liq_r = empty()                     # water at the start of each Richards step
+        t_h, liq_h, ice_h = empty(), empty(), empty()   # state at the start of each heat step
         s = self.scratch
+        device_errors()
         for d in reversed(range(self.days)):
-            liq = ck[d].clone()
+            state = {k: ck[d][k].clone() for k in HEAT_STATE}
+            heat = self._heat_dict(state, p)
+            liq = state["h2osoi_liq"]
             for k in range(self.spd):   # recompute the day's states from its checkpoint
-                states[k].copy_(liq)
+                t_h[k].copy_(state["t_soisno"])
+                liq_h[k].copy_(liq)
+                ice_h[k].copy_(state["h2osoi_ice"])
+                self._heat(heat)
+                liq_r[k].copy_(liq)
                 self._richards(liq, p, self.infl(d * self.spd + k))
             qbar = (2.0 * self.dtime) * (D[d] - T[d])
             for k in reversed(range(self.spd)):
-                st = MfState(states[k], s["smp_l"], s["hk_l"], s["qcharge"], s["nsubsteps"])
+                st = MfState(liq_r[k], s["smp_l"], s["hk_l"], s["qcharge"], s["nsubsteps"])
                 mi = MfIn(b["z"], b["zi"], b["dz"], b["qflx_rootsoi"], p["watsat"], p["hksat"],
                           p["bsw"], b["sucsat"], b["icefrac"], b["zwt"],
                           self.infl(d * self.spd + k))
                 adjoint.soilwm_moisture_form_adjoint(
                     st, mi, None, b["nbedrock"], self.nlevsno, self.nlevsoi, self.nlevgrnd,
                     self.dtime, self.cfg, liq_bar, qbar, self.workspace, grad_scale=g)
+                hstate = dict(heat, t_soisno=t_h[k], h2osoi_liq=liq_h[k], h2osoi_ice=ice_h[k])
+                adjoint.soil_temperature_adjoint(hstate, self.dtime, t_bar, liq_bar, ice_bar,
+                                                 self.nlevsno, self.nlevgrnd, self.nlevsoi,
+                                                 grad_scale=g)
+        flags = device_errors()
+        if flags:
+            raise RuntimeError(f"device error bits {flags:#x} during the gradient (64: a column "
+                               "outside the heat adjoint's scope or with a row interchange)")
         # grad_scale holds sum_j p_j dL/dp_j = m dL/dm for p = m * p0
         return Lc, g / m
 
@@ -158,13 +191,14 @@ class Calibration:
         return mstar, T
 
 
-def fd_check(nc: int = 64, days: int = 1, rel_step: float = 1e-6, seed: int = 5, device="cuda"):
+def fd_check(nc: int = 64, days: int = 1, rel_step: float = 1e-6, seed: int = 5, device="cuda",
+             freezing: bool = False):
     """Adjoint gradient against central finite differences (step rel_step * m)
     for every column and parameter. Columns are independent, so one perturbed run
     per parameter and sign covers all columns; a column whose substep path
     changes under the perturbation is excluded and counted."""
     import torch
-    cal = Calibration(nc, days, seed=seed, device=device)
+    cal = Calibration(nc, days, seed=seed, device=device, freezing=freezing)
     _, T = cal.make_target()
     m0 = torch.ones((3, nc), dtype=torch.float64, device=device)
     Lc, g = cal.loss_and_gradient(m0, T)
@@ -209,10 +243,12 @@ def main(argv=None) -> int:
     ap.add_argument("--fd-days", type=int, default=30)
     args = ap.parse_args(argv)
     out = {}
+    keys = ("columns", "days", "rel_step", "max_rel_err", "max_elementwise_rel_err",
+            "median_rel_err", "excluded_columns")
     r = fd_check(args.fd_columns, args.fd_days)
-    out["fd_check"] = {k: r[k] for k in ("columns", "days", "rel_step", "max_rel_err",
-                                         "max_elementwise_rel_err", "median_rel_err",
-                                         "excluded_columns")}
+    out["fd_check"] = {k: r[k] for k in keys}
+    r = fd_check(args.fd_columns, 2, freezing=True)
+    out["fd_check_freezing"] = {k: r[k] for k in keys}
     print(json.dumps(out), flush=True)
     cal = Calibration(args.nc, args.days)
     _, T = cal.make_target()

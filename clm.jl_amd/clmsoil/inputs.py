@@ -279,15 +279,21 @@ def production_day(nc=1 << 20, seed=2, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrn
     return h
 
 
-def calibration_columns(nc=262144, seed=3, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrnd=NLEVGRND):
-    """Config 3: snow-free, unfrozen columns for the differentiable calibration.
+def calibration_columns(nc=262144, seed=3, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nlevgrnd=NLEVGRND,
+                        freezing=False):
+    """Config 3: snow-free columns for the differentiable calibration.
 
     Texture and pedotransfer as config 0, nbedrock ~ U{5..20}, liquid water
     theta ~ U(0.3, 0.95) x watsat with no ice, t_soisno ~ U(283, 303) K, ground
-    heat flux forcing hs ~ U(0, 40) W/m2 (warming only, so no layer reaches
-    freezing within 30 days and the heat step never changes the water state),
-    dhsdT = -4*0.97*sigma*T_top^3. The Richards lower boundary is BC_FLUX (free
-    drainage), whose qcharge is the drainage the calibration loss uses.
+    heat flux forcing hs ~ U(0, 40) W/m2 (warming: no layer reaches freezing
+    within 30 days), dhsdT = -4*0.97*sigma*T_top^3. The Richards lower boundary
+    is BC_FLUX (free drainage), whose qcharge is the drainage the calibration
+    loss uses.
+
+    freezing=True draws columns around the freezing point instead, so that
+    layers freeze and thaw: t_soisno ~ U(266, 280) K, a quarter of the water of
+    each layer below 273.15 K frozen, hs ~ U(-60, 30) W/m2. Still no snow, no
+    standing surface water and no excess ice.
     """
     rng = np.random.Generator(np.random.PCG64(seed))
     sand, clay, om = _texture(rng, nc)
@@ -299,14 +305,21 @@ def calibration_columns(nc=262144, seed=3, nlevsno=NLEVSNO, nlevsoi=NLEVSOI, nle
     liq = np.zeros((nlev, nc))
     liq[nlevsno:] = rng.uniform(0.3, 0.95, (nlevgrnd, nc)) * props["watsat"] * dzs[:, None] * DENH2O
     t = np.zeros((nlev, nc))
-    t[nlevsno:] = rng.uniform(283.0, 303.0, (nlevgrnd, nc))
+    ice = np.zeros((nlev, nc))
+    if freezing:
+        t[nlevsno:] = rng.uniform(266.0, 280.0, (nlevgrnd, nc))
+        frozen = t[nlevsno:] < TFRZ
+        ice[nlevsno:] = np.where(frozen, 0.25 * liq[nlevsno:], 0.0)
+        liq[nlevsno:] = np.where(frozen, 0.75 * liq[nlevsno:], liq[nlevsno:])
+    else:
+        t[nlevsno:] = rng.uniform(283.0, 303.0, (nlevgrnd, nc))
     nb = rng.integers(5, nlevsoi + 1, nc).astype(np.int64)
-    hs = rng.uniform(0.0, 40.0, nc)
+    hs = rng.uniform(-60.0, 30.0, nc) if freezing else rng.uniform(0.0, 40.0, nc)
     dhsdT = -4.0 * 0.97 * SB * t[nlevsno] ** 3
     return dict(
         nc=nc, nlevsno=nlevsno, nlevsoi=nlevsoi, nlevgrnd=nlevgrnd, dtime=1800.0,
         snl=snl, nbedrock=nb, z=z, dz=dz, zi=zi, t_soisno=t, h2osoi_liq=liq,
-        h2osoi_ice=np.zeros((nlev, nc)), excess_ice=np.zeros((nlevgrnd, nc)),
+        h2osoi_ice=ice, excess_ice=np.zeros((nlevgrnd, nc)),
         watsat=props["watsat"], tkmg=props["tkmg"], tkdry=props["tkdry"],
         csol=props["csol"], bsw=props["bsw"], sucsat=props["sucsat"], hksat=props["hksat"],
         frac_sno_eff=np.zeros(nc), frac_h2osfc=np.zeros(nc),

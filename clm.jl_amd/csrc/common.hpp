@@ -174,6 +174,7 @@ enum DeviceError : unsigned {
     DEV_ERR_TAPE = 8u,
     DEV_ERR_SUBGRID = 16u,
     DEV_ERR_JWT = 32u,
+    DEV_ERR_ADJ = 64u,
 };
 
 }  // namespace clmsoil

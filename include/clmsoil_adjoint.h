@@ -52,6 +52,34 @@ int clmsoil_soilwm_moisture_form_adjoint(const clmsoil_mf_args *a, const clmsoil
                                          void *workspace, int64_t workspace_bytes,
                                          void *stream);
 
+/* Adjoint of one heat step (clmsoil_soil_temperature: soil_temperature.jl:358-593
+ * with phase_change_beta!, :1929-2289) on soil columns that may freeze and thaw:
+ * no snow layers, no standing surface water, no unresolved snow and no excess
+ * ice (snl == 0, frac_h2osfc == 0, h2osno_no_layers == 0, excess_ice == 0).
+ * Other columns, and columns whose elimination would interchange rows, set
+ * CLMSOIL_DEVERR_ADJ and are left unchanged.
+ *
+ * `a` describes the state at the START of the step (t_soisno, h2osoi_liq and
+ * h2osoi_ice are read, nothing is written; the output fields of `a` are not
+ * used and may be NULL). The call replays the step with the heat kernel's
+ * arithmetic, so the melt/freeze labels are the primal's; they, and the max/min
+ * selections, are piecewise constant in the inputs.
+ *
+ *   t_bar, liq_bar, ice_bar  [ld, nlevsno+nlevgrnd] in: dL/d(t_soisno,
+ *                h2osoi_liq, h2osoi_ice) after the step; out: before the step
+ *                (soil levels; snow levels pass through unchanged)
+ *   grad_bsw, grad_watsat    [ld, nlevgrnd] per-layer dL/dp (+=), optional
+ *   grad_scale   [3, ld] optional (+=): rows 0 and 2 get sum_j p_j dL/dp_j for
+ *                p = bsw and watsat (row 1, hksat, is untouched: the heat step
+ *                does not read it) — the layout of the Richards adjoint's.
+ * Reference: Enzyme's reverse pass through soil_temperature! in the reverse
+ * driver (driver_reverse.jl:169-190) with the band_solve! rule
+ * (enzyme_rules.jl:72-107). */
+int clmsoil_soil_temperature_adjoint(const clmsoil_heat_args *a, const clmsoil_levels *lev,
+                                     double dtime, double *t_bar, double *liq_bar,
+                                     double *ice_bar, double *grad_bsw, double *grad_watsat,
+                                     double *grad_scale, void *stream);
+
 /* Reverse of tridiagonal_multi! (tridiagonal.jl:86-142), the batched form of the
  * tridiagonal_solve! rule (enzyme_rules.jl:152-202): with u the forward
  * solution, lambda = A^-T ubar over rows jtop..nlevs; rbar += lambda,

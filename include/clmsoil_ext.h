@@ -22,6 +22,8 @@ extern "C" {
 #define CLMSOIL_DEVERR_TAPE     8u   /* adjoint: more accepted substeps than the tape holds */
 #define CLMSOIL_DEVERR_SUBGRID 16u   /* ground heat flux: patch ranges and patch columns disagree */
 #define CLMSOIL_DEVERR_JWT     32u   /* methane transport: jwt outside 0..nlevsoi   */
+#define CLMSOIL_DEVERR_ADJ     64u   /* heat adjoint: column outside its scope (snow, surface water,
+                                         excess ice) or a row interchange in its solve */
 
 /* Synchronize `stream`, return and clear the current device's error bits (the
  * flags of every call on the device since the last read). The reference raises

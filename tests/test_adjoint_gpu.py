@@ -226,3 +226,23 @@ def test_calibration_gradient_30_days(gpu):
     assert r["max_rel_err"] is not None and r["max_rel_err"] <= 1e-6, r["max_rel_err"]
     assert r["excluded_columns"] <= 4
     assert np.all(np.isfinite(r["grad"]))
+
+
+def test_calibration_gradient_with_freezing_columns(gpu):
+    """Columns around the freezing point: layers freeze and thaw, water moves
+    between h2osoi_liq and h2osoi_ice in the heat step, and the gradient runs
+    through the heat-step adjoint as well (2 days, 96 steps). Layers that sit at
+    their supercooled-water limit switch between frozen and unfrozen labels
+    under a 1e-6 perturbation of the multipliers; such columns are excluded,
+    and at least 16 columns with phase change must remain."""
+    import torch
+    cal = calibration.Calibration(64, 2, seed=5, freezing=True)
+    ice0 = cal.base["h2osoi_ice"].clone()
+    D, ck, _ = cal.forward(torch.ones((3, 64), dtype=torch.float64, device="cuda"),
+                           keep_checkpoints=True)
+    froze = (ck[1]["h2osoi_ice"] != ice0).any(dim=0).cpu().numpy()
+    assert froze.mean() > 0.3
+    r = calibration.fd_check(nc=64, days=2, freezing=True)
+    assert r["max_rel_err"] is not None and r["max_rel_err"] <= 1e-6, r["max_rel_err"]
+    assert (r["valid"].all(axis=0) & froze).sum() >= 16
+    assert np.all(np.isfinite(r["grad"]))
