@@ -24,8 +24,8 @@
 // Scheduling. Columns need from 1 to ~40 substep iterations. A plain launch
 // (one column per group for its lifetime) runs every wavefront for as many
 // iterations as its slowest column. The persistent launch instead keeps a fixed
-// grid resident and lets every group fetch its next column from a work counter
-// as soon as its column is done (wavefront-aggregated atomics, one counter per
+// grid resident and lets groups fetch their next columns from a work counter
+// once enough of them are done (wavefront-aggregated atomics, one counter per
 // XCD over a contiguous column range so neighbouring columns share an L2; idle
 // groups move on to other XCDs' ranges at the end).
 #include <atomic>
@@ -109,7 +109,14 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persi
         // ---- refill groups that have no column ----
         const bool want = !active && !done;
         const unsigned long long need = __ballot(want && k == 0);
-        if (need) {
+        const unsigned long long busy = __ballot(active && k == 0);
+        // A refill runs RColumn::load for the refilling groups only, with the
+        // rest of the wavefront masked off; waiting until REFILL_MIN groups are
+        // idle (or none is busy) shares that pass among them. 7 of 16 groups at
+        // G = 4 was the fastest on the production day (6..8 within 0.5 %; 1:
+        // +6 %, 11: +7 %, 16: +53 %).
+        constexpr int REFILL_MIN = (64 / G) * 7 / 16 > 0 ? (64 / G) * 7 / 16 : 1;
+        if (need && (__popcll(need) >= REFILL_MIN || !busy)) {
             const int leader = __ffsll((long long)need) - 1;
             const int cnt = __popcll(need);
             const int x = (xcd + steal) % NUM_XCD;
