@@ -103,6 +103,7 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persi
         col.lc = richards_lds + threadIdx.x;
     }
     bool active = false;   // group holds a column
+    bool pending = false;  // group's finished column not yet written back
     bool done = false;     // no work left for this group
     int steal = 0;         // XCD ranges exhausted so far (wavefront-uniform)
     for (;;) {
@@ -117,6 +118,11 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persi
         // +6 %, 11: +7 %, 16: +53 %).
         constexpr int REFILL_MIN = (64 / G) * 7 / 16 > 0 ? (64 / G) * 7 / 16 : 1;
         if (need && (__popcll(need) >= REFILL_MIN || !busy)) {
+            // finished columns are written back in the same masked pass
+            if (pending) {
+                col.store(P, k);
+                pending = false;
+            }
             const int leader = __ffsll((long long)need) - 1;
             const int cnt = __popcll(need);
             const int x = (xcd + steal) % NUM_XCD;
@@ -150,11 +156,12 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persi
         }
         if (active) {
             if (col.iterate(P, k)) {
-                col.store(P, k);
+                pending = true;
                 active = false;
             }
         }
     }
+    if (pending) col.store(P, k);   // every group has refilled once more before it is done
 }
 
 template <int G, bool EXP, int LCS>
