@@ -170,6 +170,64 @@ def zero_flux_day(dev, snap, nc, steps):
             "mean_substep_iterations": dev["nsubsteps"].float().mean().item()}
 
 
+def learned_production_day(dev, snap, nc, steps):
+    """The committed first-substep network (trained on config-0 columns) on the
+    production day itself (heat + Richards, BC_WATERTABLE): the day as one graph
+    launch with the network fused into the Richards kernel, against the plain day
+    from the same warmed-up state; then both again launched step by step to
+    count substep iterations and rejections over the whole day."""
+    import torch
+    from clmsoil import api, day, learned
+    pred = learned.DtPredictor.load()
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_WATERTABLE)
+    nrej = torch.zeros(nc, dtype=torch.float64, device="cuda")
+    st = api.MfState(dev["h2osoi_liq"], dev["smp_l"], dev["hk_l"], dev["qcharge"], dev["nsubsteps"],
+                     nrej)
+    mi = api.MfIn(dev["z"], dev["zi"], dev["dz"], dev["qflx_rootsoi"], dev["watsat"], dev["hksat"],
+                  dev["bsw"], dev["sucsat"], dev["icefrac"], dev["zwt"], dev["qflx_infl"],
+                  dev["smpmin"])
+    out = {}
+    for mode in ("plain", "learned"):
+        p = pred if mode == "learned" else None
+        for k, v in snap.items():
+            dev[k].copy_(v)
+        plan = day.DayPlan({k: dev[k] for k in HEAT_KEYS} | {"nc": nc}, st, mi, None,
+                           dev["nbedrock"], 12, 20, 25, 1800.0, cfg, steps, predictor=p)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        plan.launch()
+        b.record()
+        torch.cuda.synchronize()
+        plan.close()
+        ms = a.elapsed_time(b)
+        for k, v in snap.items():
+            dev[k].copy_(v)
+        iters = torch.zeros((), dtype=torch.float64, device="cuda")
+        rej = torch.zeros((), dtype=torch.float64, device="cuda")
+        for _ in range(steps):
+            api.soil_temperature({k: dev[k] for k in HEAT_KEYS} | {"nc": nc}, 1800.0)
+            if p is not None:
+                p.solve(st, mi, None, dev["nbedrock"], 12, 13, 20, 1800.0, cfg, 25)
+            else:
+                api.soilwm_moisture_form_solve(st, mi, None, dev["nbedrock"], 12, 13, 20, 1800.0,
+                                               cfg, 25)
+            iters += dev["nsubsteps"].sum()
+            rej += nrej.sum()
+        torch.cuda.synchronize()
+        assert api.device_errors() == 0
+        out[mode] = {"column_steps_per_s": nc * steps / (ms / 1e3), "ms_per_step": ms / steps,
+                     "mean_iterations": float(iters) / (nc * steps),
+                     "rejected_rate": float(rej) / float(iters)}
+    out["speedup"] = out["learned"]["column_steps_per_s"] / out["plain"]["column_steps_per_s"]
+    out["workload"] = (f"{nc} production-day columns x {steps} steps of heat + Richards "
+                       f"(BC_WATERTABLE), each day one graph launch; the network fused into "
+                       f"the Richards kernel; iteration counts from a step-by-step rerun "
+                       f"(solve without the aquifer recharge pass)")
+    return out
+
+
 def learned_vs_plain(nc, steps):
     """The committed first-substep network against the plain controller: the same
     config-0 columns (20 soil layers, BC_ZERO_FLUX, per-step infiltration
@@ -180,7 +238,7 @@ def learned_vs_plain(nc, steps):
     pred = learned.DtPredictor.load()
     r = learned.compare_controllers(pred, nc, steps, 2024, cfg, hindsight=True)
     r["workload"] = (f"{nc} config-0 columns x {steps} steps, Richards only (BC_ZERO_FLUX), "
-                     f"prediction kernel included in the learned timing; 'hindsight' runs each "
+                     f"the network fused into the Richards kernel; 'hindsight' runs each "
                      f"column-step with the best of the 12 candidate first substeps, found by "
                      f"trying them all first (untimed): the ceiling of any first-substep "
                      f"predictor under the unchanged controller")
@@ -846,6 +904,7 @@ def main(argv=None):
     if not args.no_zero_flux:
         out["zero_flux_day"] = zero_flux_day(dev, snap, nc, args.steps)
     if not args.no_learned:
+        out["learned_production_day"] = learned_production_day(dev, snap, nc, args.steps)
         out["learned_vs_plain"] = learned_vs_plain(nc, args.steps)
     if not args.no_calibration and nc == 1 << 20:
         out["calibration"] = calibration_section()

@@ -21,6 +21,8 @@ def declare(lib) -> None:
     lib.clmsoil_set_richards_schedule.restype = I
     lib.clmsoil_set_richards_residency.argtypes = [I]
     lib.clmsoil_set_richards_residency.restype = I
+    lib.clmsoil_set_richards_refill.argtypes = [I, I]
+    lib.clmsoil_set_richards_refill.restype = I
     lib.clmsoil_set_zd_block.argtypes = [I]
     lib.clmsoil_set_zd_block.restype = I
     lib.clmsoil_math_eval.argtypes = [I, P, P, P, C.c_int64, P]
@@ -37,6 +39,8 @@ def _declare_learned(lib) -> None:
     lib.clmsoil_richards_features.restype = I
     lib.clmsoil_mlp_predict_dtsub0.argtypes = [P, P, D, P, P, P, P]
     lib.clmsoil_mlp_predict_dtsub0.restype = I
+    lib.clmsoil_soilwm_moisture_form_solve_learned.argtypes = [P, P, D, P, P, P, P]
+    lib.clmsoil_soilwm_moisture_form_solve_learned.restype = I
     lib.clmsoil_mlp_forward.argtypes = [P, P, L, L, P, P]
     lib.clmsoil_mlp_forward.restype = I
     lib.clmsoil_mlp_grad.argtypes = [P, P, P, L, L, L, P, P, P]

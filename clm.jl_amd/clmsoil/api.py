@@ -486,6 +486,14 @@ def set_richards_residency(blocks_per_cu: int) -> None:
     _abi.check(L, L.clmsoil_set_richards_residency(int(blocks_per_cu)), "set_richards_residency")
 
 
+def set_richards_refill(plain_sixteenths: int, learned_sixteenths: int) -> None:
+    """Resident Richards grid: refill a wavefront's idle lane groups once n/16 of
+    them are idle (plain step, step with the fused first-substep network)."""
+    L = _lib()
+    _abi.check(L, L.clmsoil_set_richards_refill(int(plain_sixteenths), int(learned_sixteenths)),
+               "set_richards_refill")
+
+
 def set_zd_block(block: int) -> None:
     """Workgroup size of the Zeng-Decker 2009 step (64, 128 or 256)."""
     L = _lib()

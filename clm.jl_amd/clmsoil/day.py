@@ -3,8 +3,9 @@
 CLM.jl launches every kernel of a step separately and synchronises after each
 (src/infrastructure/kernels.jl:85-91). A DayPlan records `nsteps` steps on
 fixed arrays once and replays them with one graph launch. Each step is the heat
-step, then the optional learned first-substep prediction, then the Richards
-step with its aquifer pass.
+step, then the Richards step with its aquifer pass. With a predictor, the
+learned network sets each column's first substep inside the Richards kernel
+(fused=True), or in a launch of its own before it (fused=False; same results).
 """
 from __future__ import annotations
 
@@ -19,7 +20,8 @@ from .bind import byref, heat_args, heat_config, levels, mf_args
 class DayPlan:
     def __init__(self, heat_state: dict, st: MfState, mfin: MfIn, mask, nbedrock, nlevsno: int,
                  nlevsoi: int, nlevgrnd: int, dtime: float, cfg: SoilWaterMovementConfig,
-                 nsteps: int, predictor=None, snow_thermal_cond_method: str = "Jordan1991"):
+                 nsteps: int, predictor=None, snow_thermal_cond_method: str = "Jordan1991",
+                 fused: bool = True):
         import torch
         code = {"Jordan1991": 1, "Sturm1997": 2}.get(snow_thermal_cond_method)
         if code is None:
@@ -31,7 +33,7 @@ class DayPlan:
         hd.update(nlevsno=nlevsno, nlevgrnd=nlevgrnd, nlevsoi=nlevsoi)
         md = _mf_dict(st, mfin, mask, nbedrock, nlevsno, nlevsoi, nlevgrnd)
         self.dtsub0 = None
-        if predictor is not None:
+        if predictor is not None and not fused:
             self.dtsub0 = torch.full((int(md["nc"]),), float(dtime), dtype=torch.float64,
                                      device=nbedrock.device)
         c = cfg.to_abi()
@@ -43,7 +45,7 @@ class DayPlan:
         p = C.c_void_p()
         pp = (C.c_void_p(predictor.params.data_ptr()) if predictor else None,
               C.c_void_p(predictor.norm.data_ptr()) if predictor else None,
-              C.c_void_p(self.dtsub0.data_ptr()) if predictor else None)
+              C.c_void_p(self.dtsub0.data_ptr()) if self.dtsub0 is not None else None)
         _abi.check(L, L.clmsoil_day_plan_create(byref(ha), byref(ma), byref(lev), float(dtime),
                                                 byref(hc), byref(c), *pp, int(nsteps), C.byref(p)),
                    "day_plan_create")

@@ -31,7 +31,8 @@ from pathlib import Path
 import numpy as np
 
 from . import _abi
-from .api import MfIn, MfState, SoilWaterMovementConfig, _mf_dict, _stream, soilwm_moisture_form_solve
+from .api import (MfIn, MfState, SoilWaterMovementConfig, _mf_dict, _stream, richards_workspace,
+                  soilwm_moisture_form_solve)
 from .bind import byref, levels, mf_args
 
 NIN, NHID, NPARAM, NNORM, GRAD_TILE = 24, 64, 5825, 50, 64
@@ -111,6 +112,29 @@ class DtPredictor:
                                                    _p(self.params), _p(self.norm), _p(out),
                                                    _stream()), "mlp_predict_dtsub0")
         return out
+
+    def solve(self, st: MfState, mfin: MfIn, mask, nbedrock, joff: int, joff_zi: int,
+              nlevsoi: int, dtime: float, cfg: SoilWaterMovementConfig,
+              nlevgrnd: int | None = None, workspace=None) -> None:
+        """soilwm_moisture_form_solve with the network fused into the Richards
+        kernel: each column's first substep is predicted as it is loaded
+        (clmsoil_soilwm_moisture_form_solve_learned). Same arguments as
+        api.soilwm_moisture_form_solve; mfin.dtsub0 must be None."""
+        from .api import MOISTURE_FORM, CLMSoilError, _check_moisture_form_bc, richards_workspace
+        _check_moisture_form_bc(cfg)
+        if joff_zi != joff + 1:
+            raise CLMSoilError("joff_zi must equal joff + 1 (zi has one more level)")
+        c = cfg.to_abi()
+        c.soilwater_movement_method = MOISTURE_FORM
+        d = _mf_dict(st, mfin, mask, nbedrock, joff, nlevsoi, nlevgrnd or nlevsoi + 5)
+        if d["nc"] == 0:
+            return
+        d["workspace"] = workspace if workspace is not None else richards_workspace(nbedrock.device)
+        L = _lib()
+        a, lev = mf_args(d), levels(d)
+        _abi.check(L, L.clmsoil_soilwm_moisture_form_solve_learned(
+            byref(a), byref(lev), float(dtime), byref(c), _p(self.params), _p(self.norm),
+            _stream()), "soilwm_moisture_form_solve_learned")
 
     def save(self, path) -> None:
         np.savez(path, params=self.params.detach().cpu().numpy(),
@@ -208,10 +232,17 @@ def mf_views(dev, dtsub0=None):
     return st, mi
 
 
-def run_step(dev, cfg: SoilWaterMovementConfig, dtsub0=None):
+def run_step(dev, cfg: SoilWaterMovementConfig, dtsub0=None, pred: "DtPredictor | None" = None,
+             workspace=None):
+    """One Richards step of the config-0 state `dev`: plain, from given first
+    substeps `dtsub0`, or with `pred`'s network fused into the kernel."""
     st, mi = mf_views(dev, dtsub0)
-    soilwm_moisture_form_solve(st, mi, None, dev["nbedrock"], dev["nlevsno"], dev["nlevsno"] + 1,
-                               dev["nlevsoi"], dev["dtime"], cfg, dev["nlevgrnd"])
+    args = (st, mi, None, dev["nbedrock"], dev["nlevsno"], dev["nlevsno"] + 1, dev["nlevsoi"],
+            dev["dtime"], cfg, dev["nlevgrnd"], workspace)
+    if pred is not None:
+        pred.solve(*args)
+    else:
+        soilwm_moisture_form_solve(*args)
 
 
 def log_plain_controller(nc: int, steps: int, seed: int, cfg: SoilWaterMovementConfig,
@@ -318,8 +349,9 @@ def compare_controllers(pred: DtPredictor | None, nc: int, steps: int, seed: int
     controller and with the learned first substep (when `pred` is given), and,
     with `hindsight`, with each column's best candidate first substep chosen by
     trying them all first (untimed): the ceiling any first-substep predictor can
-    reach under the unchanged controller. Returns, per controller: column-steps/s
-    of the timed GPU work (prediction included), mean substep iterations,
+    reach under the unchanged controller. The learned controller runs the
+    network fused into the Richards kernel. Returns, per controller:
+    column-steps/s of the timed GPU work (prediction included), mean substep iterations,
     rejected-substep rate, and the largest water-balance residual
     |dW - (infl - qcharge - roots) dt| in kg/m2."""
     import torch
@@ -329,7 +361,8 @@ def compare_controllers(pred: DtPredictor | None, nc: int, steps: int, seed: int
     for mode in modes:
         dev, infl = config0_day(nc, seed, device)
         dtime = float(dev["dtime"])
-        dt0 = torch.empty(nc, dtype=torch.float64, device=device) if mode == "learned" else None
+        dt0 = None
+        ws = richards_workspace(device)
         work = dict(dev)
         evs = []
         iters = torch.zeros((), dtype=torch.float64, device=device)
@@ -344,10 +377,9 @@ def compare_controllers(pred: DtPredictor | None, nc: int, steps: int, seed: int
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
             if mode == "learned":
-                st, mi = mf_views(dev)
-                pred.predict_dtsub0(st, mi, None, dev["nbedrock"], dev["nlevsno"], dev["nlevsoi"],
-                                    dev["nlevgrnd"], dtime, out=dt0)
-            run_step(dev, cfg, dt0)
+                run_step(dev, cfg, pred=pred, workspace=ws)   # inference fused in the kernel
+            else:
+                run_step(dev, cfg, dt0, workspace=ws)
             e1.record()
             evs.append((e0, e1))
             roots = dev["qflx_rootsoi"][:int(dev["nlevsoi"])].sum(dim=0)

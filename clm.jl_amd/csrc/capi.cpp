@@ -17,6 +17,7 @@
 #include "../../include/clmsoil.h"
 #include "../../include/clmsoil_ext.h"
 #include "../../include/clmsoil_adjoint.h"
+#include "../../include/clmsoil_learned.h"
 #include "../../include/clmsoil_zd09.h"
 #include "../../include/clmsoil_ghf.h"
 #include "../../include/clmsoil_hydro.h"
@@ -157,6 +158,7 @@ RichardsParams richards_params(const clmsoil_mf_args *a, const clmsoil_levels *l
     P.expensive = cfg->flux_calculation == CLMSOIL_FLUX_EXPENSIVE;
     P.aquifer_pass = aquifer ? 1 : 0;
     P.max_iter = 100000;
+    P.refill_sixteenths = g_tuning.richards_refill.load(std::memory_order_relaxed);
     P.dtime = dtime; P.e_ice = cfg->e_ice; P.dtmin = cfg->dtmin;
     P.xtu = cfg->xTolerUpper; P.xtl = cfg->xTolerLower; P.verysmall = cfg->verySmall;
     P.err = err;
@@ -204,6 +206,23 @@ int run_richards(const RichardsParams &P, hipStream_t stream) {
                            g_tuning.richards_block.load(std::memory_order_relaxed), persistent,
                            g_tuning.richards_blocks_per_cu.load(std::memory_order_relaxed),
                            stream) == hipSuccess
+               ? CLMSOIL_OK
+               : CLMSOIL_ERR_HIP;
+}
+
+// The step with the first-substep network fused into the column load (G = 4,
+// 256-thread blocks whatever the tuning says; the persistent grid when the
+// workspace is there).
+int run_richards_learned(const RichardsParams &P, hipStream_t stream) {
+    const bool persistent =
+        g_tuning.richards_persistent.load(std::memory_order_relaxed) != 0 && P.work_counters;
+    if (P.call_err) {
+        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, P.call_err, 16);
+        if (hipGetLastError() != hipSuccess) return CLMSOIL_ERR_HIP;
+    }
+    return launch_richards_learned(P, persistent,
+                                   g_tuning.richards_blocks_per_cu.load(std::memory_order_relaxed),
+                                   stream) == hipSuccess
                ? CLMSOIL_OK
                : CLMSOIL_ERR_HIP;
 }
@@ -308,6 +327,15 @@ int clmsoil_set_richards_residency(int blocks_per_cu) {
     return CLMSOIL_OK;
 }
 
+int clmsoil_set_richards_refill(int plain_sixteenths, int learned_sixteenths) {
+    if (plain_sixteenths < 1 || plain_sixteenths > 16 || learned_sixteenths < 1 ||
+        learned_sixteenths > 16)
+        return CLMSOIL_ERR_ARG;
+    g_tuning.richards_refill.store(plain_sixteenths);
+    g_tuning.richards_refill_learned.store(learned_sixteenths);
+    return CLMSOIL_OK;
+}
+
 int clmsoil_set_zd_block(int block) {
     if (block != 64 && block != 128 && block != 256) return CLMSOIL_ERR_ARG;
     g_tuning.zd_block.store(block);
@@ -347,6 +375,25 @@ int clmsoil_soilwm_moisture_form_solve_static(const clmsoil_mf_args *a,
     m.workspace_bytes = RICHARDS_WORKSPACE_BYTES;
     RichardsParams P = richards_params(&m, lev, dtime, cfg, false, s->err);
     return run_richards(P, (hipStream_t)stream);
+}
+
+int clmsoil_soilwm_moisture_form_solve_learned(const clmsoil_mf_args *a,
+                                               const clmsoil_levels *lev, double dtime,
+                                               const clmsoil_swm_config *cfg,
+                                               const float *params, const float *norm,
+                                               void *stream) {
+    int st = validate_mf(a, lev, dtime, cfg);
+    if (st) return st;
+    if (!params || !norm) return CLMSOIL_ERR_NOT_TRAINED;
+    if (a->dtsub0) return CLMSOIL_ERR_ARG;   // the network supplies the first substep
+    if (a->nc == 0) return CLMSOIL_OK;
+    DeviceState *s = nullptr;
+    if ((st = current_device_state(&s))) return st;
+    RichardsParams P = richards_params(a, lev, dtime, cfg, false, s->err);
+    P.mlp = params;
+    P.mlp_norm = norm;
+    P.refill_sixteenths = g_tuning.richards_refill_learned.load(std::memory_order_relaxed);
+    return run_richards_learned(P, (hipStream_t)stream);
 }
 
 int clmsoil_soilwater_moisture_form(const clmsoil_mf_args *a, const clmsoil_levels *lev,

@@ -26,6 +26,7 @@ RichardsParams richards_params(const clmsoil_mf_args *a, const clmsoil_levels *l
                                double dtime, const clmsoil_swm_config *cfg, bool aquifer,
                                unsigned *err);
 int run_richards(const RichardsParams &P, hipStream_t stream);
+int run_richards_learned(const RichardsParams &P, hipStream_t stream);
 int run_heat(const clmsoil_heat_args *a, const clmsoil_levels *lev, double dtime,
              const clmsoil_heat_config *cfg, hipStream_t stream, unsigned *err);
 }  // namespace clmsoil
@@ -65,8 +66,11 @@ int clmsoil_day_plan_create(const clmsoil_heat_args *heat, const clmsoil_mf_args
         heat_cfg->snow_thermal_cond_method != 2)
         return CLMSOIL_ERR_ARG;
     const bool learned = mlp_params || mlp_norm || dtsub0;
-    if (learned && (!mlp_params || !mlp_norm || !dtsub0)) return CLMSOIL_ERR_NOT_TRAINED;
+    if (learned && (!mlp_params || !mlp_norm)) return CLMSOIL_ERR_NOT_TRAINED;
     if (learned && mf->dtsub0) return CLMSOIL_ERR_ARG;
+    // the network inside the Richards kernel, unless the caller wants the
+    // predictions in dtsub0 (a separate launch before each step)
+    const bool fused = learned && !dtsub0;
     DeviceState *s = nullptr;
     if ((st = current_device_state(&s))) return st;
     auto *p = new (std::nothrow) clmsoil_day_plan();
@@ -78,12 +82,17 @@ int clmsoil_day_plan_create(const clmsoil_heat_args *heat, const clmsoil_mf_args
         return CLMSOIL_ERR_HIP;
     }
     clmsoil_mf_args m = *mf;
-    if (learned) m.dtsub0 = dtsub0;
+    if (learned && !fused) m.dtsub0 = dtsub0;
     m.workspace = p->workspace;
     m.workspace_bytes = RICHARDS_WORKSPACE_BYTES;
     const int bc = swm_cfg->lower_boundary_condition;
     const bool aquifer = bc == CLMSOIL_BC_WATERTABLE || bc == CLMSOIL_BC_AQUIFER;
     RichardsParams P = richards_params(&m, lev, dtime, swm_cfg, aquifer, s->err);
+    if (fused) {
+        P.mlp = mlp_params;
+        P.mlp_norm = mlp_norm;
+        P.refill_sixteenths = g_tuning.richards_refill_learned.load(std::memory_order_relaxed);
+    }
 
     hipStream_t cs = nullptr;
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
@@ -96,9 +105,9 @@ int clmsoil_day_plan_create(const clmsoil_heat_args *heat, const clmsoil_mf_args
     } else {
         for (int k = 0; k < nsteps && st == CLMSOIL_OK; ++k) {
             st = run_heat(heat, lev, dtime, heat_cfg, cs, s->err);
-            if (st == CLMSOIL_OK && learned)
+            if (st == CLMSOIL_OK && learned && !fused)
                 st = clmsoil_mlp_predict_dtsub0(&m, lev, dtime, mlp_params, mlp_norm, dtsub0, cs);
-            if (st == CLMSOIL_OK) st = run_richards(P, cs);
+            if (st == CLMSOIL_OK) st = fused ? run_richards_learned(P, cs) : run_richards(P, cs);
         }
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(cs, &g);

@@ -3,10 +3,12 @@
 // and the Adam update. No counterpart in the reference (CLM.jl starts every
 // column at dtsub = dtime, src/biogeophys/soil_water_movement.jl:1259).
 //
-// Inference and feature extraction run one lane per column. The weights are read
-// at wavefront-uniform addresses (the loops are fully unrolled over constant
-// indices), so they come through scalar loads and the scalar cache; only the
-// column's features and activations live in VGPRs.
+// Feature extraction and the plain forward pass run one lane per column; their
+// weights are read at wavefront-uniform addresses (the loops are fully unrolled
+// over constant indices), so they come through scalar loads and the scalar
+// cache. The dtsub0 predictor runs the network on the matrix cores for 16
+// columns per wavefront (mlp_wave.hpp), exactly as the Richards kernel does when
+// the inference is fused into its column load.
 //
 // Training: clmsoil_mlp_grad gives one wavefront a tile of 64 samples. Each lane
 // runs forward and backward for its sample and leaves its activations and deltas
@@ -20,6 +22,7 @@
 #include "../../include/clmsoil.h"
 #include "../../include/clmsoil_learned.h"
 #include "common.hpp"
+#include "mlp_wave.hpp"
 #include "state.hpp"
 
 namespace clmsoil {
@@ -115,16 +118,46 @@ __global__ __launch_bounds__(256) void features_kernel(FeatParams F, float *x, i
     for (int i = 0; i < NIN; ++i) x[(int64_t)i * ldx + c] = f[i];
 }
 
+// Prediction for 16 columns per wavefront, 4 lanes per column (the Richards
+// kernel's layout): features of column c in lanes 4c'..4c'+3 as the fused
+// inference in RColumn::learned_first_substep forms them, then mlp_wave16, so
+// dtsub0 here equals the fused first substep bit for bit (before its clamp).
 __global__ __launch_bounds__(256) void predict_kernel(FeatParams F, const float *__restrict__ P,
                                                       double *dtsub0) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= F.a.nc) return;
-    if (F.a.mask && !F.a.mask[c]) return;
-    float f[NIN];
-    column_features(F, c, f);
-    const float y = mlp_eval(P, f);
-    const double lr = (double)F.norm[2 * NIN] + (double)F.norm[2 * NIN + 1] * (double)y;
-    dtsub0[c] = F.dtime * exp(lr);
+    const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+    const int k = (int)(threadIdx.x & 3);
+    const clmsoil_mf_args &a = F.a;
+    const bool live = c < a.nc && (!a.mask || a.mask[c]);
+    int n = live ? (int)a.nbedrock[c] : 0;
+    n = n < 0 ? 0 : (n > F.nlevsoi ? F.nlevsoi : n);
+    float fe[6];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const int j = 4 * i + k + 1;
+        float raw = 0.0f;
+        if (live && j <= n) {
+            const int64_t os = c + (int64_t)(F.nlevsno + j - 1) * a.ld;
+            double v = jl_max(a.h2osoi_liq[os], 1.0e-6) / (M_TO_MM * a.dz[os]);
+            v = v / a.watsat[c + (int64_t)(j - 1) * a.ld];
+            v = jl_min(v, 1.0);
+            v = jl_max(0.01, v);
+            raw = (float)v;
+        }
+        fe[i] = mlp_standardise(F.norm, 4 * i + k, raw);
+    }
+    float r5 = 0.0f;
+    if (live) {
+        if (k == 0) r5 = (float)log(a.hksat[c]);
+        else if (k == 1) r5 = (float)a.bsw[c];
+        else if (k == 2) r5 = (float)a.qflx_infl[c];
+        else r5 = (float)F.dtime;
+    }
+    fe[5] = mlp_standardise(F.norm, 20 + k, r5);
+    const float y = mlp_wave16(P, fe[0], fe[1], fe[2], fe[3], fe[4], fe[5]);   // every lane of the wavefront
+    if (live && k == 0) {
+        const double lr = (double)F.norm[2 * NIN] + (double)F.norm[2 * NIN + 1] * (double)y;
+        dtsub0[c] = F.dtime * exp(lr);
+    }
 }
 
 __global__ __launch_bounds__(256) void forward_kernel(const float *__restrict__ P,
@@ -309,7 +342,7 @@ int clmsoil_mlp_predict_dtsub0(const clmsoil_mf_args *a, const clmsoil_levels *l
     if (a->nc == 0) return CLMSOIL_OK;
     if (!params || !norm) return CLMSOIL_ERR_NOT_TRAINED;
     if (!dtsub0) return CLMSOIL_ERR_NULL;
-    const int64_t grid = (a->nc + 255) / 256;
+    const int64_t grid = (a->nc * 4 + 255) / 256;
     hipLaunchKernelGGL(predict_kernel, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream,
                        F, params, dtsub0);
     return launched();

@@ -20,6 +20,7 @@
 // them all).
 #pragma once
 #include "common.hpp"
+#include "mlp_wave.hpp"
 #include "richards.hpp"
 
 namespace clmsoil {
@@ -178,6 +179,45 @@ struct RColumn {
         qc = 0.0;
         props_valid = false;
         return true;
+    }
+
+    // First substep from the network of clmsoil_learned.h for a column just
+    // loaded (`on`): its features are the node saturations of the loaded state
+    // (the reference's s2, :1273-1286, 0 below bedrock), log ksat and b of layer
+    // 1, the infiltration flux and dtime; dtsub0 = dtime * exp(y_mean + y_scale*y)
+    // clamped to [min(dtmin, dtime), dtime], dtime when it is NaN. Every lane of
+    // the wavefront calls this; lanes with `on` false contribute nothing.
+    __device__ __forceinline__ void learned_first_substep(const RichardsParams &P, int k, bool on) {
+        static_assert(G == 4, "the network is evaluated for groups of 4 lanes");
+        float fe[6];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            const int j = i * G + k + 1;
+            float raw = 0.0f;
+            if (on && j <= n) {
+                double v = jl_max(liq[i], 1.0e-6) / mdz[i];
+                v = v / watsat[i];
+                v = jl_min(v, 1.0);
+                v = jl_max(0.01, v);
+                raw = (float)v;
+            }
+            fe[i] = mlp_standardise(P.mlp_norm, 4 * i + k, raw);
+        }
+        float r5 = 0.0f;
+        if (on) {
+            if (k == 0) r5 = (float)log(P.hksat[c]);
+            else if (k == 1) r5 = (float)P.bsw[c];
+            else if (k == 2) r5 = (float)qflx_infl;
+            else r5 = (float)P.dtime;
+        }
+        fe[5] = mlp_standardise(P.mlp_norm, 20 + k, r5);
+        const float y = mlp_wave16(P.mlp, fe[0], fe[1], fe[2], fe[3], fe[4], fe[5]);
+        if (on) {
+            const double dtime = P.dtime;
+            const double lr = (double)P.mlp_norm[2 * mlp::NIN] + (double)P.mlp_norm[2 * mlp::NIN + 1] * (double)y;
+            const double d0 = dtime * exp(lr), lo = jl_min(P.dtmin, dtime);
+            dtsub = (d0 != d0) ? dtime : jl_min(jl_max(d0, lo), dtime);
+        }
     }
 
     // Hydraulic properties, interface fluxes and the flux terms of the system

@@ -67,21 +67,28 @@ static size_t richards_lds_bytes() {
            sizeof(double);
 }
 
-// One column per group for the group's lifetime.
-template <int G, bool EXP, int LCS>
+// One column per group for the group's lifetime. LEARN: the first substep comes
+// from the network, evaluated by the whole wavefront right after the loads.
+template <int G, bool EXP, int LCS, bool LEARN>
 __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kernel(RichardsParams P) {
     fill_math_tables();
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t c = gtid / G;
     const int k = (int)(threadIdx.x % G);
-    if (c >= P.nc) return;
-    if (P.mask && !P.mask[c]) return;
+    const bool live = c < P.nc && (!P.mask || P.mask[c]);
     RColumn<G, EXP, false, LCS> col;
     if constexpr (LCS > 0) {
         extern __shared__ double richards_lds[];
         col.lc = richards_lds + threadIdx.x;
     }
-    if (!col.load(P, c, k)) return;
+    if constexpr (LEARN) {
+        const bool ok = live && col.load(P, c, k);
+        col.learned_first_substep(P, k, ok);   // all lanes of the wavefront
+        if (!ok) return;
+    } else {
+        if (!live) return;
+        if (!col.load(P, c, k)) return;
+    }
     while (!col.iterate(P, k)) {
     }
     col.store(P, k);
@@ -89,7 +96,7 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kerne
 
 // Persistent: a resident grid whose groups refill from per-XCD column ranges.
 // Every wavefront leaves the loop once all eight ranges are exhausted.
-template <int G, bool EXP, int LCS>
+template <int G, bool EXP, int LCS, bool LEARN>
 __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persistent(RichardsParams P) {
     fill_math_tables();
     const int k = (int)(threadIdx.x % G);
@@ -102,6 +109,9 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persi
         extern __shared__ double richards_lds[];
         col.lc = richards_lds + threadIdx.x;
     }
+    // refill threshold: P.refill_sixteenths / 16 of the wavefront's groups idle
+    int refill_min = (64 / G) * P.refill_sixteenths / 16;
+    refill_min = refill_min < 1 ? 1 : refill_min;
     bool active = false;   // group holds a column
     bool pending = false;  // group's finished column not yet written back
     bool done = false;     // no work left for this group
@@ -111,13 +121,11 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persi
         const bool want = !active && !done;
         const unsigned long long need = __ballot(want && k == 0);
         const unsigned long long busy = __ballot(active && k == 0);
-        // A refill runs RColumn::load for the refilling groups only, with the
-        // rest of the wavefront masked off; waiting until REFILL_MIN groups are
-        // idle (or none is busy) shares that pass among them. 7 of 16 groups at
-        // G = 4 was the fastest on the production day (6..8 within 0.5 %; 1:
-        // +6 %, 11: +7 %, 16: +53 %).
-        constexpr int REFILL_MIN = (64 / G) * 7 / 16 > 0 ? (64 / G) * 7 / 16 : 1;
-        if (need && (__popcll(need) >= REFILL_MIN || !busy)) {
+        // A refill runs RColumn::load (and with LEARN the network, for the whole
+        // wavefront) for the refilling groups only, with the rest of the
+        // wavefront masked off; waiting until refill_min groups are idle (or none
+        // is busy) shares that pass among them.
+        if (need && (__popcll(need) >= refill_min || !busy)) {
             // finished columns are written back in the same masked pass
             if (pending) {
                 col.store(P, k);
@@ -143,6 +151,8 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persi
                 const bool on = !P.mask || P.mask[cidx];
                 active = on ? col.load(P, cidx, k) : false;
             }
+            // first substeps of the columns just loaded: the whole wavefront
+            if constexpr (LEARN) col.learned_first_substep(P, k, cidx >= 0 && active);
             // a wavefront that got fewer columns than it asked for moves on to the
             // next XCD range; after the last range its idle groups are done
             if (__ballot(want && k == 0 && cidx < 0)) {
@@ -164,31 +174,31 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persi
     if (pending) col.store(P, k);   // every group has refilled once more before it is done
 }
 
-template <int G, bool EXP, int LCS>
+template <int G, bool EXP, int LCS, bool LEARN = false>
 static hipError_t launch_g(const RichardsParams &P, hipStream_t stream, int block) {
     const int64_t threads = P.nc * G;
     const int64_t grid = (threads + block - 1) / block;
     if (richards_lds_bytes<G, EXP, LCS>() > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)richards_mf_kernel<G, EXP, LCS>,
+        hipError_t e = hipFuncSetAttribute((const void *)richards_mf_kernel<G, EXP, LCS, LEARN>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)richards_lds_bytes<G, EXP, LCS>());
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((richards_mf_kernel<G, EXP, LCS>), dim3((unsigned)grid), dim3(block),
+    hipLaunchKernelGGL((richards_mf_kernel<G, EXP, LCS, LEARN>), dim3((unsigned)grid), dim3(block),
                        (richards_lds_bytes<G, EXP, LCS>()), stream, P);
     return hipGetLastError();
 }
 
 // Resident blocks per CU of richards_mf_persistent at a block size, from the
 // occupancy calculator, cached per (device, kernel, block) after the first query.
-template <int G, bool EXP, int LCS>
+template <int G, bool EXP, int LCS, bool LEARN>
 static int resident_blocks(int dev, int block) {
     static std::atomic<int> cache[16][3];
     const int bi = block == 64 ? 0 : (block == 128 ? 1 : 2);
     if (dev < 0 || dev >= 16) dev = 0;
     int nb = cache[dev][bi].load(std::memory_order_relaxed);
     if (nb > 0) return nb;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G, EXP, LCS>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G, EXP, LCS, LEARN>,
                                                      block, richards_lds_bytes<G, EXP, LCS>()) !=
         hipSuccess)
         return -1;
@@ -197,7 +207,7 @@ static int resident_blocks(int dev, int block) {
     return nb;
 }
 
-template <int G, bool EXP, int LCS>
+template <int G, bool EXP, int LCS, bool LEARN = false>
 static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int block,
                            int blocks_per_cu) {
     int dev = 0, cus = 0;
@@ -205,13 +215,13 @@ static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int bloc
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return hipErrorUnknown;
     if (richards_lds_bytes<G, EXP, LCS>() > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)richards_mf_persistent<G, EXP, LCS>,
+        hipError_t e = hipFuncSetAttribute((const void *)richards_mf_persistent<G, EXP, LCS, LEARN>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)richards_lds_bytes<G, EXP, LCS>());
         if (e != hipSuccess) return e;
     }
     int nb = blocks_per_cu;
-    if (nb <= 0) nb = resident_blocks<G, EXP, LCS>(dev, block);
+    if (nb <= 0) nb = resident_blocks<G, EXP, LCS, LEARN>(dev, block);
     if (nb <= 0) return hipErrorUnknown;
     // Any grid size is correct: groups only take work from the counters and never
     // wait on each other, so blocks that start late simply find less work.
@@ -219,7 +229,7 @@ static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int bloc
     grid = (grid + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     const int64_t need = (P.nc * G + block - 1) / block;
     if (grid > need) grid = (need + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
-    hipLaunchKernelGGL((richards_mf_persistent<G, EXP, LCS>), dim3((unsigned)grid), dim3(block),
+    hipLaunchKernelGGL((richards_mf_persistent<G, EXP, LCS, LEARN>), dim3((unsigned)grid), dim3(block),
                        (richards_lds_bytes<G, EXP, LCS>()), stream, P);
     return hipGetLastError();
 }
@@ -251,6 +261,17 @@ hipError_t launch_richards(const RichardsParams &P, int G, int block, bool persi
     if (P.nc == 0) return hipSuccess;
     return P.expensive ? launch_exp<true>(P, G, block, persistent, blocks_per_cu, stream)
                        : launch_exp<false>(P, G, block, persistent, blocks_per_cu, stream);
+}
+
+hipError_t launch_richards_learned(const RichardsParams &P, bool persistent, int blocks_per_cu,
+                                   hipStream_t stream) {
+    if (P.nc == 0) return hipSuccess;
+    if (!P.mlp || !P.mlp_norm) return hipErrorInvalidValue;
+    if (persistent)
+        return P.expensive ? launch_p<4, true, 256, true>(P, stream, 256, blocks_per_cu)
+                           : launch_p<4, false, 256, true>(P, stream, 256, blocks_per_cu);
+    return P.expensive ? launch_g<4, true, 256, true>(P, stream, 256)
+                       : launch_g<4, false, 256, true>(P, stream, 256);
 }
 
 }  // namespace clmsoil

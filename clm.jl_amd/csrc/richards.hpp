@@ -12,11 +12,16 @@ struct RichardsParams {
     double *h2osoi_liq, *smp_l, *hk_l, *qcharge, *nsubsteps, *nrejected;
     const double *z, *zi, *dz, *qflx_rootsoi, *watsat, *hksat, *bsw, *sucsat, *icefrac;
     const double *zwt, *qflx_infl, *smpmin, *dtsub0;
+    // first-substep network (packed weights and normalisation block of
+    // clmsoil_learned.h), evaluated for each column as it is loaded; NULL = none
+    const float *mlp, *mlp_norm;
     int nlevsno, nlevsoi;
     int lower_bc;
     int expensive;
     int aquifer_pass;       // run compute_qcharge!'s pass after the solve
     int max_iter;           // substep-iteration cap (hang guard)
+    int refill_sixteenths;  // persistent launch: refill when this many 16ths of a
+                            // wavefront's groups are idle (1..16)
     double dtime, e_ice, dtmin, xtu, xtl, verysmall;
     unsigned *err;          // device error word
     unsigned *call_err;     // the call's own error word (workspace), or NULL
@@ -34,5 +39,9 @@ struct RichardsParams {
 // calculator says fit on a CU).
 hipError_t launch_richards(const RichardsParams &P, int G, int block, bool persistent,
                            int blocks_per_cu, hipStream_t stream);
+// The same step with the first-substep network fused into the column load
+// (P.mlp and P.mlp_norm set): G = 4 lanes per column, 256-thread blocks.
+hipError_t launch_richards_learned(const RichardsParams &P, bool persistent, int blocks_per_cu,
+                                   hipStream_t stream);
 
 }  // namespace clmsoil

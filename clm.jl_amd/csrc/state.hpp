@@ -18,6 +18,12 @@ struct Tuning {
     std::atomic<int> heat_block{256};
     std::atomic<int> richards_persistent{1};   // resident grid with per-group column refill
     std::atomic<int> richards_blocks_per_cu{0};  // persistent grid size; 0 = occupancy query
+    // persistent grid: refill once this many 16ths of a wavefront's groups are
+    // idle, plain and with the fused first-substep network (which evaluates the
+    // whole wavefront at every refill). Measured on the production day at G = 4:
+    // plain 7 fastest (6..8 within 0.5 %; 1: +6 %, 11: +7 %, 16: +53 %).
+    std::atomic<int> richards_refill{7};
+    std::atomic<int> richards_refill_learned{7};
     std::atomic<int> zd_block{128};              // Zeng-Decker 2009 step, one lane per column
     std::atomic<int> ghf_block{256};             // ground heat flux
 };

@@ -6,8 +6,9 @@
  * arrays as one graph. Each step is the heat step (clmsoil_soil_temperature)
  * followed by the Richards step (clmsoil_soilwater_moisture_form, which adds the
  * aquifer recharge pass when the lower boundary condition uses one).
- * Optionally the learned first-substep predictor runs before each Richards
- * step. One graph launch then replays the whole day.
+ * Optionally the learned first-substep predictor sets each column's first
+ * substep, inside the Richards kernel or as a launch before it. One graph
+ * launch then replays the whole day.
  *
  * The graph holds the argument values and launch shapes (clmsoil_set_tuning &
  * co.) of the moment it was created; change them, and you must create a new
@@ -29,9 +30,12 @@ extern "C" {
 
 typedef struct clmsoil_day_plan clmsoil_day_plan;
 
-/* mlp_params / mlp_norm / dtsub0: all NULL for the plain controller, or all set
- * (see clmsoil_learned.h) to predict dtsub0 [nc] before every Richards step; the
- * plan then passes dtsub0 to the Richards step (mf->dtsub0 must be NULL). */
+/* mlp_params / mlp_norm / dtsub0 (see clmsoil_learned.h): all NULL for the plain
+ * controller; mlp_params and mlp_norm set with dtsub0 NULL for the network fused
+ * into every Richards step (clmsoil_soilwm_moisture_form_solve_learned); all
+ * three set to predict dtsub0 [nc] in a launch of its own before every Richards
+ * step, which then reads it (same results as the fused form). mf->dtsub0 must
+ * be NULL whenever the network is used. */
 int clmsoil_day_plan_create(const clmsoil_heat_args *heat, const clmsoil_mf_args *mf,
                             const clmsoil_levels *lev, double dtime,
                             const clmsoil_heat_config *heat_cfg,

@@ -47,6 +47,12 @@ int clmsoil_set_richards_schedule(int persistent);
 /* Blocks per CU of the resident Richards grid (0, the default: as many as the
  * occupancy calculator says fit; 1..32 override it). Results are identical. */
 int clmsoil_set_richards_residency(int blocks_per_cu);
+/* Resident Richards grid: a wavefront refills its idle lane groups once
+ * n/16 of them are idle (or none is busy), n in 1..16; separately for the plain
+ * step and for the step with the fused first-substep network
+ * (clmsoil_soilwm_moisture_form_solve_learned), whose refills also evaluate the
+ * network for the whole wavefront. Defaults 7 and 7. Results are identical. */
+int clmsoil_set_richards_refill(int plain_sixteenths, int learned_sixteenths);
 /* Workgroup size (64, 128 — the default — or 256) of the Zeng-Decker 2009 step
  * (include/clmsoil_zd09.h). Results are identical. */
 int clmsoil_set_zd_block(int block);

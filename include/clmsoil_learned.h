@@ -57,6 +57,23 @@ int clmsoil_mlp_predict_dtsub0(const clmsoil_mf_args *a, const clmsoil_levels *l
                                double dtime, const float *params, const float *norm,
                                double *dtsub0, void *stream);
 
+/* The moisture-form step of clmsoil_soilwm_moisture_form_solve
+ * (soil_water_movement.jl:1498-1517) with this network fused into the kernel:
+ * each column's first substep is predicted from its state as the column is
+ * loaded (the features above, the network on the matrix cores for the 16
+ * columns of a wavefront), instead of dtsub = dtime (:1259). The result is the
+ * same as clmsoil_mlp_predict_dtsub0 into a dtsub0 array followed by
+ * clmsoil_soilwm_moisture_form_solve with that array, without the extra launch
+ * and the nc doubles of memory. a->dtsub0 must be NULL (CLMSOIL_ERR_ARG);
+ * params/norm NULL gives CLMSOIL_ERR_NOT_TRAINED. Workspace, masks, tuning and
+ * error reporting as in clmsoil_soilwm_moisture_form_solve; the kernel always
+ * runs 4 lanes per column. */
+int clmsoil_soilwm_moisture_form_solve_learned(const clmsoil_mf_args *a,
+                                               const clmsoil_levels *lev, double dtime,
+                                               const clmsoil_swm_config *cfg,
+                                               const float *params, const float *norm,
+                                               void *stream);
+
 /* Raw network on standardised inputs x [24][ldx] -> y [n]. */
 int clmsoil_mlp_forward(const float *params, const float *x, int64_t n, int64_t ldx, float *y,
                         void *stream);

@@ -24,15 +24,15 @@ def _views(dev):
     return st, mi
 
 
-@pytest.mark.parametrize("use_predictor", [False, True])
-def test_day_plan_matches_eager_steps(gpu, use_predictor):
+@pytest.mark.parametrize("mode", ["plain", "fused", "two_launch"])
+def test_day_plan_matches_eager_steps(gpu, mode):
     import torch
     nsteps = 3
     dev = tuning.production_state(20000, seed=9)
     snap = {k: dev[k].clone() for k in CHECK}
     cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
                                       lower_boundary_condition=api.BC_WATERTABLE)
-    pred = learned.DtPredictor.load() if use_predictor else None
+    pred = learned.DtPredictor.load() if mode != "plain" else None
     # eager reference
     for _ in range(nsteps):
         tuning.heat_step(dev)
@@ -48,7 +48,8 @@ def test_day_plan_matches_eager_steps(gpu, use_predictor):
         dev[k].copy_(snap[k])
     st, mi = _views(dev)
     plan = day.DayPlan({k: dev[k] for k in tuning.HEAT_KEYS} | {"nc": dev["nc"]}, st, mi, None,
-                       dev["nbedrock"], 12, 20, 25, 1800.0, cfg, nsteps, predictor=pred)
+                       dev["nbedrock"], 12, 20, 25, 1800.0, cfg, nsteps, predictor=pred,
+                       fused=mode != "two_launch")
     for replay in range(2):
         for k in CHECK:
             dev[k].copy_(snap[k])

@@ -124,3 +124,69 @@ def test_trained_model_cuts_rejected_substeps(gpu):
     assert l["rejected_rate"] < 0.5 * p["rejected_rate"]
     assert l["mean_iterations"] < p["mean_iterations"]
     assert p["max_balance_residual"] < 1e-9 and l["max_balance_residual"] < 1e-9
+
+
+@pytest.mark.parametrize("persistent", [True, False])
+def test_fused_inference_equals_predict_then_solve(gpu, persistent):
+    """The network fused into the Richards kernel's column load gives the same
+    bits as the prediction launch followed by the solve from its first
+    substeps, on the resident-grid and the one-column-per-group launches, with
+    masked columns and a partial last wavefront (1000 columns)."""
+    import torch
+    lib = clmsoil_lib()
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_ZERO_FLUX)
+    pred = learned.DtPredictor.load()
+    nc = 1000
+    d = _state(nc, 17)
+    d["nrejected"] = np.zeros(nc)
+    mask = np.random.default_rng(5).random(nc) > 0.1
+    keys = ("h2osoi_liq", "smp_l", "hk_l", "qcharge", "nsubsteps", "nrejected")
+    assert lib.clmsoil_set_richards_schedule(1 if persistent else 0) == 0
+    try:
+        ta, tb, tc = api.to_device(d), api.to_device(d), api.to_device(d)
+        m = _dev(mask)
+        st, mi = learned.mf_views(ta)
+        dt0 = pred.predict_dtsub0(st, mi, m, ta["nbedrock"], 12, 20, 25, 1800.0)
+        st, mi = learned.mf_views(ta, dt0)
+        api.soilwm_moisture_form_solve(st, mi, m, ta["nbedrock"], 12, 13, 20, 1800.0, cfg, 25)
+        st, mi = learned.mf_views(tb)
+        pred.solve(st, mi, m, tb["nbedrock"], 12, 13, 20, 1800.0, cfg, 25)
+        st, mi = learned.mf_views(tc)
+        api.soilwm_moisture_form_solve(st, mi, m, tc["nbedrock"], 12, 13, 20, 1800.0, cfg, 25)
+        torch.cuda.synchronize()
+    finally:
+        lib.clmsoil_set_richards_schedule(1)
+    assert api.device_errors() == 0
+    for k in keys:
+        assert np.array_equal(ta[k].cpu().numpy(), tb[k].cpu().numpy()), k
+    # masked columns are untouched, and the network changed the substep paths
+    assert np.array_equal(tb["h2osoi_liq"].cpu().numpy()[:, ~mask], d["h2osoi_liq"][:, ~mask])
+    nb, nc_plain = tb["nsubsteps"].cpu().numpy(), tc["nsubsteps"].cpu().numpy()
+    assert np.sum(nb[mask] != nc_plain[mask]) > nc // 10
+    assert nb[mask].sum() < nc_plain[mask].sum()
+
+
+def test_fused_inference_host_validation(gpu):
+    import ctypes as C
+    from clmsoil import bind
+    lib = clmsoil_lib()
+    d = _state(8, 1)
+    t = api.to_device(d)
+    t.update(nc=8, nlevsno=12, nlevsoi=20, nlevgrnd=25, mask=None, smpmin=None,
+             nrejected=None, dtsub0=None, workspace=None)
+    a, lev = bind.mf_args(t), bind.levels(t)
+    c = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                    lower_boundary_condition=api.BC_ZERO_FLUX).to_abi()
+    pred = learned.DtPredictor.load()
+    pp, pn = C.c_void_p(pred.params.data_ptr()), C.c_void_p(pred.norm.data_ptr())
+    f = lib.clmsoil_soilwm_moisture_form_solve_learned
+    assert f(C.byref(a), C.byref(lev), 1800.0, C.byref(c), None, pn, None) == 8   # not trained
+    t["dtsub0"] = t["qcharge"]
+    a2 = bind.mf_args(t)
+    assert f(C.byref(a2), C.byref(lev), 1800.0, C.byref(c), pp, pn, None) == 7   # dtsub0 given
+
+
+def clmsoil_lib():
+    import clmsoil
+    return clmsoil.load_library()
