@@ -71,6 +71,22 @@ __device__ __forceinline__ double log10_w(double x) {
     return clm_log10(x);
 }
 
+// A pointer field of a kernel's by-value argument block, loaded from the kernarg
+// segment where it is needed (one scalar load). A kernel with dozens of pointer
+// arguments otherwise holds all of them in SGPRs from its entry on; past ~100
+// SGPRs the compiler spills them to VGPR lanes and reads every use back with two
+// v_readlane. OFF is the field's byte offset in the block, which must be the
+// kernel's first argument.
+template <typename T, unsigned OFF>
+__device__ __forceinline__ T karg_ptr() {
+    static_assert(sizeof(T) == 8 && OFF % 4 == 0, "64-bit field at a dword offset");
+    T v;
+    asm volatile("s_load_dwordx2 %0, %1, %2\n\ts_waitcnt lgkmcnt(0)"
+                 : "=s"(v)
+                 : "s"(__builtin_amdgcn_kernarg_segment_ptr()), "i"(OFF));
+    return v;
+}
+
 // Column-major [c, j] access with 1-based level j (the reference's index).
 struct Mat {
     double *p;

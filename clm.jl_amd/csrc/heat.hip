@@ -36,6 +36,7 @@
 // kernel writes its outputs); with G = 4 the original band rows wait for the
 // pivoting fallback in LDS; diagnostics are written as soon as they are known.
 #include <atomic>
+#include <cstddef>
 
 #include <hip/hip_runtime.h>
 // The log and exp tables of clm_math.h are read from a per-block copy in LDS
@@ -95,6 +96,12 @@ constexpr int heat_lds_doubles_per_wave() {
 
 }  // namespace
 
+// Argument pointers are fetched from the kernarg segment in the phase that uses
+// them (karg_ptr), so that only that phase's pointers occupy SGPRs.
+#define HP(field)                                                                        \
+    karg_ptr<decltype(clmsoil_heat_args::field),                                        \
+             (unsigned)(offsetof(HeatParams, a) + offsetof(clmsoil_heat_args, field))>()
+
 template <int G>
 __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c, const int k,
                                             double *band_tile, double *saved_rows) {
@@ -106,28 +113,31 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     const int nlevsno = P.nlevsno, nlevgrnd = P.nlevgrnd;
     const int nlev = nlevsno + nlevgrnd;
     const int js1 = nlevsno + 1;                       // level index of soil layer 1
-    const int64_t snl = a.snl[c];
+    const int64_t snl = HP(snl)[c];
     if (snl < -nlevsno || snl > 0) {
         if (k == 0) atomicOr(P.err, (unsigned)DEV_ERR_SNL);
         return;
     }
     const int jtop = (int)snl + nlevsno + 1;           // top active level
-    const int nbedrock = (int)a.nbedrock[c];
+    const int nbedrock = (int)HP(nbedrock)[c];
     const double dtime = P.dtime;
     const double cnfac = CNFAC, omc = 1.0 - CNFAC;
 
     // ---- column scalars ----
-    const double frac_sno = a.frac_sno_eff[c];
-    const double frac_h2osfc = a.frac_h2osfc[c];
-    const double dhsdT = a.dhsdT[c];
-    const double hs_top_snow = a.hs_top_snow[c], hs_soil = a.hs_soil[c], hs_h2osfc = a.hs_h2osfc[c];
-    const double eflx_bot = a.eflx_bot ? a.eflx_bot[c] : 0.0;
-    double t_h2osfc = a.t_h2osfc[c];
-    double h2osfc = a.h2osfc[c];
-    double h2osno_nl = a.h2osno_no_layers[c];
-    double snow_depth = a.snow_depth[c];
-    double int_snow = a.int_snow ? a.int_snow[c] : 0.0;
-    double snomelt_accum = a.snomelt_accum ? a.snomelt_accum[c] : 0.0;
+    const double frac_sno = HP(frac_sno_eff)[c];
+    const double frac_h2osfc = HP(frac_h2osfc)[c];
+    const double dhsdT = HP(dhsdT)[c];
+    const double hs_top_snow = HP(hs_top_snow)[c], hs_soil = HP(hs_soil)[c],
+                 hs_h2osfc = HP(hs_h2osfc)[c];
+    const double *eflx_bot_p = HP(eflx_bot);
+    const double eflx_bot = eflx_bot_p ? eflx_bot_p[c] : 0.0;
+    double t_h2osfc = HP(t_h2osfc)[c];
+    double h2osfc = HP(h2osfc)[c];
+    double h2osno_nl = HP(h2osno_no_layers)[c];
+    double snow_depth = HP(snow_depth)[c];
+    const double *int_snow_p = HP(int_snow), *snomelt_accum_p = HP(snomelt_accum);
+    double int_snow = int_snow_p ? int_snow_p[c] : 0.0;
+    double snomelt_accum = snomelt_accum_p ? snomelt_accum_p[c] : 0.0;
 
     // saturated conductivity tkmg * TKWAT^(fl*watsat) * TKICE^((1-fl)*watsat): the
     // logarithms of the two fixed bases once per column
@@ -137,6 +147,12 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     // cv (:781-872), one level at a time so that only t, z, zi, dz, sabg, thk and
     // cv stay live afterwards ----
     double t[S], z[S], dz[S], zit[S], sabg[S], thk[S], cv[S];
+    {
+    const double *p_t = HP(t_soisno), *p_liq = HP(h2osoi_liq), *p_ice = HP(h2osoi_ice);
+    const double *p_z = HP(z), *p_dz = HP(dz), *p_zi = HP(zi);
+    const double *p_ws = HP(watsat), *p_tkmg = HP(tkmg), *p_tkdry = HP(tkdry), *p_csol = HP(csol);
+    const double *p_exc = HP(excess_ice), *p_sabg = HP(sabg_lyr_col);
+    double *p_thk = HP(thk), *p_bw = HP(bw);
 #pragma unroll
     for (int i = 0; i < S; ++i) {
         const int jj = i * G + k + 1;
@@ -149,22 +165,22 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         sabg[i] = 0.0;
         const int64_t o = c + (int64_t)(jj - 1) * ld;
         if (act) {
-            t[i] = a.t_soisno[o];
-            liq = a.h2osoi_liq[o];
-            ice = a.h2osoi_ice[o];
-            z[i] = __builtin_nontemporal_load(&a.z[o]);
-            dz[i] = __builtin_nontemporal_load(&a.dz[o]);
-            zit[i] = __builtin_nontemporal_load(&a.zi[o]);          // zi[c, jj] (top interface)
+            t[i] = p_t[o];
+            liq = p_liq[o];
+            ice = p_ice[o];
+            z[i] = __builtin_nontemporal_load(&p_z[o]);
+            dz[i] = __builtin_nontemporal_load(&p_dz[o]);
+            zit[i] = __builtin_nontemporal_load(&p_zi[o]);          // zi[c, jj] (top interface)
             if (j >= 1) {
                 const int64_t os = c + (int64_t)(j - 1) * ld;
-                wsat = __builtin_nontemporal_load(&a.watsat[os]);
-                tkmg = __builtin_nontemporal_load(&a.tkmg[os]);
-                tkdry = __builtin_nontemporal_load(&a.tkdry[os]);
-                csol = __builtin_nontemporal_load(&a.csol[os]);
-                exc = a.excess_ice[os];
+                wsat = __builtin_nontemporal_load(&p_ws[os]);
+                tkmg = __builtin_nontemporal_load(&p_tkmg[os]);
+                tkdry = __builtin_nontemporal_load(&p_tkdry[os]);
+                csol = __builtin_nontemporal_load(&p_csol[os]);
+                exc = p_exc[os];
             }
-            if (a.sabg_lyr_col && jj <= nlevsno + 1)
-                sabg[i] = a.sabg_lyr_col[c + (int64_t)(jj - 1) * ld];
+            if (p_sabg && jj <= nlevsno + 1)
+                sabg[i] = p_sabg[c + (int64_t)(jj - 1) * ld];
         }
         const double dzi = dz[i];
         double th = 0.0, bwv = 0.0;
@@ -198,8 +214,8 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         }
         thk[i] = th;
         if (act) {
-            if (a.thk) a.thk[o] = th;
-            if (a.bw && j <= 0) a.bw[o] = bwv;
+            if (p_thk) p_thk[o] = th;
+            if (p_bw && j <= 0) p_bw[o] = bwv;
         }
         double v = 1.0;
         if (j >= 1) {
@@ -214,6 +230,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         // keep the next level's loads below this point: the other wavefront on the
         // SIMD hides their latency, and registers stay free for two wavefronts
         __builtin_amdgcn_sched_barrier(0);
+    }
     }
     // ---- interface conductivity tk, fact, fn (:605-629, :1271-1310) ----
     double tk[S], fact[S], fn[S];
@@ -351,13 +368,13 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     // soil layer 1's value and store -cnfac*fn, to which the post-solve part adds
     // -(1-cnfac)*fn1 (the same two roundings as computing the sum at once).
     const double fn_s1 = level_value<G, S>(fn, js1, k);
-    if (a.eflx_fgr) {
+    if (double *p_fgr = HP(eflx_fgr)) {
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             const int jj = i * G + k + 1;
             const int j = jj - nlevsno;
             if (jj >= jtop && jj <= nlev && j >= 1)
-                a.eflx_fgr[c + (int64_t)(j - 1) * ld] = (j < nlevgrnd) ? -cnfac * fn[i] : 0.0;
+                p_fgr[c + (int64_t)(j - 1) * ld] = (j < nlevgrnd) ? -cnfac * fn[i] : 0.0;
         }
     }
     // Compiler-only barrier: the level inputs used again after the solve (water,
@@ -556,10 +573,11 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     double fn1[S];
     {
         double zr[S];
+        const double *p_z = HP(z);
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             const int jj = i * G + k + 1;
-            zr[i] = (jj >= jtop && jj <= nlev) ? a.z[c + (int64_t)(jj - 1) * ld] : 0.0;
+            zr[i] = (jj >= jtop && jj <= nlev) ? p_z[c + (int64_t)(jj - 1) * ld] : 0.0;
         }
 #pragma unroll
         for (int i = 0; i < S; ++i) {
@@ -574,14 +592,17 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     }
     // water and ice of every level, read again (unchanged in memory until the end)
     double liq[S], ice[S];
+    {
+    const double *p_liq = HP(h2osoi_liq), *p_ice = HP(h2osoi_ice);
 #pragma unroll
     for (int i = 0; i < S; ++i) {
         const int jj = i * G + k + 1;
         liq[i] = ice[i] = 0.0;
         if (jj >= jtop && jj <= nlev) {
-            liq[i] = a.h2osoi_liq[c + (int64_t)(jj - 1) * ld];
-            ice[i] = a.h2osoi_ice[c + (int64_t)(jj - 1) * ld];
+            liq[i] = p_liq[c + (int64_t)(jj - 1) * ld];
+            ice[i] = p_ice[c + (int64_t)(jj - 1) * ld];
         }
+    }
     }
 
     // ---- phase_change_h2osfc! (:1757-1826) ----
@@ -670,6 +691,12 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     double h2osno_nl_1 = h2osno_nl, snow_depth_1 = snow_depth, qsm_set = 0.0;
     bool unresolved = false;
     bool tgt_written[S];
+    {
+    double *p_t = HP(t_soisno), *p_liq = HP(h2osoi_liq), *p_ice = HP(h2osoi_ice);
+    double *p_exc = HP(excess_ice), *p_qm = HP(qflx_snomelt_lyr), *p_qf = HP(qflx_snofrz_lyr);
+    const double *p_ws = HP(watsat), *p_bsw = HP(bsw), *p_suc = HP(sucsat), *p_dz = HP(dz);
+    double *p_fact = HP(fact), *p_fgr = HP(eflx_fgr), *p_exs = HP(exice_subs);
+    int64_t *p_imelt = HP(imelt);
 #pragma unroll
     for (int i = 0; i < S; ++i) {
         const int jj = i * G + k + 1;
@@ -681,10 +708,10 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         tgt_written[i] = false;
         if (!act) {
             // phase_change_h2osfc! writes t_soisno[c, joff] also without snow layers
-            if (jj == nlevsno && wrote_t0) a.t_soisno[o] = tnew[i];
+            if (jj == nlevsno && wrote_t0) p_t[o] = tnew[i];
             if (j <= 0 && jj >= 1) {
-                a.qflx_snomelt_lyr[o] = 0.0;
-                a.qflx_snofrz_lyr[o] = 0.0;
+                p_qm[o] = 0.0;
+                p_qf[o] = 0.0;
             }
             continue;
         }
@@ -692,11 +719,11 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         const int64_t os = c + (int64_t)(j - 1) * ld;
         double exc_i = 0.0, wsat_i = 0.0, bsw_i = 0.0, sucsat_i = 0.0, dz_i = 1.0;
         if (j >= 1) {
-            exc_i = a.excess_ice[os];
-            wsat_i = a.watsat[os];
-            bsw_i = a.bsw[os];
-            sucsat_i = a.sucsat[os];
-            dz_i = a.dz[o];
+            exc_i = p_exc[os];
+            wsat_i = p_ws[os];
+            bsw_i = p_bsw[os];
+            sucsat_i = p_suc[os];
+            dz_i = p_dz[o];
         }
         const double wice0 = ice[i];
         const double wexice0 = j >= 1 ? exc_i : 0.0;
@@ -802,21 +829,22 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         }
         tnew[i] = tt;
         // write back this level (:1929-2289 outputs)
-        a.t_soisno[o] = tt;
-        a.h2osoi_liq[o] = liqv;
-        a.h2osoi_ice[o] = icev;
-        if (j >= 1 && excv != wexice0) a.excess_ice[os] = excv;
+        p_t[o] = tt;
+        p_liq[o] = liqv;
+        p_ice[o] = icev;
+        if (j >= 1 && excv != wexice0) p_exc[os] = excv;
         if (j <= 0) {
-            a.qflx_snomelt_lyr[o] = qmelt_l[i];
-            a.qflx_snofrz_lyr[o] = qfrz_l[i];
+            p_qm[o] = qmelt_l[i];
+            p_qf[o] = qfrz_l[i];
         }
-        if (a.fact) a.fact[o] = fact[i];
-        if (a.imelt) a.imelt[o] = imelt[i];
+        if (p_fact) p_fact[o] = fact[i];
+        if (p_imelt) p_imelt[o] = imelt[i];
         if (j >= 1) {
-            if (a.eflx_fgr && j < nlevgrnd) a.eflx_fgr[os] = a.eflx_fgr[os] - (1.0 - cnfac) * fn1[i];
-            if (a.exice_subs)
-                a.exice_subs[os] = tgt_written[i] ? jl_max(0.0, (wexice0 - excv) / DENICE) : 0.0;
+            if (p_fgr && j < nlevgrnd) p_fgr[os] = p_fgr[os] - (1.0 - cnfac) * fn1[i];
+            if (p_exs)
+                p_exs[os] = tgt_written[i] ? jl_max(0.0, (wexice0 - excv) / DENICE) : 0.0;
         }
+    }
     }
     // column accumulators, summed over levels in the reference's order
     double xmf = 0.0, qflx_snomelt = 0.0, qflx_snofrz = 0.0, qflx_snow_drain = 0.0;
@@ -870,24 +898,24 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
 
     // ---- write back the column scalars (levels were written above) ----
     if (k == 0) {
-        a.t_h2osfc[c] = t_h2osfc;
-        a.h2osfc[c] = h2osfc;
-        a.h2osno_no_layers[c] = h2osno_nl;
-        a.snow_depth[c] = snow_depth;
-        if (a.int_snow) a.int_snow[c] = int_snow;
-        a.t_grnd[c] = tg;
-        a.xmf[c] = xmf;
-        a.xmf_h2osfc[c] = xmf_h2osfc;
-        a.qflx_snomelt[c] = qflx_snomelt;
-        a.qflx_snofrz[c] = qflx_snofrz;
-        a.qflx_snow_drain[c] = qflx_snow_drain;
-        a.qflx_h2osfc_to_ice[c] = qflx_h2osfc_to_ice;
-        a.eflx_fgr12[c] = -cnfac * fn_s1 - (1.0 - cnfac) * fn1_s1;
-        if (a.c_h2osfc) a.c_h2osfc[c] = c_h2osfc;
-        if (a.eflx_snomelt) a.eflx_snomelt[c] = qflx_snomelt * hfus;
-        if (a.eflx_snomelt_r) a.eflx_snomelt_r[c] = qflx_snomelt * hfus;
-        if (a.eflx_h2osfc_to_snow) a.eflx_h2osfc_to_snow[c] = eflx_h2osfc_to_snow;
-        if (a.snomelt_accum) a.snomelt_accum[c] = snomelt_accum;
+        HP(t_h2osfc)[c] = t_h2osfc;
+        HP(h2osfc)[c] = h2osfc;
+        HP(h2osno_no_layers)[c] = h2osno_nl;
+        HP(snow_depth)[c] = snow_depth;
+        if (double *p = HP(int_snow)) p[c] = int_snow;
+        HP(t_grnd)[c] = tg;
+        HP(xmf)[c] = xmf;
+        HP(xmf_h2osfc)[c] = xmf_h2osfc;
+        HP(qflx_snomelt)[c] = qflx_snomelt;
+        HP(qflx_snofrz)[c] = qflx_snofrz;
+        HP(qflx_snow_drain)[c] = qflx_snow_drain;
+        HP(qflx_h2osfc_to_ice)[c] = qflx_h2osfc_to_ice;
+        HP(eflx_fgr12)[c] = -cnfac * fn_s1 - (1.0 - cnfac) * fn1_s1;
+        if (double *p = HP(c_h2osfc)) p[c] = c_h2osfc;
+        if (double *p = HP(eflx_snomelt)) p[c] = qflx_snomelt * hfus;
+        if (double *p = HP(eflx_snomelt_r)) p[c] = qflx_snomelt * hfus;
+        if (double *p = HP(eflx_h2osfc_to_snow)) p[c] = eflx_h2osfc_to_snow;
+        if (double *p = HP(snomelt_accum)) p[c] = snomelt_accum;
     }
 }
 
@@ -904,7 +932,9 @@ __global__ __launch_bounds__(256, CLMSOIL_HEAT_WAVES) void heat_kernel(HeatParam
     const int64_t c = gtid / G;
     const int k = (int)(threadIdx.x % G);
     if (c >= P.a.nc) return;
-    if (P.a.mask && !P.a.mask[c]) return;
+    if (const uint8_t *mask = HP(mask)) {
+        if (!mask[c]) return;
+    }
     double *area = heat_lds + (int64_t)(threadIdx.x / 64) * heat_lds_doubles_per_wave<G>();
     heat_column<G>(P, c, k, area, area + TILE_DOUBLES);
 }

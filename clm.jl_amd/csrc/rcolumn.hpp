@@ -19,11 +19,19 @@
 // state and length) to a tape [k][j][c] for the adjoint (k < kmax; kacc counts
 // them all).
 #pragma once
+#include <cstddef>
+
 #include "common.hpp"
 #include "mlp_wave.hpp"
 #include "richards.hpp"
 
 namespace clmsoil {
+
+// Argument pointers of the Richards kernels (RichardsParams, their first kernel
+// argument) are fetched from the kernarg segment where they are used, so that
+// they occupy no SGPRs across the substep loop.
+#define RP(field) \
+    karg_ptr<decltype(RichardsParams::field), (unsigned)offsetof(RichardsParams, field)>()
 
 template <int G>
 __device__ __forceinline__ double nxt(const double *x, int i, int k, int S) {
@@ -89,7 +97,7 @@ struct RColumn {
         ld_ = P.ld;
         const int64_t ld = P.ld;
         const int joff = P.nlevsno, joff_zi = P.nlevsno + 1;
-        n = (int)P.nbedrock[c];
+        n = (int)RP(nbedrock)[c];
         // water-table index (BC_WATERTABLE, :1346-1350): the first interface jj in
         // 1..nlevsoi with zwt <= zi[jj] gives jwt = jj - 1 (nlevsoi when none).
         // Each lane tests its own interfaces and the group takes the minimum, so
@@ -97,13 +105,14 @@ struct RColumn {
         jwt = P.nlevsoi;
         zwt = 0.0;
         if (P.lower_bc == CLMSOIL_BC_WATERTABLE || P.aquifer_pass) {
-            zwt = P.zwt[c];
+            zwt = RP(zwt)[c];
             int jf = P.nlevsoi;
+            const double *p_zi = RP(zi);
 #pragma unroll
             for (int i = 0; i < S; ++i) {
                 const int jj = i * G + k + 1;
                 if (jj <= P.nlevsoi) {
-                    const double zij = P.zi[c + (int64_t)(joff_zi + jj - 1) * ld];
+                    const double zij = p_zi[c + (int64_t)(joff_zi + jj - 1) * ld];
                     if (zwt <= zij && jj - 1 < jf) jf = jj - 1;
                 }
             }
@@ -114,6 +123,10 @@ struct RColumn {
             return false;
         }
         const clm_dd L10 = clm_log_ext_normal(10.0);   // impedance 10^(-e_ice * icefrac)
+        const double *p_liq = RP(h2osoi_liq), *p_dz = RP(dz), *p_ws = RP(watsat), *p_hks = RP(hksat);
+        const double *p_icf = RP(icefrac), *p_z = RP(z);
+        const double *p_bsw = LC ? RP(bsw) : nullptr, *p_suc = LC ? RP(sucsat) : nullptr,
+                     *p_root = LC ? RP(qflx_rootsoi) : nullptr;
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             const int j = i * G + k + 1;
@@ -124,20 +137,20 @@ struct RColumn {
             if (j <= P.nlevsoi) {
                 const int64_t o = soil(j);
                 const int64_t os = c + (int64_t)(joff + j - 1) * ld;
-                lq = P.h2osoi_liq[os];
-                dzv = __builtin_nontemporal_load(&P.dz[os]);
-                ws = __builtin_nontemporal_load(&P.watsat[o]);
-                hks = __builtin_nontemporal_load(&P.hksat[o]);
-                icf = __builtin_nontemporal_load(&P.icefrac[o]);
-                zj = __builtin_nontemporal_load(&P.z[os]);
+                lq = p_liq[os];
+                dzv = __builtin_nontemporal_load(&p_dz[os]);
+                ws = __builtin_nontemporal_load(&p_ws[o]);
+                hks = __builtin_nontemporal_load(&p_hks[o]);
+                icf = __builtin_nontemporal_load(&p_icf[o]);
+                zj = __builtin_nontemporal_load(&p_z[os]);
                 if (j < P.nlevsoi) {
-                    icf1 = __builtin_nontemporal_load(&P.icefrac[o + ld]);
-                    zj1 = __builtin_nontemporal_load(&P.z[os + ld]);
+                    icf1 = __builtin_nontemporal_load(&p_icf[o + ld]);
+                    zj1 = __builtin_nontemporal_load(&p_z[os + ld]);
                 }
                 if constexpr (LC) {
-                    bs = P.bsw[o];
-                    su = P.sucsat[o];
-                    rt = P.qflx_rootsoi[o];
+                    bs = p_bsw[o];
+                    su = p_suc[o];
+                    rt = p_root[o];
                 }
             }
             if (j <= n) {
@@ -165,14 +178,14 @@ struct RColumn {
             if constexpr (EXP) qin[i] = qout[i] = di1[i] = do1[i] = 0.0;
         }
         qn_a = qn_b = 0.0;
-        qflx_infl = P.qflx_infl[c];
+        qflx_infl = RP(qflx_infl)[c];
         nsubstep = 0;
         nrej = 0;
         iter = 0;
         const double dtime = P.dtime;
         dtsub = dtime;
-        if (P.dtsub0) {
-            const double d0 = P.dtsub0[c], lo = jl_min(P.dtmin, dtime);
+        if (const double *p_d0 = RP(dtsub0)) {
+            const double d0 = p_d0[c], lo = jl_min(P.dtmin, dtime);
             dtsub = (d0 != d0) ? dtime : jl_min(jl_max(d0, lo), dtime);
         }
         dtdone = 0.0;
@@ -205,8 +218,8 @@ struct RColumn {
         }
         float r5 = 0.0f;
         if (on) {
-            if (k == 0) r5 = (float)log(P.hksat[c]);
-            else if (k == 1) r5 = (float)P.bsw[c];
+            if (k == 0) r5 = (float)log(RP(hksat)[c]);
+            else if (k == 1) r5 = (float)RP(bsw)[c];
             else if (k == 2) r5 = (float)qflx_infl;
             else r5 = (float)P.dtime;
         }
@@ -236,6 +249,8 @@ struct RColumn {
         }
         // node suction and its derivative (:1300-1309)
         double smp[S], dsmpdw[S], bsw[S];
+        const double *p_bsw = LC ? nullptr : RP(bsw), *p_suc = LC ? nullptr : RP(sucsat);
+        double *p_smp = LC ? nullptr : RP(smp_l), *p_hk = LC ? nullptr : RP(hk_l);
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             const int j = i * G + k + 1;
@@ -245,8 +260,8 @@ struct RColumn {
                     b = cache(LC_BSW, i);
                     suc = cache(LC_SUC, i);
                 } else {
-                    b = P.bsw[soil(j)];
-                    suc = P.sucsat[soil(j)];
+                    b = p_bsw[soil(j)];
+                    suc = p_suc[soil(j)];
                 }
             }
             bsw[i] = b;
@@ -256,7 +271,7 @@ struct RColumn {
             smp[i] = smpv;
             dsmpdw[i] = dsmpds / watsat[i];
             if constexpr (LC) cache(LC_SMP, i) = smpv;
-            else if (j <= n && P.smp_l) P.smp_l[soil(j)] = smpv;
+            else if (j <= n && p_smp) p_smp[soil(j)] = smpv;
         }
         // interface conductivity (:1287-1299) and fluxes (:1334-1367)
         double qo[S], d1[S];
@@ -271,7 +286,7 @@ struct RColumn {
             const double hkv = kh(i) * pow_w(s1, 2.0 * b + 3.0);
             const double dhkdw = (2.0 * b + 3.0) * hkv / s1;
             if constexpr (LC) cache(LC_HK, i) = hkv;
-            else if (j <= n && P.hk_l) P.hk_l[soil(j)] = hkv;
+            else if (j <= n && p_hk) p_hk[soil(j)] = hkv;
             const double smpn = nxt<G>(smp, i, k, S);
             const double dsmpdwn = nxt<G>(dsmpdw, i, k, S);
             const double watsatn = nxt<G>(watsat, i, k, S);
@@ -307,6 +322,7 @@ struct RColumn {
             c2[i] = e2;
         }
         // inflow at the top of each layer (:1371-1378) and the rows' flux terms
+        const double *p_root = LC ? nullptr : RP(qflx_rootsoi);
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             const int j = i * G + k + 1;
@@ -319,7 +335,7 @@ struct RColumn {
             double root = 0.0;
             if (j <= n) {
                 if constexpr (LC) root = cache(LC_ROOT, i);
-                else root = P.qflx_rootsoi[soil(j)];
+                else root = p_root[soil(j)];
             }
             fnet[i] = qi - qo[i] - root;
             a0[i] = dqidw0;
@@ -408,6 +424,7 @@ struct RColumn {
         }
         // ---- error estimate (:1441-1457) ----
         double errorMax = 0.0;
+        const double *p_root = LC ? nullptr : RP(qflx_rootsoi);
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             const int j = i * G + k + 1;
@@ -418,7 +435,7 @@ struct RColumn {
                 double root = 0.0;
                 if (j <= n) {
                     if constexpr (LC) root = cache(LC_ROOT, i);
-                    else root = P.qflx_rootsoi[soil(j)];
+                    else root = p_root[soil(j)];
                 }
                 const double dwm = prv<G>(dwat, i, k);
                 const double dwn = nxt<G>(dwat, i, k, S);
@@ -480,14 +497,17 @@ struct RColumn {
         const int64_t ld = P.ld;
         const int joff = P.nlevsno;
         const double dtime = P.dtime;
+        {
+        double *p_liq = RP(h2osoi_liq), *p_smp = RP(smp_l), *p_hk = RP(hk_l);
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             const int j = i * G + k + 1;
-            if (j <= n) P.h2osoi_liq[c + (int64_t)(joff + j - 1) * ld] = liq[i];
+            if (j <= n) p_liq[c + (int64_t)(joff + j - 1) * ld] = liq[i];
             if constexpr (LC) {
-                if (j <= n && P.smp_l) P.smp_l[soil(j)] = cache(LC_SMP, i);
-                if (j <= n && P.hk_l) P.hk_l[soil(j)] = cache(LC_HK, i);
+                if (j <= n && p_smp) p_smp[soil(j)] = cache(LC_SMP, i);
+                if (j <= n && p_hk) p_hk[soil(j)] = cache(LC_HK, i);
             }
+        }
         }
         // liquid and thickness of layer jwt+1, read from its owner lane's registers
         double liq_w = 0.0, mdz_w = M_TO_MM;
@@ -507,27 +527,28 @@ struct RColumn {
                 const int64_t o = c + (int64_t)(jw1 - 1) * ld;
                 double vwc = 0.0;
                 if (jw1 <= n) vwc = jl_max(liq_w, 1.0e-6) / mdz_w;   // dz * denh2o
-                const double wh_zwt = -P.sucsat[o] - zwt * M_TO_MM;
-                double s1 = jl_max(vwc / P.watsat[o], 0.01);
+                const double wh_zwt = -RP(sucsat)[o] - zwt * M_TO_MM;
+                double s1 = jl_max(vwc / RP(watsat)[o], 0.01);
                 s1 = jl_min(1.0, s1);
-                const double ka = 0.0 * P.hksat[o] * clm_pow(s1, 2.0 * P.bsw[o] + 3.0);
-                const double smpmin = P.smpmin ? P.smpmin[c] : -1.0e8;
+                const double ka = 0.0 * RP(hksat)[o] * clm_pow(s1, 2.0 * RP(bsw)[o] + 3.0);
+                const double *p_smpmin = RP(smpmin);
+                const double smpmin = p_smpmin ? p_smpmin[c] : -1.0e8;
                 const double smp1 = jl_max(smpmin, 0.0);
                 const int jm = jwt > 1 ? jwt : 1;
-                const double wh = smp1 - P.z[c + (int64_t)(joff + jm - 1) * ld] * M_TO_MM;
+                const double wh = smp1 - RP(z)[c + (int64_t)(joff + jm - 1) * ld] * M_TO_MM;
                 double q;
                 if (jwt == 0)
                     q = -ka * (wh_zwt - wh) / ((zwt + 1.0e-3) * M_TO_MM);
                 else
                     q = -ka * (wh_zwt - wh) /
-                        ((zwt - P.z[c + (int64_t)(joff + jwt - 1) * ld]) * M_TO_MM * 2.0);
+                        ((zwt - RP(z)[c + (int64_t)(joff + jwt - 1) * ld]) * M_TO_MM * 2.0);
                 q = jl_max(-10.0 / dtime, q);
                 q = jl_min(10.0 / dtime, q);
                 qcharge = q;
             }
-            P.qcharge[c] = qcharge;
-            P.nsubsteps[c] = (double)nsubstep;
-            if (P.nrejected) P.nrejected[c] = (double)nrej;
+            RP(qcharge)[c] = qcharge;
+            RP(nsubsteps)[c] = (double)nsubstep;
+            if (double *p_nr = RP(nrejected)) p_nr[c] = (double)nrej;
         }
     }
 };
