@@ -111,11 +111,13 @@ __device__ __forceinline__ double group_bcast(double v, int k);
 template <>
 __device__ __forceinline__ double group_bcast<1>(double v, int) { return v; }
 
+// Quad permutations read a valid lane for every lane, so the DPP move needs no
+// "old" value (mov_dpp: none is materialised, unlike update_dpp's).
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
     int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
 // quad_perm encoding: sel0 | sel1<<2 | sel2<<4 | sel3<<6
@@ -163,7 +165,7 @@ __device__ __forceinline__ double group_prev<4>(double v) {
 
 template <int CTRL>
 __device__ __forceinline__ int dpp_i32(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
 }
 // Min of an int over the G lanes of a group.
 template <int G>

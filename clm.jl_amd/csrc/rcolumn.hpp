@@ -33,18 +33,21 @@ namespace clmsoil {
 #define RP(field) \
     karg_ptr<decltype(RichardsParams::field), (unsigned)offsetof(RichardsParams, field)>()
 
+// Value of layer j+1 (j-1) for the layer in slot i of lane k. In the group
+// rotation only the last (first) lane reads across the slot boundary, from the
+// first (last) lane, so that lane sends its next (previous) slot instead: one
+// select and one DPP move per call.
 template <int G>
 __device__ __forceinline__ double nxt(const double *x, int i, int k, int S) {
-    // value of layer j+1 for the layer in slot i of lane k
-    double a = group_next<G>(x[i]);
-    double b = (i + 1 < S) ? group_next<G>(x[i + 1]) : 0.0;
-    return (k == G - 1) ? b : a;
+    if constexpr (G == 1) return (i + 1 < S) ? x[i + 1] : 0.0;
+    const double up = (i + 1 < S) ? x[i + 1] : 0.0;
+    return group_next<G>(k == 0 ? up : x[i]);
 }
 template <int G>
 __device__ __forceinline__ double prv(const double *x, int i, int k) {
-    double a = group_prev<G>(x[i]);
-    double b = (i > 0) ? group_prev<G>(x[i - 1]) : 0.0;
-    return (k == 0) ? b : a;
+    if constexpr (G == 1) return (i > 0) ? x[i - 1] : 0.0;
+    const double dn = (i > 0) ? x[i - 1] : 0.0;
+    return group_prev<G>(k == G - 1 ? dn : x[i]);
 }
 
 template <int G, bool EXP = false, bool REC = false, int LCS = 0>
