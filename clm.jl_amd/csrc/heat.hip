@@ -102,23 +102,31 @@ constexpr int heat_lds_doubles_per_wave() {
     karg_ptr<decltype(clmsoil_heat_args::field),                                        \
              (unsigned)(offsetof(HeatParams, a) + offsetof(clmsoil_heat_args, field))>()
 
-template <int G>
-__device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c, const int k,
+// NSNO/NGRND > 0: the level counts are compile-time constants (the CLM5 grid),
+// 0: read from P.
+template <int G, int NSNO = 0, int NGRND = 0>
+__device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c, const int k_lane,
                                             double *band_tile, double *saved_rows) {
     constexpr int NL = MAX_NLEVSNO + MAX_NLEVGRND;   // 37 snow+soil levels
     constexpr int S = (NL + G - 1) / G;
     constexpr bool SAVE_ROWS = heat_saves_rows<G>();
     const clmsoil_heat_args &a = P.a;
     const int64_t ld = a.ld;
-    const int nlevsno = P.nlevsno, nlevgrnd = P.nlevgrnd;
+    const int nlevsno = NSNO > 0 ? NSNO : P.nlevsno, nlevgrnd = NGRND > 0 ? NGRND : P.nlevgrnd;
     const int nlev = nlevsno + nlevgrnd;
     const int js1 = nlevsno + 1;                       // level index of soil layer 1
     const int64_t snl = HP(snl)[c];
     if (snl < -nlevsno || snl > 0) {
-        if (k == 0) atomicOr(P.err, (unsigned)DEV_ERR_SNL);
+        if (k_lane == 0) atomicOr(P.err, (unsigned)DEV_ERR_SNL);
         return;
     }
-    const int jtop = (int)snl + nlevsno + 1;           // top active level
+    int jtop = (int)snl + nlevsno + 1;                 // top active level
+    int k = k_lane;                                    // lane of the group
+    // The per-level lane masks (jj >= jtop, the owner lane of a row, ...) are
+    // recomputed in each phase rather than kept from the first one: hiding k and
+    // jtop from the optimizer at the phase boundaries keeps dozens of 64-bit masks
+    // from living through the kernel (where they would be spilled to VGPR lanes).
+#define HEAT_PHASE() asm volatile("" : "+v"(k), "+v"(jtop))
     const int nbedrock = (int)HP(nbedrock)[c];
     const double dtime = P.dtime;
     const double cnfac = CNFAC, omc = 1.0 - CNFAC;
@@ -232,6 +240,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         __builtin_amdgcn_sched_barrier(0);
     }
     }
+    HEAT_PHASE();
     // ---- interface conductivity tk, fact, fn (:605-629, :1271-1310) ----
     double tk[S], fact[S], fn[S];
 #pragma unroll
@@ -286,6 +295,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     const double b3_h = 1.0 + omc * (dtime / c_h2osfc) * tk_h2osfc / dzm_h - (dtime / c_h2osfc) * dhsdT;
     const double b2_h = -omc * (dtime / c_h2osfc) * tk_h2osfc / dzm_h;
 
+    HEAT_PHASE();
     // ---- level rows: RHS and bands (:1365-1748) ----
     // sub = coupling to the level above (band 4, or band 5 snow-0 for soil 1),
     // sup = coupling to the next level below (band 2, or band 1 for snow 0),
@@ -394,6 +404,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         }
     }
 
+    HEAT_PHASE();
     // ---- solve ----
     double x[S];
     double x_h = 0.0;
@@ -561,6 +572,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         }
     }
 
+    HEAT_PHASE();
     // ---- unpack (:149-167) and fn1 (:170-206) ----
     double tnew[S];
 #pragma unroll
@@ -605,6 +617,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     }
     }
 
+    HEAT_PHASE();
     // ---- phase_change_h2osfc! (:1757-1826) ----
     double xmf_h2osfc = 0.0, qflx_h2osfc_to_ice = 0.0, eflx_h2osfc_to_snow = 0.0;
     bool wrote_t0 = false;
@@ -684,6 +697,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             if (i * G + k + 1 == nlevsno) { tnew[i] = t0; ice[i] = ice0; }
     }
 
+    HEAT_PHASE();
     // ---- phase_change_beta! (:1929-2289), Float64 (hard) branch ----
     const double tfrz = TFRZ, hfus = HFUS;
     int imelt[S];
@@ -881,6 +895,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         }
     }
 
+    HEAT_PHASE();
     // ---- ground temperature (:224-247) and soil heat flux (:250-269) ----
     const double t_top = level_value<G, S>(tnew, jtop, k);
     const double t_s1 = level_value<G, S>(tnew, js1, k);
@@ -920,7 +935,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
 }
 
 // Dynamic LDS: one area of heat_lds_doubles_per_wave<G>() doubles per wavefront.
-template <int G>
+template <int G, int NSNO, int NGRND>
 __global__ __launch_bounds__(256, CLMSOIL_HEAT_WAVES) void heat_kernel(HeatParams P) {
     extern __shared__ double heat_lds[];
     for (int t = threadIdx.x; t < 256 * 3; t += blockDim.x)
@@ -936,10 +951,10 @@ __global__ __launch_bounds__(256, CLMSOIL_HEAT_WAVES) void heat_kernel(HeatParam
         if (!mask[c]) return;
     }
     double *area = heat_lds + (int64_t)(threadIdx.x / 64) * heat_lds_doubles_per_wave<G>();
-    heat_column<G>(P, c, k, area, area + TILE_DOUBLES);
+    heat_column<G, NSNO, NGRND>(P, c, k, area, area + TILE_DOUBLES);
 }
 
-template <int G>
+template <int G, int NSNO, int NGRND>
 static hipError_t launch_g(const HeatParams &P, hipStream_t stream, int block) {
     const int64_t threads = P.a.nc * G;
     const int64_t grid = (threads + block - 1) / block;
@@ -947,23 +962,27 @@ static hipError_t launch_g(const HeatParams &P, hipStream_t stream, int block) {
     if (lds > 65536) {
         static std::atomic<int> raised{0};
         if (!raised.load(std::memory_order_relaxed)) {
-            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&heat_kernel<G>),
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&heat_kernel<G, NSNO, NGRND>),
                                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                                      (int)lds);
             if (e != hipSuccess) return e;
             raised.store(1, std::memory_order_relaxed);
         }
     }
-    hipLaunchKernelGGL(heat_kernel<G>, dim3((unsigned)grid), dim3(block), lds, stream, P);
+    hipLaunchKernelGGL((heat_kernel<G, NSNO, NGRND>), dim3((unsigned)grid), dim3(block), lds, stream, P);
     return hipGetLastError();
 }
 
 hipError_t launch_heat(const HeatParams &P, int G, int block, hipStream_t stream) {
     if (P.a.nc == 0) return hipSuccess;
+    // the CLM5 grid (12 snow + 25 ground levels) with its level counts compiled in
+    const bool clm5 = P.nlevsno == MAX_NLEVSNO && P.nlevgrnd == MAX_NLEVGRND;
     switch (G) {
-        case 1: return launch_g<1>(P, stream, block);
-        case 2: return launch_g<2>(P, stream, block);
-        default: return launch_g<4>(P, stream, block);
+        case 1: return launch_g<1, 0, 0>(P, stream, block);
+        case 2: return launch_g<2, 0, 0>(P, stream, block);
+        default:
+            return clm5 ? launch_g<4, MAX_NLEVSNO, MAX_NLEVGRND>(P, stream, block)
+                        : launch_g<4, 0, 0>(P, stream, block);
     }
 }
 
