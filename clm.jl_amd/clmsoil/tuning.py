@@ -4,7 +4,9 @@ Searched, on the workload itself (same columns, same state):
   * Richards: lanes per column G in {1, 2, 4} (64/G columns per wavefront),
     workgroup size in {64, 128, 256}, schedule (one-shot grid or resident grid
     with column refill) and, for the resident grid, blocks per CU (0 = occupancy
-    calculator, or 1..4);
+    calculator, or 1..4); then, for the best resident-grid shape, the refill
+    threshold (a wavefront refills its idle lane groups once n/16 of them are
+    idle: fewer refill passes against more idle lanes);
   * heat: G in {1, 2, 4} and workgroup size in {64, 128, 256}. The per-wavefront
     LDS tile of the pivoting fallback (38 x 7 band entries + right-hand side)
     comes with the workgroup size: 64 / 128 / 256 threads hold 1 / 2 / 4 tiles.
@@ -25,6 +27,8 @@ TABLE = Path(__file__).resolve().parent / "tuning_table.json"
 GS = (1, 2, 4)
 BLOCKS = (64, 128, 256)
 RESIDENCY = (0, 1, 2, 3, 4)
+REFILLS = (4, 6, 7, 8, 9, 10, 12)
+DEFAULT_REFILL = 8
 
 _RICHARDS_STATE = ("h2osoi_liq",)
 _HEAT_STATE = ("t_soisno", "h2osoi_liq", "h2osoi_ice", "excess_ice", "t_h2osfc", "h2osfc",
@@ -116,6 +120,21 @@ def autotune(nc: int = 1 << 20, seed: int = 2, reps: int = 3, warm_steps: int = 
                     if log:
                         log(results["richards"][-1])
     rbest = min(results["richards"], key=lambda r: r["ms"])
+    results["refill"] = []
+    if rbest["persistent"]:
+        api.set_tuning(rbest["g"], rbest["block"], g0[2], g0[3])
+        api.set_richards_schedule(True)
+        api.set_richards_residency(rbest["blocks_per_cu"])
+        for n in REFILLS:
+            api.set_richards_refill(n, 7)
+            ms = _time(lambda: richards_step(dev, cfg), reset(_RICHARDS_STATE), reps)
+            results["refill"].append({"refill": n, "ms": ms})
+            if log:
+                log(results["refill"][-1])
+        best_refill = min(results["refill"], key=lambda r: r["ms"])
+        rbest = dict(rbest, refill=best_refill["refill"], ms=best_refill["ms"])
+    else:
+        rbest = dict(rbest, refill=DEFAULT_REFILL)
     for g in GS:
         for blk in BLOCKS:
             api.set_tuning(rbest["g"], rbest["block"], g, blk)
@@ -126,7 +145,8 @@ def autotune(nc: int = 1 << 20, seed: int = 2, reps: int = 3, warm_steps: int = 
     hbest = min(results["heat"], key=lambda r: r["ms"])
     assert api.device_errors() == 0
     entry = {"device": device_key(), "workload": "production_day", "columns": nc,
-             "richards": {k: rbest[k] for k in ("g", "block", "persistent", "blocks_per_cu")},
+             "richards": {k: rbest[k] for k in ("g", "block", "persistent", "blocks_per_cu",
+                                                 "refill")},
              "heat": {k: hbest[k] for k in ("g", "block")},
              "richards_ms": rbest["ms"], "heat_ms": hbest["ms"], "candidates": results}
     apply(entry)
@@ -164,6 +184,7 @@ def apply(entry: dict | None) -> bool:
     api.set_tuning(r["g"], r["block"], h["g"], h["block"])
     api.set_richards_schedule(bool(r["persistent"]))
     api.set_richards_residency(r["blocks_per_cu"])
+    api.set_richards_refill(int(r.get("refill", DEFAULT_REFILL)), 7)
     return True
 
 

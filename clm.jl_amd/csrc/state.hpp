@@ -20,9 +20,10 @@ struct Tuning {
     std::atomic<int> richards_blocks_per_cu{0};  // persistent grid size; 0 = occupancy query
     // persistent grid: refill once this many 16ths of a wavefront's groups are
     // idle, plain and with the fused first-substep network (which evaluates the
-    // whole wavefront at every refill). Measured on the production day at G = 4:
-    // plain 7 fastest (6..8 within 0.5 %; 1: +6 %, 11: +7 %, 16: +53 %).
-    std::atomic<int> richards_refill{7};
+    // whole wavefront at every refill). Measured on the 1M-column production
+    // state at G = 4 (ms per step): 4: 2.89, 6: 2.81, 7: 2.77, 8: 2.74, 9: 2.74,
+    // 10: 2.76; learned on config 0: 7 fastest (4: +11 %, 10: +1 %, 16: +9 %).
+    std::atomic<int> richards_refill{8};
     std::atomic<int> richards_refill_learned{7};
     std::atomic<int> zd_block{128};              // Zeng-Decker 2009 step, one lane per column
     std::atomic<int> ghf_block{256};             // ground heat flux

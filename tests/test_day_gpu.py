@@ -67,6 +67,9 @@ def test_autotune_small(gpu, tmp_path):
     entry = tuning.autotune(nc=8192, reps=1, warm_steps=1)
     assert entry["richards"]["g"] in (1, 2, 4) and entry["heat"]["block"] in (64, 128, 256)
     assert len(entry["candidates"]["richards"]) == 3 * 3 * (1 + 5)
+    if entry["richards"]["persistent"]:
+        assert len(entry["candidates"]["refill"]) == len(tuning.REFILLS)
+    assert entry["richards"]["refill"] in tuning.REFILLS + (tuning.DEFAULT_REFILL,)
     p = tmp_path / "table.json"
     tuning.save_entry(entry, p)
     assert tuning.load_table(p)[0]["columns"] == 8192
@@ -74,3 +77,4 @@ def test_autotune_small(gpu, tmp_path):
     api.set_tuning(4, 256, 4, 256)
     api.set_richards_schedule(True)
     api.set_richards_residency(0)
+    api.set_richards_refill(tuning.DEFAULT_REFILL, 7)
