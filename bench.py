@@ -25,12 +25,26 @@ HBM_PEAK_GBS = 8000.0
 # device copy (fetch x2.0, write x1.0); recorded in
 # profiles/hbm_traffic_production_day_1M.md. FP64 FLOP per launch (fma = 2) from
 # the SQ_INSTS_VALU_*_F64 counters of the same kernels, recorded in
-# profiles/sq_counters_production_day_1M.md. Both belong to the kernel sources of
-# commit PROFILED_AT (the bench line names it); a later kernel change needs a new
-# profile.
-MEASURED_TRAFFIC_1M = {"heat": 8.494e9, "richards": 4.188e9}
-FP64_FLOP_1M = {"heat": 3.42e10, "richards": 4.19e10}
-PROFILED_AT = "866cf32"
+# profiles/sq_counters_production_day_1M.md. Both were measured on the kernel
+# sources of commit PROFILED_AT, whose hash is PROFILED_SOURCES: when the sources
+# differ, the bench line reports the traffic as null (the profile is stale).
+MEASURED_TRAFFIC_1M = {"heat": 6.317e9, "richards": 3.814e9}
+FP64_FLOP_1M = {"heat": 3.094e10, "richards": 4.153e10}
+PROFILED_AT = "2f7c472"
+PROFILED_SOURCES = "ae7adf4fb4dc48fb"
+KERNEL_SOURCES = ("common.hpp", "clm_math.h", "clm_math_tables.h", "mlp_wave.hpp", "rcolumn.hpp",
+                  "richards.hpp", "richards.hip", "heat.hpp", "heat.hip")
+
+
+def kernel_sources_hash() -> str:
+    """sha256 (first 16 hex digits) of the heat and Richards kernel sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        h.update((REPO / "clm.jl_amd" / "csrc" / f).read_bytes())
+    return h.hexdigest()[:16]
+
+
 FP64_PEAK_TFLOPS = 78.6
 METRIC = "column-steps/s @1M cols FP64 soil solves; HBM GB/s frac; learned vs plain dt"
 
@@ -871,6 +885,9 @@ def main(argv=None):
     dom = max(kern, key=lambda k: kern[k][0])
     ms, by = kern[dom]
     achieved = by / (ms / 1e3) / 1e9
+    # the counter figures belong to this workload and to the profiled kernel sources
+    fresh = kernel_sources_hash() == PROFILED_SOURCES
+    profiled = fresh and nc == 1 << 20 and args.seed == 2
     out = {
         "metric": METRIC, "value": value, "unit": "column-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": total_ms / args.steps,
@@ -886,15 +903,14 @@ def main(argv=None):
                               else "defaults")},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC_1M[dom] if (nc == 1 << 20 and args.seed == 2)
-                     else None,
-                     "traffic_profiled_at": PROFILED_AT},
+                     "traffic": MEASURED_TRAFFIC_1M[dom] if profiled else None,
+                     "traffic_profiled_at": PROFILED_AT, "profile_current": fresh},
         # the same kernel against the FP64 vector pipe (FLOP from its counters)
         "fp64_view": ({"kernel": dom, "achieved": FP64_FLOP_1M[dom] / (ms / 1e3) / 1e12,
                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                        "frac": FP64_FLOP_1M[dom] / (ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
                        "profiled_at": PROFILED_AT}
-                      if (nc == 1 << 20 and args.seed == 2) else None),
+                      if profiled else None),
         "kernels_ms": {"heat": heat_ms, "richards": rich_ms},
         "eager_column_steps_per_s": nc * args.steps / (eager_ms / 1e3),
         "fused_step_hbm_frac": (b_r + b_h) / (total_ms / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS,
