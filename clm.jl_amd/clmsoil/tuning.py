@@ -28,7 +28,7 @@ GS = (1, 2, 4)
 BLOCKS = (64, 128, 256)
 RESIDENCY = (0, 1, 2, 3, 4)
 REFILLS = (4, 6, 7, 8, 9, 10, 12)
-DEFAULT_REFILL = 8
+DEFAULT_REFILL = 7
 
 _RICHARDS_STATE = ("h2osoi_liq",)
 _HEAT_STATE = ("t_soisno", "h2osoi_liq", "h2osoi_ice", "excess_ice", "t_h2osfc", "h2osfc",

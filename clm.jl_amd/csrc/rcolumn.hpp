@@ -293,29 +293,34 @@ struct RColumn {
             const double smpn = nxt<G>(smp, i, k, S);
             const double dsmpdwn = nxt<G>(dsmpdw, i, k, S);
             const double watsatn = nxt<G>(watsat, i, k, S);
+            // The interior interface (j < n, :1334-1342) and the water-table bottom
+            // (j == n, :1346-1367) have the same expression shapes: the bottom uses
+            // dhkdw/watsat instead of 0.5*dhkdw/watsat and -smp instead of
+            // smpn - smp. Selecting those operands first lets both cases share one
+            // set of divisions, each case getting exactly its own expressions.
+            const bool last = j == n;
+            const double dhkds1 = (last ? dhkdw : 0.5 * dhkdw) / watsat[i];
+            const double num = last ? -smp[i] : smpn - smp[i];
+            const double dni = dn(i);
+            const double qv = -hkv * num / dni + hkv;
+            const double e1v = (hkv * dsmpdw[i] - dhkds1 * num) / dni + dhkds1;
             double q = 0.0, e1 = 0.0, e2 = 0.0;
             if (j < n) {
-                const double dhkds1 = 0.5 * dhkdw / watsat[i];
                 const double dhkds2 = 0.5 * dhkdw / watsatn;
-                const double num = smpn - smp[i];
-                const double dni = dn(i);
-                q = -hkv * num / dni + hkv;
-                e1 = (hkv * dsmpdw[i] - dhkds1 * num) / dni + dhkds1;
+                q = qv;
+                e1 = e1v;
                 e2 = (-hkv * dsmpdwn - dhkds2 * num) / dni + dhkds2;
-            } else if (j == n) {
+            } else if (last) {
                 if (lower_bc == CLMSOIL_BC_WATERTABLE) {
                     if (!(j > jwt)) {
-                        const double dhkds1l = dhkdw / watsat[i];
-                        const double num = -smp[i];
-                        const double dni = dn(i);
-                        q = -hkv * num / dni + hkv;
-                        e1 = (hkv * dsmpdw[i] - dhkds1l * num) / dni + dhkds1l;
+                        q = qv;
+                        e1 = e1v;
                     }
                     qn_a = q;          // qcharge = qout + dqodw1 * dwat (:1476-1478)
                     qn_b = e1;
                 } else if (lower_bc == CLMSOIL_BC_FLUX) {
                     q = hkv;
-                    e1 = dhkdw / watsat[i];
+                    e1 = dhkds1;       // dhkdw / watsat
                     qn_a = hkv;        // qcharge = hk + dhkdw * dwat (:1471-1472)
                     qn_b = dhkdw;
                 }

@@ -21,9 +21,10 @@ struct Tuning {
     // persistent grid: refill once this many 16ths of a wavefront's groups are
     // idle, plain and with the fused first-substep network (which evaluates the
     // whole wavefront at every refill). Measured on the 1M-column production
-    // state at G = 4 (ms per step): 4: 2.89, 6: 2.81, 7: 2.77, 8: 2.74, 9: 2.74,
-    // 10: 2.76; learned on config 0: 7 fastest (4: +11 %, 10: +1 %, 16: +9 %).
-    std::atomic<int> richards_refill{8};
+    // state at G = 4 (ms per step): 4: 2.95, 6: 2.87, 7: 2.81, 8: 2.81, 9: 2.81,
+    // 10: 2.83, 12: 2.94; learned on config 0: 7 fastest (4: +11 %, 10: +1 %,
+    // 16: +9 %).
+    std::atomic<int> richards_refill{7};
     std::atomic<int> richards_refill_learned{7};
     std::atomic<int> zd_block{128};              // Zeng-Decker 2009 step, one lane per column
     std::atomic<int> ghf_block{256};             // ground heat flux

@@ -51,7 +51,7 @@ int clmsoil_set_richards_residency(int blocks_per_cu);
  * n/16 of them are idle (or none is busy), n in 1..16; separately for the plain
  * step and for the step with the fused first-substep network
  * (clmsoil_soilwm_moisture_form_solve_learned), whose refills also evaluate the
- * network for the whole wavefront. Defaults 8 and 7. Results are identical. */
+ * network for the whole wavefront. Defaults 7 and 7. Results are identical. */
 int clmsoil_set_richards_refill(int plain_sixteenths, int learned_sixteenths);
 /* Workgroup size (64, 128 — the default — or 256) of the Zeng-Decker 2009 step
  * (include/clmsoil_zd09.h). Results are identical. */
