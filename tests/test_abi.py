@@ -125,3 +125,16 @@ def test_transport_entries_validate_on_the_host():
     assert lib.clmsoil_ch4_tran(C.byref(bl), 1800.0, C.byref(q), None) == _abi.ERR_SIZE
     be = bind.ch4_tran_args(m | {"nc": 0, "ld": 8})
     assert lib.clmsoil_ch4_tran(C.byref(be), 1800.0, C.byref(q), None) == ok
+
+
+def test_tuning_setters_validate_on_the_host():
+    """Launch-tuning setters accept only the shapes the kernels are built for
+    (G = 8 stays rejected) and refill thresholds of 1..16 sixteenths."""
+    lib = clmsoil.load_library()
+    assert lib.clmsoil_set_tuning(8, 256, 4, 256) == _abi.ERR_ARG
+    assert lib.clmsoil_set_tuning(4, 96, 4, 256) == _abi.ERR_ARG
+    assert lib.clmsoil_set_richards_refill(0, 7) == _abi.ERR_ARG
+    assert lib.clmsoil_set_richards_refill(7, 17) == _abi.ERR_ARG
+    assert lib.clmsoil_set_richards_refill(16, 1) == 0
+    assert lib.clmsoil_set_richards_refill(7, 7) == 0
+    assert lib.clmsoil_set_richards_residency(33) == _abi.ERR_ARG
