@@ -23,6 +23,8 @@ def declare(lib) -> None:
     lib.clmsoil_set_richards_residency.restype = I
     lib.clmsoil_set_richards_refill.argtypes = [I, I]
     lib.clmsoil_set_richards_refill.restype = I
+    lib.clmsoil_set_richards_lds_cache.argtypes = [I]
+    lib.clmsoil_set_richards_lds_cache.restype = I
     lib.clmsoil_set_zd_block.argtypes = [I]
     lib.clmsoil_set_zd_block.restype = I
     lib.clmsoil_math_eval.argtypes = [I, P, P, P, C.c_int64, P]

@@ -494,6 +494,13 @@ def set_richards_refill(plain_sixteenths: int, learned_sixteenths: int) -> None:
                "set_richards_refill")
 
 
+def set_richards_lds_cache(fields: int) -> None:
+    """Per-layer fields of a Richards column kept in the kernel's LDS cache (0, 5 or 7;
+    the cache exists at G = 4 with 256-thread blocks)."""
+    L = _lib()
+    _abi.check(L, L.clmsoil_set_richards_lds_cache(int(fields)), "set_richards_lds_cache")
+
+
 def set_zd_block(block: int) -> None:
     """Workgroup size of the Zeng-Decker 2009 step (64, 128 or 256)."""
     L = _lib()

@@ -6,7 +6,10 @@ Searched, on the workload itself (same columns, same state):
     with column refill) and, for the resident grid, blocks per CU (0 = occupancy
     calculator, or 1..4); then, for the best resident-grid shape, the refill
     threshold (a wavefront refills its idle lane groups once n/16 of them are
-    idle: fewer refill passes against more idle lanes);
+    idle: fewer refill passes against more idle lanes) and, at G = 4 with
+    256-thread blocks, the LDS tile of the column: how many per-layer fields live
+    in the lane-private LDS cache (0, 5 or 7: 0, 50 or 70 KB per block) rather
+    than in registers or memory;
   * heat: G in {1, 2, 4} and workgroup size in {64, 128, 256}. The per-wavefront
     LDS tile of the pivoting fallback (38 x 7 band entries + right-hand side)
     comes with the workgroup size: 64 / 128 / 256 threads hold 1 / 2 / 4 tiles.
@@ -29,6 +32,8 @@ BLOCKS = (64, 128, 256)
 RESIDENCY = (0, 1, 2, 3, 4)
 REFILLS = (4, 6, 7, 8, 9, 10, 12)
 DEFAULT_REFILL = 7
+LDS_FIELDS = (0, 5, 7)
+DEFAULT_LDS_FIELDS = 7
 
 _RICHARDS_STATE = ("h2osoi_liq",)
 _HEAT_STATE = ("t_soisno", "h2osoi_liq", "h2osoi_ice", "excess_ice", "t_h2osfc", "h2osfc",
@@ -135,6 +140,20 @@ def autotune(nc: int = 1 << 20, seed: int = 2, reps: int = 3, warm_steps: int = 
         rbest = dict(rbest, refill=best_refill["refill"], ms=best_refill["ms"])
     else:
         rbest = dict(rbest, refill=DEFAULT_REFILL)
+    results["lds_fields"] = []
+    if rbest["g"] == 4 and rbest["block"] == 256:
+        api.set_richards_refill(rbest["refill"], 7)
+        for f in LDS_FIELDS:
+            api.set_richards_lds_cache(f)
+            ms = _time(lambda: richards_step(dev, cfg), reset(_RICHARDS_STATE), reps)
+            results["lds_fields"].append({"lds_fields": f, "ms": ms})
+            if log:
+                log(results["lds_fields"][-1])
+        best_f = min(results["lds_fields"], key=lambda r: r["ms"])
+        rbest = dict(rbest, lds_fields=best_f["lds_fields"], ms=best_f["ms"])
+        api.set_richards_lds_cache(best_f["lds_fields"])
+    else:
+        rbest = dict(rbest, lds_fields=DEFAULT_LDS_FIELDS)
     for g in GS:
         for blk in BLOCKS:
             api.set_tuning(rbest["g"], rbest["block"], g, blk)
@@ -146,7 +165,7 @@ def autotune(nc: int = 1 << 20, seed: int = 2, reps: int = 3, warm_steps: int = 
     assert api.device_errors() == 0
     entry = {"device": device_key(), "workload": "production_day", "columns": nc,
              "richards": {k: rbest[k] for k in ("g", "block", "persistent", "blocks_per_cu",
-                                                 "refill")},
+                                                 "refill", "lds_fields")},
              "heat": {k: hbest[k] for k in ("g", "block")},
              "richards_ms": rbest["ms"], "heat_ms": hbest["ms"], "candidates": results}
     apply(entry)
@@ -185,6 +204,7 @@ def apply(entry: dict | None) -> bool:
     api.set_richards_schedule(bool(r["persistent"]))
     api.set_richards_residency(r["blocks_per_cu"])
     api.set_richards_refill(int(r.get("refill", DEFAULT_REFILL)), 7)
+    api.set_richards_lds_cache(int(r.get("lds_fields", DEFAULT_LDS_FIELDS)))
     return True
 
 

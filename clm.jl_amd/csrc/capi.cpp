@@ -205,6 +205,7 @@ int run_richards(const RichardsParams &P, hipStream_t stream) {
     return launch_richards(P, g_tuning.richards_g.load(std::memory_order_relaxed),
                            g_tuning.richards_block.load(std::memory_order_relaxed), persistent,
                            g_tuning.richards_blocks_per_cu.load(std::memory_order_relaxed),
+                           g_tuning.richards_lds_fields.load(std::memory_order_relaxed),
                            stream) == hipSuccess
                ? CLMSOIL_OK
                : CLMSOIL_ERR_HIP;
@@ -333,6 +334,12 @@ int clmsoil_set_richards_refill(int plain_sixteenths, int learned_sixteenths) {
         return CLMSOIL_ERR_ARG;
     g_tuning.richards_refill.store(plain_sixteenths);
     g_tuning.richards_refill_learned.store(learned_sixteenths);
+    return CLMSOIL_OK;
+}
+
+int clmsoil_set_richards_lds_cache(int fields) {
+    if (fields != 0 && fields != 5 && fields != 7) return CLMSOIL_ERR_ARG;
+    g_tuning.richards_lds_fields.store(fields);
     return CLMSOIL_OK;
 }
 

@@ -50,7 +50,7 @@ __device__ __forceinline__ double prv(const double *x, int i, int k) {
     return group_prev<G>(k == G - 1 ? dn : x[i]);
 }
 
-template <int G, bool EXP = false, bool REC = false, int LCS = 0>
+template <int G, bool EXP = false, bool REC = false, int LCS = 0, int LCF = 7>
 struct RColumn {
     static constexpr int NMAX = MAX_NLEVSOI;
     static constexpr int S = (NMAX + G - 1) / G;
@@ -77,17 +77,20 @@ struct RColumn {
     // lc[(f*S + i) * LCS], so a wavefront's access is one contiguous, conflict-free
     // line and every offset is an immediate.
     static constexpr bool LC = LCS > 0;
+    // LCF: fields in the cache, 7 (all below) or 5 (khat and den stay in registers)
+    static_assert(LCF == 5 || LCF == 7, "cached field sets: 5 or 7");
+    static constexpr bool LC_KD = LC && LCF >= 7;
     double *lc = nullptr;
     enum { LC_BSW = 0, LC_SUC = 1, LC_ROOT = 2, LC_SMP = 3, LC_HK = 4, LC_KHAT = 5, LC_DEN = 6,
-           LC_FIELDS = 7 };
+           LC_FIELDS = LCF };
     __device__ __forceinline__ double &cache(int f, int i) const { return lc[(f * S + i) * LCS]; }
     // khat and den are read once per property update: in LDS when cached
     __device__ __forceinline__ double &kh(int i) {
-        if constexpr (LC) return cache(LC_KHAT, i);
+        if constexpr (LC_KD) return cache(LC_KHAT, i);
         else return khat[i];
     }
     __device__ __forceinline__ double &dn(int i) {
-        if constexpr (LC) return cache(LC_DEN, i);
+        if constexpr (LC_KD) return cache(LC_DEN, i);
         else return den[i];
     }
 

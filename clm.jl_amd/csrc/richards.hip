@@ -61,10 +61,10 @@ template <int G>
 constexpr int richards_cache_stride(int block) {
     return (G == 4 && block == 256) ? 256 : 0;
 }
-template <int G, bool EXP, int LCS>
+template <int G, bool EXP, int LCS, int LCF = 7>
 static size_t richards_lds_bytes() {
-    return (size_t)RColumn<G, EXP, false, LCS>::LC_FIELDS * RColumn<G, EXP, false, LCS>::S * LCS *
-           sizeof(double);
+    return (size_t)RColumn<G, EXP, false, LCS, LCF>::LC_FIELDS *
+           RColumn<G, EXP, false, LCS, LCF>::S * LCS * sizeof(double);
 }
 
 // One column per group for the group's lifetime. LEARN: the first substep comes
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kerne
 
 // Persistent: a resident grid whose groups refill from per-XCD column ranges.
 // Every wavefront leaves the loop once all eight ranges are exhausted.
-template <int G, bool EXP, int LCS, bool LEARN>
+template <int G, bool EXP, int LCS, bool LEARN, int LCF = 7>
 __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persistent(RichardsParams P) {
     fill_math_tables();
     const int k = (int)(threadIdx.x % G);
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persi
     const int xcd = (int)(blockIdx.x % NUM_XCD);
     const int64_t chunk = (P.nc + NUM_XCD - 1) / NUM_XCD;
     unsigned *ctrs = P.work_counters;
-    RColumn<G, EXP, false, LCS> col;
+    RColumn<G, EXP, false, LCS, LCF> col;
     if constexpr (LCS > 0) {
         extern __shared__ double richards_lds[];
         col.lc = richards_lds + threadIdx.x;
@@ -191,15 +191,15 @@ static hipError_t launch_g(const RichardsParams &P, hipStream_t stream, int bloc
 
 // Resident blocks per CU of richards_mf_persistent at a block size, from the
 // occupancy calculator, cached per (device, kernel, block) after the first query.
-template <int G, bool EXP, int LCS, bool LEARN>
+template <int G, bool EXP, int LCS, bool LEARN, int LCF = 7>
 static int resident_blocks(int dev, int block) {
     static std::atomic<int> cache[16][3];
     const int bi = block == 64 ? 0 : (block == 128 ? 1 : 2);
     if (dev < 0 || dev >= 16) dev = 0;
     int nb = cache[dev][bi].load(std::memory_order_relaxed);
     if (nb > 0) return nb;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G, EXP, LCS, LEARN>,
-                                                     block, richards_lds_bytes<G, EXP, LCS>()) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G, EXP, LCS, LEARN, LCF>,
+                                                     block, richards_lds_bytes<G, EXP, LCS, LCF>()) !=
         hipSuccess)
         return -1;
     if (nb < 1) nb = 1;
@@ -207,21 +207,21 @@ static int resident_blocks(int dev, int block) {
     return nb;
 }
 
-template <int G, bool EXP, int LCS, bool LEARN = false>
+template <int G, bool EXP, int LCS, bool LEARN = false, int LCF = 7>
 static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int block,
                            int blocks_per_cu) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return hipErrorUnknown;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return hipErrorUnknown;
-    if (richards_lds_bytes<G, EXP, LCS>() > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)richards_mf_persistent<G, EXP, LCS, LEARN>,
+    if (richards_lds_bytes<G, EXP, LCS, LCF>() > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)richards_mf_persistent<G, EXP, LCS, LEARN, LCF>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)richards_lds_bytes<G, EXP, LCS>());
+                                           (int)richards_lds_bytes<G, EXP, LCS, LCF>());
         if (e != hipSuccess) return e;
     }
     int nb = blocks_per_cu;
-    if (nb <= 0) nb = resident_blocks<G, EXP, LCS, LEARN>(dev, block);
+    if (nb <= 0) nb = resident_blocks<G, EXP, LCS, LEARN, LCF>(dev, block);
     if (nb <= 0) return hipErrorUnknown;
     // Any grid size is correct: groups only take work from the counters and never
     // wait on each other, so blocks that start late simply find less work.
@@ -229,22 +229,25 @@ static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int bloc
     grid = (grid + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     const int64_t need = (P.nc * G + block - 1) / block;
     if (grid > need) grid = (need + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
-    hipLaunchKernelGGL((richards_mf_persistent<G, EXP, LCS, LEARN>), dim3((unsigned)grid), dim3(block),
-                       (richards_lds_bytes<G, EXP, LCS>()), stream, P);
+    hipLaunchKernelGGL((richards_mf_persistent<G, EXP, LCS, LEARN, LCF>), dim3((unsigned)grid), dim3(block),
+                       (richards_lds_bytes<G, EXP, LCS, LCF>()), stream, P);
     return hipGetLastError();
 }
 
 template <bool EXP>
 static hipError_t launch_exp(const RichardsParams &P, int G, int block, bool persistent,
-                             int blocks_per_cu, hipStream_t stream) {
-    const bool cached = richards_cache_stride<4>(block) > 0;
+                             int blocks_per_cu, int lds_fields, hipStream_t stream) {
+    // the LDS column cache needs 256-thread blocks at G = 4; lds_fields 0 keeps
+    // every field in registers or memory
+    const bool cached = richards_cache_stride<4>(block) > 0 && lds_fields > 0;
     if (persistent) {
         switch (G) {
             case 1: return launch_p<1, EXP, 0>(P, stream, block, blocks_per_cu);
             case 2: return launch_p<2, EXP, 0>(P, stream, block, blocks_per_cu);
             default:
-                return cached ? launch_p<4, EXP, 256>(P, stream, block, blocks_per_cu)
-                              : launch_p<4, EXP, 0>(P, stream, block, blocks_per_cu);
+                if (!cached) return launch_p<4, EXP, 0>(P, stream, block, blocks_per_cu);
+                return lds_fields == 5 ? launch_p<4, EXP, 256, false, 5>(P, stream, block, blocks_per_cu)
+                                       : launch_p<4, EXP, 256>(P, stream, block, blocks_per_cu);
         }
     }
     switch (G) {
@@ -257,10 +260,10 @@ static hipError_t launch_exp(const RichardsParams &P, int G, int block, bool per
 }
 
 hipError_t launch_richards(const RichardsParams &P, int G, int block, bool persistent,
-                           int blocks_per_cu, hipStream_t stream) {
+                           int blocks_per_cu, int lds_fields, hipStream_t stream) {
     if (P.nc == 0) return hipSuccess;
-    return P.expensive ? launch_exp<true>(P, G, block, persistent, blocks_per_cu, stream)
-                       : launch_exp<false>(P, G, block, persistent, blocks_per_cu, stream);
+    return P.expensive ? launch_exp<true>(P, G, block, persistent, blocks_per_cu, lds_fields, stream)
+                       : launch_exp<false>(P, G, block, persistent, blocks_per_cu, lds_fields, stream);
 }
 
 hipError_t launch_richards_learned(const RichardsParams &P, bool persistent, int blocks_per_cu,

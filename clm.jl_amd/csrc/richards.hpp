@@ -37,8 +37,11 @@ struct RichardsParams {
 // selects the resident-grid launch whose groups refill from work counters.
 // blocks_per_cu sets the persistent grid (0 = as many blocks as the occupancy
 // calculator says fit on a CU).
+// lds_fields: per-layer fields the G = 4 / 256-thread kernels keep in a
+// lane-private LDS cache of the column (7: bsw, sucsat, root sink, smp, hk, khat,
+// node distance; 5: the first five; 0: none, all in registers or memory).
 hipError_t launch_richards(const RichardsParams &P, int G, int block, bool persistent,
-                           int blocks_per_cu, hipStream_t stream);
+                           int blocks_per_cu, int lds_fields, hipStream_t stream);
 // The same step with the first-substep network fused into the column load
 // (P.mlp and P.mlp_norm set): G = 4 lanes per column, 256-thread blocks.
 hipError_t launch_richards_learned(const RichardsParams &P, bool persistent, int blocks_per_cu,

@@ -26,6 +26,9 @@ struct Tuning {
     // 16: +9 %).
     std::atomic<int> richards_refill{7};
     std::atomic<int> richards_refill_learned{7};
+    // per-layer fields of the Richards column in the LDS cache (0, 5 or 7; the
+    // cache exists at G = 4 with 256-thread blocks)
+    std::atomic<int> richards_lds_fields{7};
     std::atomic<int> zd_block{128};              // Zeng-Decker 2009 step, one lane per column
     std::atomic<int> ghf_block{256};             // ground heat flux
 };

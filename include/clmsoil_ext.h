@@ -53,6 +53,12 @@ int clmsoil_set_richards_residency(int blocks_per_cu);
  * (clmsoil_soilwm_moisture_form_solve_learned), whose refills also evaluate the
  * network for the whole wavefront. Defaults 7 and 7. Results are identical. */
 int clmsoil_set_richards_refill(int plain_sixteenths, int learned_sixteenths);
+/* Per-layer fields of a Richards column the G = 4, 256-thread kernels keep in a
+ * lane-private LDS cache instead of registers or memory: 7 (default: bsw, sucsat,
+ * root sink, smp, hk, the impedance-scaled conductivity and the node distance;
+ * 70 KB per block), 5 (the first five; 50 KB) or 0 (no cache). The step with the
+ * fused first-substep network always uses 7. Results are identical. */
+int clmsoil_set_richards_lds_cache(int fields);
 /* Workgroup size (64, 128 — the default — or 256) of the Zeng-Decker 2009 step
  * (include/clmsoil_zd09.h). Results are identical. */
 int clmsoil_set_zd_block(int block);

@@ -138,3 +138,6 @@ def test_tuning_setters_validate_on_the_host():
     assert lib.clmsoil_set_richards_refill(16, 1) == 0
     assert lib.clmsoil_set_richards_refill(7, 7) == 0
     assert lib.clmsoil_set_richards_residency(33) == _abi.ERR_ARG
+    assert lib.clmsoil_set_richards_lds_cache(6) == _abi.ERR_ARG
+    for f in (0, 5, 7):
+        assert lib.clmsoil_set_richards_lds_cache(f) == 0

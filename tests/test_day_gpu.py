@@ -70,6 +70,7 @@ def test_autotune_small(gpu, tmp_path):
     if entry["richards"]["persistent"]:
         assert len(entry["candidates"]["refill"]) == len(tuning.REFILLS)
     assert entry["richards"]["refill"] in tuning.REFILLS + (tuning.DEFAULT_REFILL,)
+    assert entry["richards"]["lds_fields"] in tuning.LDS_FIELDS
     p = tmp_path / "table.json"
     tuning.save_entry(entry, p)
     assert tuning.load_table(p)[0]["columns"] == 8192
@@ -78,3 +79,4 @@ def test_autotune_small(gpu, tmp_path):
     api.set_richards_schedule(True)
     api.set_richards_residency(0)
     api.set_richards_refill(tuning.DEFAULT_REFILL, 7)
+    api.set_richards_lds_cache(tuning.DEFAULT_LDS_FIELDS)

@@ -251,3 +251,32 @@ def test_two_streams_disjoint_column_blocks(gpu):
 def clmsoil_lib():
     import clmsoil
     return clmsoil.load_library()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fields", [0, 5])
+def test_lds_cache_variants_give_the_same_bits(gpu, fields):
+    """The Richards kernels with 0 or 5 per-layer fields in the LDS column cache
+    against the default 7 (checked against the oracle by the tests above): same
+    bits on ragged production-day columns, resident grid and one-shot grid."""
+    import torch
+    from clmsoil import tuning
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_WATERTABLE)
+    keys = ("h2osoi_liq", "smp_l", "hk_l", "qcharge", "nsubsteps")
+    try:
+        for persistent in (True, False):
+            api.set_richards_schedule(persistent)
+            out = {}
+            for f in (7, fields):
+                api.set_richards_lds_cache(f)
+                dev = tuning.production_state(20000, seed=13)
+                tuning.richards_step(dev, cfg)
+                torch.cuda.synchronize()
+                out[f] = {k: dev[k].cpu().numpy() for k in keys}
+            for k in keys:
+                assert np.array_equal(out[7][k], out[fields][k], equal_nan=True), (persistent, k)
+    finally:
+        api.set_richards_lds_cache(7)
+        api.set_richards_schedule(True)
+    assert api.device_errors() == 0
