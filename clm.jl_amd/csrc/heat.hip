@@ -161,12 +161,17 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     const double *p_ws = HP(watsat), *p_tkmg = HP(tkmg), *p_tkdry = HP(tkdry), *p_csol = HP(csol);
     const double *p_exc = HP(excess_ice), *p_sabg = HP(sabg_lyr_col);
     double *p_thk = HP(thk), *p_bw = HP(bw);
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
+    // The loads of slot i+1 are issued before the arithmetic of slot i (one slot
+    // ahead), so their latency overlaps that arithmetic; the slot's geometry goes
+    // straight into its arrays, the rest waits in these registers.
+    struct LevelIn {
+        double liq, ice, wsat, tkmg, tkdry, csol, exc;
+    };
+    auto load_slot = [&](int i) {
         const int jj = i * G + k + 1;
         const int j = jj - nlevsno;
         const bool act = jj >= jtop && jj <= nlev;
-        double liq = 0.0, ice = 0.0, wsat = 0.0, tkmg = 0.0, tkdry = 0.0, csol = 0.0, exc = 0.0;
+        LevelIn v{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         t[i] = 0.0;
         z[i] = zit[i] = 0.0;
         dz[i] = 1.0;
@@ -174,22 +179,37 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         const int64_t o = c + (int64_t)(jj - 1) * ld;
         if (act) {
             t[i] = p_t[o];
-            liq = p_liq[o];
-            ice = p_ice[o];
+            v.liq = p_liq[o];
+            v.ice = p_ice[o];
             z[i] = __builtin_nontemporal_load(&p_z[o]);
             dz[i] = __builtin_nontemporal_load(&p_dz[o]);
             zit[i] = __builtin_nontemporal_load(&p_zi[o]);          // zi[c, jj] (top interface)
             if (j >= 1) {
                 const int64_t os = c + (int64_t)(j - 1) * ld;
-                wsat = __builtin_nontemporal_load(&p_ws[os]);
-                tkmg = __builtin_nontemporal_load(&p_tkmg[os]);
-                tkdry = __builtin_nontemporal_load(&p_tkdry[os]);
-                csol = __builtin_nontemporal_load(&p_csol[os]);
-                exc = p_exc[os];
+                v.wsat = __builtin_nontemporal_load(&p_ws[os]);
+                v.tkmg = __builtin_nontemporal_load(&p_tkmg[os]);
+                v.tkdry = __builtin_nontemporal_load(&p_tkdry[os]);
+                v.csol = __builtin_nontemporal_load(&p_csol[os]);
+                v.exc = p_exc[os];
             }
             if (p_sabg && jj <= nlevsno + 1)
                 sabg[i] = p_sabg[c + (int64_t)(jj - 1) * ld];
         }
+        return v;
+    };
+    LevelIn nxt_in = load_slot(0);
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const int jj = i * G + k + 1;
+        const int j = jj - nlevsno;
+        const bool act = jj >= jtop && jj <= nlev;
+        const int64_t o = c + (int64_t)(jj - 1) * ld;
+        const LevelIn cur = nxt_in;
+        if (i + 1 < S) nxt_in = load_slot(i + 1);
+        // the next slot's loads stay above this point, this slot's arithmetic below
+        __builtin_amdgcn_sched_barrier(0);
+        const double liq = cur.liq, ice = cur.ice, wsat = cur.wsat, tkmg = cur.tkmg,
+                     tkdry = cur.tkdry, csol = cur.csol, exc = cur.exc;
         const double dzi = dz[i];
         double th = 0.0, bwv = 0.0;
         if (act && j >= 1) {
@@ -235,8 +255,7 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             else v = THIN_SFCLAYER;
         }
         cv[i] = v;
-        // keep the next level's loads below this point: the other wavefront on the
-        // SIMD hides their latency, and registers stay free for two wavefronts
+        // the loads of the slot after next stay below this point
         __builtin_amdgcn_sched_barrier(0);
     }
     }
