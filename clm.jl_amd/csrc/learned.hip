@@ -119,9 +119,9 @@ __global__ __launch_bounds__(256) void features_kernel(FeatParams F, float *x, i
 }
 
 // Prediction for 16 columns per wavefront, 4 lanes per column (the Richards
-// kernel's layout): features of column c in lanes 4c'..4c'+3 as the fused
-// inference in RColumn::learned_first_substep forms them, then mlp_wave16, so
-// dtsub0 here equals the fused first substep bit for bit (before its clamp).
+// kernel's layout): the features of mlp_column_features, then mlp_wave16, as the
+// Richards kernel's fused inference does, so dtsub0 here equals the fused first
+// substep bit for bit (before its clamp).
 __global__ __launch_bounds__(256) void predict_kernel(FeatParams F, const float *__restrict__ P,
                                                       double *dtsub0) {
     const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
@@ -131,28 +131,8 @@ __global__ __launch_bounds__(256) void predict_kernel(FeatParams F, const float 
     int n = live ? (int)a.nbedrock[c] : 0;
     n = n < 0 ? 0 : (n > F.nlevsoi ? F.nlevsoi : n);
     float fe[6];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        const int j = 4 * i + k + 1;
-        float raw = 0.0f;
-        if (live && j <= n) {
-            const int64_t os = c + (int64_t)(F.nlevsno + j - 1) * a.ld;
-            double v = jl_max(a.h2osoi_liq[os], 1.0e-6) / (M_TO_MM * a.dz[os]);
-            v = v / a.watsat[c + (int64_t)(j - 1) * a.ld];
-            v = jl_min(v, 1.0);
-            v = jl_max(0.01, v);
-            raw = (float)v;
-        }
-        fe[i] = mlp_standardise(F.norm, 4 * i + k, raw);
-    }
-    float r5 = 0.0f;
-    if (live) {
-        if (k == 0) r5 = (float)log(a.hksat[c]);
-        else if (k == 1) r5 = (float)a.bsw[c];
-        else if (k == 2) r5 = (float)a.qflx_infl[c];
-        else r5 = (float)F.dtime;
-    }
-    fe[5] = mlp_standardise(F.norm, 20 + k, r5);
+    mlp_column_features(a.h2osoi_liq, a.dz, a.watsat, a.hksat, a.bsw, a.qflx_infl, a.ld, F.nlevsno,
+                        c, n, k, live, F.dtime, F.norm, fe);
     const float y = mlp_wave16(P, fe[0], fe[1], fe[2], fe[3], fe[4], fe[5]);   // every lane of the wavefront
     if (live && k == 0) {
         const double lr = (double)F.norm[2 * NIN] + (double)F.norm[2 * NIN + 1] * (double)y;

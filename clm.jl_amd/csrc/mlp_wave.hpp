@@ -49,6 +49,41 @@ __device__ __forceinline__ float mlp_standardise(const float *__restrict__ norm,
     return (raw - norm[f]) * __builtin_amdgcn_rcpf(norm[mlp::NIN + f]);
 }
 
+// Standardised features of column c for lane k of its group of 4, read from the
+// column's state in memory: node saturations of soil layers 4i+k+1 (the
+// reference's s2, soil_water_movement.jl:1273-1286; 0 below the bedrock layer n
+// and for columns that are not `live`), and in slot 5 log ksat (k = 0) and b
+// (k = 1) of layer 1, the infiltration flux (k = 2) and dtime (k = 3).
+__device__ __forceinline__ void mlp_column_features(const double *liq, const double *dz,
+                                                    const double *watsat, const double *hksat,
+                                                    const double *bsw, const double *qinfl,
+                                                    int64_t ld, int nlevsno, int64_t c, int n,
+                                                    int k, bool live, double dtime,
+                                                    const float *__restrict__ norm, float fe[6]) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const int j = 4 * i + k + 1;
+        float raw = 0.0f;
+        if (live && j <= n) {
+            const int64_t os = c + (int64_t)(nlevsno + j - 1) * ld;
+            double v = jl_max(liq[os], 1.0e-6) / (1.0e3 * dz[os]);
+            v = v / watsat[c + (int64_t)(j - 1) * ld];
+            v = jl_min(v, 1.0);
+            v = jl_max(0.01, v);
+            raw = (float)v;
+        }
+        fe[i] = mlp_standardise(norm, 4 * i + k, raw);
+    }
+    float r5 = 0.0f;
+    if (live) {
+        if (k == 0) r5 = (float)log(hksat[c]);
+        else if (k == 1) r5 = (float)bsw[c];
+        else if (k == 2) r5 = (float)qinfl[c];
+        else r5 = (float)dtime;
+    }
+    fe[5] = mlp_standardise(norm, 20 + k, r5);
+}
+
 // Network output of column m of the wavefront, returned in every lane of group m
 // (lanes 4m .. 4m+3). Lane 4m + k passes fe[i] = standardised feature 4i + k of
 // column m (i = 0..5). Every lane of the wavefront must call this (the matrix
