@@ -250,9 +250,11 @@ def learned_vs_plain(nc, steps):
     cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
                                       lower_boundary_condition=api.BC_ZERO_FLUX)
     pred = learned.DtPredictor.load()
-    r = learned.compare_controllers(pred, nc, steps, 2024, cfg, hindsight=True)
+    r = learned.compare_controllers(pred, nc, steps, 2024, cfg, hindsight=True, two_launch=True)
     r["workload"] = (f"{nc} config-0 columns x {steps} steps, Richards only (BC_ZERO_FLUX), "
-                     f"the network fused into the Richards kernel; 'hindsight' runs each "
+                     f"the network fused into the Richards kernel ('learned') or run as a "
+                     f"prediction launch before each step ('learned_two_launch', the same "
+                     f"first substeps, prediction included in the timing); 'hindsight' runs each "
                      f"column-step with the best of the 12 candidate first substeps, found by "
                      f"trying them all first (untimed): the ceiling of any first-substep "
                      f"predictor under the unchanged controller")

@@ -344,24 +344,29 @@ def best_candidate_dtsub0(dev, cfg: SoilWaterMovementConfig, work=None):
 
 
 def compare_controllers(pred: DtPredictor | None, nc: int, steps: int, seed: int,
-                        cfg: SoilWaterMovementConfig, device="cuda", hindsight=False) -> dict:
+                        cfg: SoilWaterMovementConfig, device="cuda", hindsight=False,
+                        two_launch=False) -> dict:
     """Advance the same config-0 columns through `steps` steps with the plain
     controller and with the learned first substep (when `pred` is given), and,
     with `hindsight`, with each column's best candidate first substep chosen by
     trying them all first (untimed): the ceiling any first-substep predictor can
     reach under the unchanged controller. The learned controller runs the
-    network fused into the Richards kernel. Returns, per controller:
+    network fused into the Richards kernel; with `two_launch`, also as a
+    prediction launch before each step (`learned_two_launch`, the same first
+    substeps). Returns, per controller:
     column-steps/s of the timed GPU work (prediction included), mean substep iterations,
     rejected-substep rate, and the largest water-balance residual
     |dW - (infl - qcharge - roots) dt| in kg/m2."""
     import torch
     out = {}
     modes = (["plain"] + (["learned"] if pred is not None else []) +
+             (["learned_two_launch"] if pred is not None and two_launch else []) +
              (["hindsight"] if hindsight else []))
     for mode in modes:
         dev, infl = config0_day(nc, seed, device)
         dtime = float(dev["dtime"])
-        dt0 = None
+        dt0 = (torch.empty(nc, dtype=torch.float64, device=device)
+               if mode == "learned_two_launch" else None)
         ws = richards_workspace(device)
         work = dict(dev)
         evs = []
@@ -378,6 +383,11 @@ def compare_controllers(pred: DtPredictor | None, nc: int, steps: int, seed: int
             e0.record()
             if mode == "learned":
                 run_step(dev, cfg, pred=pred, workspace=ws)   # inference fused in the kernel
+            elif mode == "learned_two_launch":
+                st, mi = mf_views(dev)
+                pred.predict_dtsub0(st, mi, None, dev["nbedrock"], dev["nlevsno"], dev["nlevsoi"],
+                                    dev["nlevgrnd"], dtime, out=dt0)
+                run_step(dev, cfg, dt0, workspace=ws)
             else:
                 run_step(dev, cfg, dt0, workspace=ws)
             e1.record()
@@ -395,6 +405,9 @@ def compare_controllers(pred: DtPredictor | None, nc: int, steps: int, seed: int
                      "max_balance_residual": float(bal)}
     if pred is not None:
         out["speedup"] = out["learned"]["column_steps_per_s"] / out["plain"]["column_steps_per_s"]
+    if pred is not None and two_launch:
+        out["speedup_two_launch"] = (out["learned_two_launch"]["column_steps_per_s"] /
+                                     out["plain"]["column_steps_per_s"])
     if hindsight:
         out["hindsight_speedup"] = (out["hindsight"]["column_steps_per_s"] /
                                     out["plain"]["column_steps_per_s"])

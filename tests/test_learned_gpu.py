@@ -119,11 +119,14 @@ def test_trained_model_cuts_rejected_substeps(gpu):
     cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
                                       lower_boundary_condition=api.BC_ZERO_FLUX)
     pred = learned.DtPredictor.load()
-    r = learned.compare_controllers(pred, 32768, 6, 99, cfg)
-    p, l = r["plain"], r["learned"]
+    r = learned.compare_controllers(pred, 32768, 6, 99, cfg, two_launch=True)
+    p, l, l2 = r["plain"], r["learned"], r["learned_two_launch"]
     assert l["rejected_rate"] < 0.5 * p["rejected_rate"]
     assert l["mean_iterations"] < p["mean_iterations"]
     assert p["max_balance_residual"] < 1e-9 and l["max_balance_residual"] < 1e-9
+    # the two forms take the same first substeps, so the same substep paths
+    assert l2["mean_iterations"] == l["mean_iterations"]
+    assert l2["rejected_rate"] == l["rejected_rate"]
 
 
 @pytest.mark.parametrize("persistent", [True, False])
