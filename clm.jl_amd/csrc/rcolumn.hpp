@@ -50,7 +50,8 @@ __device__ __forceinline__ double prv(const double *x, int i, int k) {
     return group_prev<G>(k == G - 1 ? dn : x[i]);
 }
 
-template <int G, bool EXP = false, bool REC = false, int LCS = 0, int LCF = 7>
+template <int G, bool EXP = false, bool REC = false, int LCS = 0, int LCF = 7,
+          bool GRID = false>
 struct RColumn {
     static constexpr int NMAX = MAX_NLEVSOI;
     static constexpr int S = (NMAX + G - 1) / G;
@@ -94,6 +95,16 @@ struct RColumn {
         else return den[i];
     }
 
+    // GRID: the CLM5 grid's level counts (12 snow, 20 soil) as compile-time constants
+    __device__ __forceinline__ static int nsoi(const RichardsParams &P) {
+        if constexpr (GRID) return MAX_NLEVSOI;
+        else return P.nlevsoi;
+    }
+    __device__ __forceinline__ static int nsno(const RichardsParams &P) {
+        if constexpr (GRID) return MAX_NLEVSNO;
+        else return P.nlevsno;
+    }
+
     __device__ __forceinline__ int64_t soil(int j) const { return c + (int64_t)(j - 1) * ld_; }
     int64_t ld_;
 
@@ -102,29 +113,30 @@ struct RColumn {
         c = col;
         ld_ = P.ld;
         const int64_t ld = P.ld;
-        const int joff = P.nlevsno, joff_zi = P.nlevsno + 1;
+        const int nlevsoi = nsoi(P);
+        const int joff = nsno(P), joff_zi = nsno(P) + 1;
         n = (int)RP(nbedrock)[c];
         // water-table index (BC_WATERTABLE, :1346-1350): the first interface jj in
         // 1..nlevsoi with zwt <= zi[jj] gives jwt = jj - 1 (nlevsoi when none).
         // Each lane tests its own interfaces and the group takes the minimum, so
         // every load is in flight at once (a refill stalls the whole wavefront).
-        jwt = P.nlevsoi;
+        jwt = nlevsoi;
         zwt = 0.0;
         if (P.lower_bc == CLMSOIL_BC_WATERTABLE || P.aquifer_pass) {
             zwt = RP(zwt)[c];
-            int jf = P.nlevsoi;
+            int jf = nlevsoi;
             const double *p_zi = RP(zi);
 #pragma unroll
             for (int i = 0; i < S; ++i) {
                 const int jj = i * G + k + 1;
-                if (jj <= P.nlevsoi) {
+                if (jj <= nlevsoi) {
                     const double zij = p_zi[c + (int64_t)(joff_zi + jj - 1) * ld];
                     if (zwt <= zij && jj - 1 < jf) jf = jj - 1;
                 }
             }
             jwt = group_min_i<G>(jf);
         }
-        if (n < 1 || n > P.nlevsoi) {
+        if (n < 1 || n > nlevsoi) {
             if (k == 0) P.flag((unsigned)DEV_ERR_NBEDROCK);
             return false;
         }
@@ -140,7 +152,7 @@ struct RColumn {
                    zj1 = 0.0, bs = 1.0, su = 0.0, rt = 0.0;
             // every level of the soil grid is read, above bedrock or not: the loads
             // then need only the column index and are all in flight at once
-            if (j <= P.nlevsoi) {
+            if (j <= nlevsoi) {
                 const int64_t o = soil(j);
                 const int64_t os = c + (int64_t)(joff + j - 1) * ld;
                 lq = p_liq[os];
@@ -149,7 +161,7 @@ struct RColumn {
                 hks = __builtin_nontemporal_load(&p_hks[o]);
                 icf = __builtin_nontemporal_load(&p_icf[o]);
                 zj = __builtin_nontemporal_load(&p_z[os]);
-                if (j < P.nlevsoi) {
+                if (j < nlevsoi) {
                     icf1 = __builtin_nontemporal_load(&p_icf[o + ld]);
                     zj1 = __builtin_nontemporal_load(&p_z[os + ld]);
                 }
@@ -506,7 +518,7 @@ struct RColumn {
 
     __device__ __forceinline__ void store(const RichardsParams &P, int k) {
         const int64_t ld = P.ld;
-        const int joff = P.nlevsno;
+        const int joff = nsno(P);
         const double dtime = P.dtime;
         {
         double *p_liq = RP(h2osoi_liq), *p_smp = RP(smp_l), *p_hk = RP(hk_l);
@@ -522,7 +534,7 @@ struct RColumn {
         }
         // liquid and thickness of layer jwt+1, read from its owner lane's registers
         double liq_w = 0.0, mdz_w = M_TO_MM;
-        if (P.aquifer_pass && jwt < P.nlevsoi && jwt + 1 <= n) {
+        if (P.aquifer_pass && jwt < nsoi(P) && jwt + 1 <= n) {
             const int jw1 = jwt + 1;
 #pragma unroll
             for (int i = 0; i < S; ++i)
@@ -532,7 +544,7 @@ struct RColumn {
         }
         if (k == 0) {
             double qcharge = qc;
-            if (P.aquifer_pass && jwt < P.nlevsoi) {
+            if (P.aquifer_pass && jwt < nsoi(P)) {
                 // compute_qcharge! with zero smp/imped (:1153-1217 via :1575-1598)
                 const int jw1 = jwt + 1;
                 const int64_t o = c + (int64_t)(jw1 - 1) * ld;

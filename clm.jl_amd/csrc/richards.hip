@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_kerne
 
 // Persistent: a resident grid whose groups refill from per-XCD column ranges.
 // Every wavefront leaves the loop once all eight ranges are exhausted.
-template <int G, bool EXP, int LCS, bool LEARN, int LCF = 7>
+template <int G, bool EXP, int LCS, bool LEARN, int LCF = 7, bool GRID = false>
 __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persistent(RichardsParams P) {
     fill_math_tables();
     const int k = (int)(threadIdx.x % G);
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256, CLMSOIL_RICHARDS_WAVES) void richards_mf_persi
     const int xcd = (int)(blockIdx.x % NUM_XCD);
     const int64_t chunk = (P.nc + NUM_XCD - 1) / NUM_XCD;
     unsigned *ctrs = P.work_counters;
-    RColumn<G, EXP, false, LCS, LCF> col;
+    RColumn<G, EXP, false, LCS, LCF, GRID> col;
     if constexpr (LCS > 0) {
         extern __shared__ double richards_lds[];
         col.lc = richards_lds + threadIdx.x;
@@ -191,14 +191,14 @@ static hipError_t launch_g(const RichardsParams &P, hipStream_t stream, int bloc
 
 // Resident blocks per CU of richards_mf_persistent at a block size, from the
 // occupancy calculator, cached per (device, kernel, block) after the first query.
-template <int G, bool EXP, int LCS, bool LEARN, int LCF = 7>
+template <int G, bool EXP, int LCS, bool LEARN, int LCF = 7, bool GRID = false>
 static int resident_blocks(int dev, int block) {
     static std::atomic<int> cache[16][3];
     const int bi = block == 64 ? 0 : (block == 128 ? 1 : 2);
     if (dev < 0 || dev >= 16) dev = 0;
     int nb = cache[dev][bi].load(std::memory_order_relaxed);
     if (nb > 0) return nb;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G, EXP, LCS, LEARN, LCF>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, richards_mf_persistent<G, EXP, LCS, LEARN, LCF, GRID>,
                                                      block, richards_lds_bytes<G, EXP, LCS, LCF>()) !=
         hipSuccess)
         return -1;
@@ -207,7 +207,7 @@ static int resident_blocks(int dev, int block) {
     return nb;
 }
 
-template <int G, bool EXP, int LCS, bool LEARN = false, int LCF = 7>
+template <int G, bool EXP, int LCS, bool LEARN = false, int LCF = 7, bool GRID = false>
 static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int block,
                            int blocks_per_cu) {
     int dev = 0, cus = 0;
@@ -215,13 +215,13 @@ static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int bloc
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return hipErrorUnknown;
     if (richards_lds_bytes<G, EXP, LCS, LCF>() > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)richards_mf_persistent<G, EXP, LCS, LEARN, LCF>,
+        hipError_t e = hipFuncSetAttribute((const void *)richards_mf_persistent<G, EXP, LCS, LEARN, LCF, GRID>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)richards_lds_bytes<G, EXP, LCS, LCF>());
         if (e != hipSuccess) return e;
     }
     int nb = blocks_per_cu;
-    if (nb <= 0) nb = resident_blocks<G, EXP, LCS, LEARN, LCF>(dev, block);
+    if (nb <= 0) nb = resident_blocks<G, EXP, LCS, LEARN, LCF, GRID>(dev, block);
     if (nb <= 0) return hipErrorUnknown;
     // Any grid size is correct: groups only take work from the counters and never
     // wait on each other, so blocks that start late simply find less work.
@@ -229,7 +229,7 @@ static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int bloc
     grid = (grid + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     const int64_t need = (P.nc * G + block - 1) / block;
     if (grid > need) grid = (need + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
-    hipLaunchKernelGGL((richards_mf_persistent<G, EXP, LCS, LEARN, LCF>), dim3((unsigned)grid), dim3(block),
+    hipLaunchKernelGGL((richards_mf_persistent<G, EXP, LCS, LEARN, LCF, GRID>), dim3((unsigned)grid), dim3(block),
                        (richards_lds_bytes<G, EXP, LCS, LCF>()), stream, P);
     return hipGetLastError();
 }
@@ -246,8 +246,11 @@ static hipError_t launch_exp(const RichardsParams &P, int G, int block, bool per
             case 2: return launch_p<2, EXP, 0>(P, stream, block, blocks_per_cu);
             default:
                 if (!cached) return launch_p<4, EXP, 0>(P, stream, block, blocks_per_cu);
-                return lds_fields == 5 ? launch_p<4, EXP, 256, false, 5>(P, stream, block, blocks_per_cu)
-                                       : launch_p<4, EXP, 256>(P, stream, block, blocks_per_cu);
+                if (lds_fields == 5) return launch_p<4, EXP, 256, false, 5>(P, stream, block, blocks_per_cu);
+                // the CLM5 grid (12 snow, 20 soil levels) with its level counts compiled in
+                return (P.nlevsoi == MAX_NLEVSOI && P.nlevsno == MAX_NLEVSNO)
+                           ? launch_p<4, EXP, 256, false, 7, true>(P, stream, block, blocks_per_cu)
+                           : launch_p<4, EXP, 256>(P, stream, block, blocks_per_cu);
         }
     }
     switch (G) {

@@ -280,3 +280,25 @@ def test_lds_cache_variants_give_the_same_bits(gpu, fields):
         api.set_richards_lds_cache(7)
         api.set_richards_schedule(True)
     assert api.device_errors() == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("aquifer", [False, True])
+def test_small_layout_vs_oracle(gpu, aquifer):
+    """The reference tests' smaller layout (5 snow, 8 soil, 10 ground levels) runs the
+    kernels whose level counts are read at run time (the CLM5 grid has its own,
+    with the counts compiled in): same bits as the oracle, water table and aquifer
+    pass included."""
+    d = inputs.richards_config0(nc=2048, seed=33, nlevsno=5, nlevsoi=8, nlevgrnd=10,
+                                nbedrock="random", lower_bc=api.BC_WATERTABLE)
+    d["nrejected"] = np.full(d["nc"], np.nan)
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_WATERTABLE)
+    ref = clone(d)
+    if aquifer:
+        oracle.soilwater_moisture_form(ref, cfg.to_abi())
+    else:
+        oracle.soilwm_moisture_form_solve(ref, cfg.to_abi())
+    got = run_gpu(d, cfg, aquifer=aquifer)
+    compare(got, ref, d)
+    assert np.array_equal(got["nrejected"], ref["nrejected"], equal_nan=True)
