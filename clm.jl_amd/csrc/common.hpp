@@ -77,14 +77,19 @@ __device__ __forceinline__ double log10_w(double x) {
 // SGPRs the compiler spills them to VGPR lanes and reads every use back with two
 // v_readlane. OFF is the field's byte offset in the block, which must be the
 // kernel's first argument.
+// The value is cast through a global-address-space pointer: an asm result has no
+// known address space, and accesses through a generic pointer compile to flat
+// instructions, which also count against lgkmcnt (so every LDS, DPP or scalar
+// wait waits for them too) and take a full 64-bit VGPR address.
 template <typename T, unsigned OFF>
 __device__ __forceinline__ T karg_ptr() {
     static_assert(sizeof(T) == 8 && OFF % 4 == 0, "64-bit field at a dword offset");
-    T v;
+    uint64_t v;
     asm volatile("s_load_dwordx2 %0, %1, %2\n\ts_waitcnt lgkmcnt(0)"
                  : "=s"(v)
                  : "s"(__builtin_amdgcn_kernarg_segment_ptr()), "i"(OFF));
-    return v;
+    using E = __attribute__((address_space(1))) char;
+    return (T)(char *)(E *)v;
 }
 
 // Column-major [c, j] access with 1-based level j (the reference's index).
