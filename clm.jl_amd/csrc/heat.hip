@@ -730,12 +730,38 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     const double *p_ws = HP(watsat), *p_bsw = HP(bsw), *p_suc = HP(sucsat), *p_dz = HP(dz);
     double *p_fact = HP(fact), *p_fgr = HP(eflx_fgr), *p_exs = HP(exice_subs);
     int64_t *p_imelt = HP(imelt);
+    // Soil properties of a level, read again (unchanged in memory), and its
+    // eflx_fgr (written before the solve): issued one slot ahead of the slot's
+    // arithmetic, as in the property phase.
+    struct PcIn {
+        double exc, wsat, bsw, suc, dz, fgr;
+    };
+    auto pc_load = [&](int i) {
+        const int jj = i * G + k + 1;
+        const int j = jj - nlevsno;
+        PcIn v{0.0, 0.0, 0.0, 0.0, 1.0, 0.0};
+        if (jj >= jtop && jj <= nlev && j >= 1) {
+            const int64_t os = c + (int64_t)(j - 1) * ld;
+            v.exc = p_exc[os];
+            v.wsat = p_ws[os];
+            v.bsw = p_bsw[os];
+            v.suc = p_suc[os];
+            v.dz = p_dz[c + (int64_t)(jj - 1) * ld];
+            if (p_fgr && j < nlevgrnd) v.fgr = p_fgr[os];
+        }
+        return v;
+    };
+    PcIn pc_next = pc_load(0);
 #pragma unroll
     for (int i = 0; i < S; ++i) {
         const int jj = i * G + k + 1;
         const int j = jj - nlevsno;
         const bool act = jj >= jtop && jj <= nlev;
         const int64_t o = c + (int64_t)(jj - 1) * ld;
+        const PcIn cur = pc_next;
+        if (i + 1 < S) pc_next = pc_load(i + 1);
+        // the next slot's loads stay above this point, this slot's arithmetic below
+        __builtin_amdgcn_sched_barrier(0);
         imelt[i] = 0;
         dxmf[i] = qmelt_l[i] = qfrz_l[i] = 0.0;
         tgt_written[i] = false;
@@ -748,16 +774,9 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             }
             continue;
         }
-        // soil properties of this level, read again (unchanged in memory)
         const int64_t os = c + (int64_t)(j - 1) * ld;
-        double exc_i = 0.0, wsat_i = 0.0, bsw_i = 0.0, sucsat_i = 0.0, dz_i = 1.0;
-        if (j >= 1) {
-            exc_i = p_exc[os];
-            wsat_i = p_ws[os];
-            bsw_i = p_bsw[os];
-            sucsat_i = p_suc[os];
-            dz_i = p_dz[o];
-        }
+        const double exc_i = cur.exc, wsat_i = cur.wsat, bsw_i = cur.bsw, sucsat_i = cur.suc,
+                     dz_i = cur.dz;
         const double wice0 = ice[i];
         const double wexice0 = j >= 1 ? exc_i : 0.0;
         const double wmass0 = ice[i] + liq[i] + wexice0;
@@ -873,10 +892,12 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
         if (p_fact) p_fact[o] = fact[i];
         if (p_imelt) p_imelt[o] = imelt[i];
         if (j >= 1) {
-            if (p_fgr && j < nlevgrnd) p_fgr[os] = p_fgr[os] - (1.0 - cnfac) * fn1[i];
+            if (p_fgr && j < nlevgrnd) p_fgr[os] = cur.fgr - (1.0 - cnfac) * fn1[i];
             if (p_exs)
                 p_exs[os] = tgt_written[i] ? jl_max(0.0, (wexice0 - excv) / DENICE) : 0.0;
         }
+        // the loads of the slot after next stay below this point
+        __builtin_amdgcn_sched_barrier(0);
     }
     }
     // column accumulators, summed over levels in the reference's order
@@ -888,28 +909,31 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
             h2osno_nl = group_bcast<G>(h2osno_nl_1, (js1 - 1) % G);
             snow_depth = group_bcast<G>(snow_depth_1, (js1 - 1) % G);
         }
+        // The reference adds a level's terms only where its phase change was
+        // applied (tgt_written). Elsewhere dxmf, qmelt_l and qfrz_l are +0.0, and
+        // adding +0.0 changes no sum that cannot be -0.0: xmf, qflx_snomelt and
+        // qflx_snofrz start at +0.0 (or a max(0, .) quotient) and only gain terms
+        // that are >= +0.0, NaN or (dxmf) of either sign, and a round-to-nearest
+        // sum is -0.0 only when both terms are. So those sums take every level's
+        // term. snomelt_accum is an input (it may hold -0.0): its term is sent as
+        // -1.0 (no max(0, .) quotient is negative) where the reference adds none.
 #pragma unroll
         for (int jj = 1; jj <= NL; ++jj) {
             const int i = (jj - 1) / G, kk = (jj - 1) % G;
-            if (jj < jtop || jj > nlev) continue;
-            const bool own = (k == kk);
-            const double wr = group_bcast<G>(own && tgt_written[i] ? 1.0 : 0.0, kk);
-            if (wr == 0.0) continue;
             if (jj == js1 && ur) {
                 qflx_snomelt = qs;
                 xmf = hfus * qflx_snomelt;
                 qflx_snow_drain = qflx_snomelt;
             }
-            xmf += group_bcast<G>(own ? dxmf[i] : 0.0, kk);
+            xmf += group_bcast<G>(dxmf[i], kk);
             if (jj <= nlevsno) {
-                const double qm = group_bcast<G>(own ? qmelt_l[i] : 0.0, kk);
-                const double qf = group_bcast<G>(own ? qfrz_l[i] : 0.0, kk);
-                const int im = (int)group_bcast<G>(own ? (double)imelt[i] : 0.0, kk);
-                if (im == 1) {
-                    qflx_snomelt += qm;
-                    snomelt_accum += qm * dtime * 1.0e-3;
+                const double qa =
+                    group_bcast<G>(tgt_written[i] && imelt[i] == 1 ? qmelt_l[i] : -1.0, kk);
+                if (!(qa < 0.0)) {
+                    qflx_snomelt += qa;
+                    snomelt_accum += qa * dtime * 1.0e-3;
                 }
-                if (im == 2) qflx_snofrz += qf;
+                qflx_snofrz += group_bcast<G>(qfrz_l[i], kk);
             }
         }
     }
