@@ -126,7 +126,13 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
     // recomputed in each phase rather than kept from the first one: hiding k and
     // jtop from the optimizer at the phase boundaries keeps dozens of 64-bit masks
     // from living through the kernel (where they would be spilled to VGPR lanes).
-#define HEAT_PHASE() asm volatile("" : "+v"(k), "+v"(jtop))
+// The range of k is restated after each boundary, so that with the level counts
+// compiled in, each slot's snow/soil tests still resolve at compile time.
+#define HEAT_PHASE()                                   \
+    do {                                               \
+        asm volatile("" : "+v"(k), "+v"(jtop));        \
+        __builtin_assume(k >= 0 && k < G);             \
+    } while (0)
     const int nbedrock = (int)HP(nbedrock)[c];
     const double dtime = P.dtime;
     const double cnfac = CNFAC, omc = 1.0 - CNFAC;
