@@ -184,6 +184,77 @@ def zero_flux_day(dev, snap, nc, steps):
             "mean_substep_iterations": dev["nsubsteps"].float().mean().item()}
 
 
+def production_hindsight(dev, snap, nc, steps, st, nrej):
+    """The ceiling of any first-substep predictor on the production day: at every
+    step (heat, then Richards) each column's best candidate first substep
+    dtime 2^(-k/2), k = 0..11 (fewest substep iterations), found by trying them
+    all on a copy of the state (untimed). The Richards solve from those first
+    substeps is timed against the plain solve of the same state (both eager);
+    the day follows the plain controller's states."""
+    import torch
+    from clmsoil import api, learned
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_WATERTABLE)
+    dt0 = torch.empty(nc, dtype=torch.float64, device="cuda")
+    mi = {}
+    for tag, d in (("plain", None), ("best", dt0)):
+        mi[tag] = api.MfIn(dev["z"], dev["zi"], dev["dz"], dev["qflx_rootsoi"], dev["watsat"],
+                           dev["hksat"], dev["bsw"], dev["sucsat"], dev["icefrac"], dev["zwt"],
+                           dev["qflx_infl"], dev["smpmin"], d)
+    keys = ("h2osoi_liq", "smp_l", "hk_l", "qcharge", "nsubsteps")
+
+    def solve(tag):
+        api.soilwm_moisture_form_solve(st, mi[tag], None, dev["nbedrock"], 12, 13, 20, 1800.0,
+                                       cfg, 25)
+
+    for k, v in snap.items():
+        dev[k].copy_(v)
+    ev = {"heat": [], "plain": [], "best": []}
+    it = {"plain": torch.zeros((), dtype=torch.float64, device="cuda"),
+          "best": torch.zeros((), dtype=torch.float64, device="cuda")}
+
+    def timed(tag, fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        ev[tag].append((a, b))
+
+    for _ in range(steps):
+        timed("heat", lambda: api.soil_temperature({k: dev[k] for k in HEAT_KEYS} | {"nc": nc},
+                                                   1800.0))
+        save = {k: dev[k].clone() for k in keys}
+        best_n = torch.full((nc,), float("inf"), dtype=torch.float64, device="cuda")
+        best_dt = torch.full((nc,), 1800.0, dtype=torch.float64, device="cuda")
+        for kk in range(learned.N_CANDIDATES):
+            d0 = learned.candidate_dtsub0(1800.0, kk)
+            for k in keys:
+                dev[k].copy_(save[k])
+            dt0.fill_(d0)
+            solve("best")
+            better = dev["nsubsteps"] < best_n
+            best_n = torch.where(better, dev["nsubsteps"], best_n)
+            best_dt = torch.where(better, torch.full_like(best_dt, max(d0, cfg.dtmin)), best_dt)
+        dt0.copy_(best_dt)
+        for k in keys:
+            dev[k].copy_(save[k])
+        timed("best", lambda: solve("best"))
+        it["best"] += dev["nsubsteps"].sum()
+        for k in keys:
+            dev[k].copy_(save[k])
+        timed("plain", lambda: solve("plain"))
+        it["plain"] += dev["nsubsteps"].sum()
+    torch.cuda.synchronize()
+    assert api.device_errors() == 0
+    ms = {t: sum(a.elapsed_time(b) for a, b in e) / steps for t, e in ev.items()}
+    return {"richards_ms_plain": ms["plain"], "richards_ms_best": ms["best"],
+            "heat_ms": ms["heat"],
+            "mean_iterations_plain": float(it["plain"]) / (nc * steps),
+            "mean_iterations_best": float(it["best"]) / (nc * steps),
+            "speedup_richards": ms["plain"] / ms["best"],
+            "speedup_step": (ms["heat"] + ms["plain"]) / (ms["heat"] + ms["best"])}
+
+
 def learned_production_day(dev, snap, nc, steps):
     """The committed first-substep network (trained on config-0 columns) on the
     production day itself (heat + Richards, BC_WATERTABLE): the day as one graph
@@ -235,6 +306,7 @@ def learned_production_day(dev, snap, nc, steps):
                      "mean_iterations": float(iters) / (nc * steps),
                      "rejected_rate": float(rej) / float(iters)}
     out["speedup"] = out["learned"]["column_steps_per_s"] / out["plain"]["column_steps_per_s"]
+    out["hindsight"] = production_hindsight(dev, snap, nc, steps, st, nrej)
     out["workload"] = (f"{nc} production-day columns x {steps} steps of heat + Richards "
                        f"(BC_WATERTABLE), each day one graph launch; the network fused into "
                        f"the Richards kernel; iteration counts from a step-by-step rerun "
