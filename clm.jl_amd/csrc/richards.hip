@@ -220,8 +220,12 @@ static hipError_t launch_p(const RichardsParams &P, hipStream_t stream, int bloc
                                            (int)richards_lds_bytes<G, EXP, LCS, LCF>());
         if (e != hipSuccess) return e;
     }
+    // blocks_per_cu <= 0: as many as are resident; larger than that, the blocks past
+    // the resident ones would start only once the resident ones exit (all work
+    // taken), so the grid is clamped to the resident count
+    const int resident = resident_blocks<G, EXP, LCS, LEARN, LCF, GRID>(dev, block);
     int nb = blocks_per_cu;
-    if (nb <= 0) nb = resident_blocks<G, EXP, LCS, LEARN, LCF, GRID>(dev, block);
+    if (nb <= 0 || (resident > 0 && nb > resident)) nb = resident;
     if (nb <= 0) return hipErrorUnknown;
     // Any grid size is correct: groups only take work from the counters and never
     // wait on each other, so blocks that start late simply find less work.

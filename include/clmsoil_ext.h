@@ -45,7 +45,8 @@ int clmsoil_get_tuning(int *richards_g, int *richards_block, int *heat_g, int *h
  * done; 0: one column per lane group for the launch. Results are identical. */
 int clmsoil_set_richards_schedule(int persistent);
 /* Blocks per CU of the resident Richards grid (0, the default: as many as the
- * occupancy calculator says fit; 1..32 override it). Results are identical. */
+ * occupancy calculator says fit; 1..32 ask for fewer; a request above what fits
+ * is clamped to it). Results are identical. */
 int clmsoil_set_richards_residency(int blocks_per_cu);
 /* Resident Richards grid: a wavefront refills its idle lane groups once
  * n/16 of them are idle (or none is busy), n in 1..16; separately for the plain
