@@ -443,10 +443,9 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
 #pragma unroll
         for (int jj = 1; jj <= NL; ++jj) {
             const int i = (jj - 1) / G, kk = (jj - 1) % G;
-            if (jj < jtop || jj > nlev) continue;
             double ou = 0.0, oy = 0.0, os = 0.0;
             bool piv = false;
-            if (k == kk) {
+            if (k == kk && jj >= jtop && jj <= nlev) {
                 double diag = b3[i], rhs = rv[i];
                 if (jj == js1) {
                     if (jtop <= nlevsno) {                    // pivot column: snow 0
@@ -479,9 +478,8 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
 #pragma unroll
         for (int jj = NL; jj >= 1; --jj) {
             const int i = (jj - 1) / G, kk = (jj - 1) % G;
-            if (jj < jtop || jj > nlev) continue;
             double ox = 0.0;
-            if (k == kk) {
+            if (k == kk && jj >= jtop && jj <= nlev) {
                 double s = y[i];
                 if (jj < nlev) s = s - sup[i] * xn;
                 ox = s / u0[i];

@@ -409,9 +409,8 @@ struct RColumn {
 #pragma unroll
         for (int j = 1; j <= NMAX; ++j) {
             const int i = (j - 1) / G, kk = (j - 1) % G;
-            if (j > n) break;
             double ob = 0.0, od = 0.0;
-            if (k == kk) {
+            if (k == kk && j <= n) {
                 double b2, d;
                 if (j == 1) {
                     b2 = bmx[i];
@@ -435,9 +434,8 @@ struct RColumn {
 #pragma unroll
         for (int j = NMAX; j >= 1; --j) {
             const int i = (j - 1) / G, kk = (j - 1) % G;
-            if (j > n) continue;
             double og = 0.0, od = 0.0;
-            if (k == kk) {
+            if (k == kk && j <= n) {
                 if (j < n) dwat[i] = dwat[i] - gn * dn;
                 og = gam[i];
                 od = dwat[i];
