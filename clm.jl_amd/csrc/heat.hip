@@ -443,7 +443,6 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
 #pragma unroll
         for (int jj = 1; jj <= NL; ++jj) {
             const int i = (jj - 1) / G, kk = (jj - 1) % G;
-            double ou = 0.0, oy = 0.0, os = 0.0;
             bool piv = false;
             if (k == kk && jj >= jtop && jj <= nlev) {
                 double diag = b3[i], rhs = rv[i];
@@ -466,11 +465,12 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
                 }
                 u0[i] = diag;
                 y[i] = rhs;
-                ou = diag; oy = rhs; os = sup[i];
             }
-            u0p = group_bcast<G>(ou, kk);
-            yp = group_bcast<G>(oy, kk);
-            supp = group_bcast<G>(os, kk);
+            // a broadcast reads only lane kk's slot; after a row outside the column
+            // (above its top level) the carried values are not used
+            u0p = group_bcast<G>(u0[i], kk);
+            yp = group_bcast<G>(y[i], kk);
+            supp = group_bcast<G>(sup[i], kk);
             pivot = pivot || (group_bcast<G>(piv ? 1.0 : 0.0, kk) != 0.0);
             if (jj == nlevsno) { u0_s0 = u0p; y_s0 = yp; b1_s0 = supp; }
         }
@@ -478,14 +478,12 @@ __device__ __forceinline__ void heat_column(const HeatParams &P, const int64_t c
 #pragma unroll
         for (int jj = NL; jj >= 1; --jj) {
             const int i = (jj - 1) / G, kk = (jj - 1) % G;
-            double ox = 0.0;
             if (k == kk && jj >= jtop && jj <= nlev) {
                 double s = y[i];
                 if (jj < nlev) s = s - sup[i] * xn;
-                ox = s / u0[i];
-                x[i] = ox;
+                x[i] = s / u0[i];
             }
-            xn = group_bcast<G>(ox, kk);
+            xn = group_bcast<G>(x[i], kk);   // 0 outside the column, as initialised
             if (jj == js1) x_h = (rv_h - b2_h * xn) / b3_h;
         }
         // Columns whose elimination interchanges rows (the reference's partial

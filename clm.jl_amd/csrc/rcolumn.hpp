@@ -409,7 +409,7 @@ struct RColumn {
 #pragma unroll
         for (int j = 1; j <= NMAX; ++j) {
             const int i = (j - 1) / G, kk = (j - 1) % G;
-            double ob = 0.0, od = 0.0;
+            double ob = 0.0;
             if (k == kk && j <= n) {
                 double b2, d;
                 if (j == 1) {
@@ -424,24 +424,21 @@ struct RColumn {
                     d = (rmx[i] - amx[i] * dwp) / b2;
                 }
                 dwat[i] = d;
-                ob = b2; od = d;
+                ob = b2;
             }
             bet = group_bcast<G>(ob, kk);
-            dwp = group_bcast<G>(od, kk);
+            dwp = group_bcast<G>(dwat[i], kk);
         }
         // ---- back substitution (:1434-1438) ----
         double gn = 0.0, dn = 0.0;
 #pragma unroll
+        // (a broadcast reads only lane kk's slot; rows below the column's last level
+        // hold the zeros they were initialised with)
         for (int j = NMAX; j >= 1; --j) {
             const int i = (j - 1) / G, kk = (j - 1) % G;
-            double og = 0.0, od = 0.0;
-            if (k == kk && j <= n) {
-                if (j < n) dwat[i] = dwat[i] - gn * dn;
-                og = gam[i];
-                od = dwat[i];
-            }
-            gn = group_bcast<G>(og, kk);
-            dn = group_bcast<G>(od, kk);
+            if (k == kk && j < n) dwat[i] = dwat[i] - gn * dn;
+            gn = group_bcast<G>(gam[i], kk);
+            dn = group_bcast<G>(dwat[i], kk);
         }
         // ---- error estimate (:1441-1457) ----
         double errorMax = 0.0;
