@@ -88,6 +88,42 @@ static double jl_pow(double x, double y) {
 
 double orc_ch4_jl_pow(double x, double y) { return jl_pow(x, y); }
 
+/* Diffusivity of species s (0 CH4, 1 O2) at a level of temperature t (K) and
+ * porosity ws (:1818-1840): gaseous at or above the water table (gas != 0; the
+ * Millington-Quirk organic term and the Moldrup mineral term, weighted by the
+ * organic fraction, vol_min the level's water content capped at ws), aqueous
+ * below it (ws^satpow times the temperature polynomial, reduced by the liquid
+ * share of the water when frozen); never below 1e-12. */
+static double ch4_diffusivity(const clmsoil_ch4_tran_params *p, int s, int gas, double t,
+                              double ws, double vol_min, double cellorg, double bsw, double liq,
+                              double ice) {
+    const double sn = 1.0e-12;
+    const double tsc = t - TFRZ_;
+    double d;
+    if (gas) {
+        const double f_a = 1.0 - vol_min / ws;
+        const double e = ws - vol_min;
+        const double om_frac = p->organic_max > 0.0
+                                   ? orc_jl_min(p->om_frac_sf * cellorg / p->organic_max, 1.0)
+                                   : 1.0;
+        d = (D_CON_G_[s][0] + D_CON_G_[s][1] * tsc) * 1.0e-4 *
+            (om_frac * jl_pow(f_a, 10.0 / 3.0) / (ws * ws) +
+             (1.0 - om_frac) * (e * e) * jl_pow(f_a, 3.0 / bsw)) *
+            p->scale_factor_gasdiff;
+    } else {
+        d = jl_pow(ws, p->satpow) *
+            (D_CON_W_[s][0] + D_CON_W_[s][1] * tsc + D_CON_W_[s][2] * (tsc * tsc)) * 1.0e-9 *
+            p->scale_factor_liqdiff;
+        if (t <= TFRZ_) d *= (liq / DENH2O_ + sn) / (liq / DENH2O_ + ice / DENICE_ + sn);
+    }
+    return orc_jl_max(d, sn);
+}
+
+double orc_ch4_diffusivity(const clmsoil_ch4_tran_params *p, int s, int gas, double t, double ws,
+                           double vol_min, double cellorg, double bsw, double liq, double ice) {
+    return ch4_diffusivity(p, s, gas, t, ws, vol_min, cellorg, bsw, liq, ice);
+}
+
 int orc_ch4_tran(const clmsoil_ch4_tran_args *a, double dtime, const clmsoil_ch4_tran_params *p) {
     const int nl = p->nlevsoi;
     if (nl < 1 || nl > CH4_MAXLEV - 4) return CLMSOIL_ERR_SIZE;
@@ -238,31 +274,10 @@ int orc_ch4_tran(const clmsoil_ch4_tran_args *a, double dtime, const clmsoil_ch4
             spec_grnd_cond[s] = 1.0 / (1.0 / a->grnd_ch4_cond[c] + snowres + pondres);
 
             /* gas / liquid diffusivity (:1818-1840) */
-            for (int j = 1; j <= nl; ++j) {
-                const double tsc = AT(a->t_soisno, j) - TFRZ_;
-                double d;
-                if (j <= jwt) {
-                    const double ws = AT(a->watsat, j);
-                    const double f_a = 1.0 - vol_min[j] / ws;
-                    const double e = ws - vol_min[j];
-                    const double om_frac = p->organic_max > 0.0
-                                               ? orc_jl_min(p->om_frac_sf * AT(a->cellorg, j) / p->organic_max, 1.0)
-                                               : 1.0;
-                    d = (D_CON_G_[s][0] + D_CON_G_[s][1] * tsc) * 1.0e-4 *
-                        (om_frac * jl_pow(f_a, 10.0 / 3.0) / (ws * ws) +
-                         (1.0 - om_frac) * (e * e) * jl_pow(f_a, 3.0 / AT(a->bsw, j))) *
-                        p->scale_factor_gasdiff;
-                } else {
-                    const double e = AT(a->watsat, j);
-                    d = jl_pow(e, p->satpow) *
-                        (D_CON_W_[s][0] + D_CON_W_[s][1] * tsc + D_CON_W_[s][2] * (tsc * tsc)) * 1.0e-9 *
-                        p->scale_factor_liqdiff;
-                    if (AT(a->t_soisno, j) <= TFRZ_)
-                        d *= (AT(a->h2osoi_liq, j) / DENH2O_ + sn) /
-                             (AT(a->h2osoi_liq, j) / DENH2O_ + AT(a->h2osoi_ice, j) / DENICE_ + sn);
-                }
-                diffus[j] = orc_jl_max(d, sn);
-            }
+            for (int j = 1; j <= nl; ++j)
+                diffus[j] = ch4_diffusivity(p, s, j <= jwt, AT(a->t_soisno, j), AT(a->watsat, j),
+                                            vol_min[j], AT(a->cellorg, j), AT(a->bsw, j),
+                                            AT(a->h2osoi_liq, j), AT(a->h2osoi_ice, j));
 
             /* dm1_zm1 / dp1_zp1 (:1843-1871) */
             for (int j = 1; j <= nl; ++j) {

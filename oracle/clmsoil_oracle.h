@@ -152,6 +152,10 @@ int orc_ch4_tran(const clmsoil_ch4_tran_args *a, double dtime, const clmsoil_ch4
 void orc_set_ch4_jitter(uint64_t seed);
 void orc_ch4_debug(double *err_raw, double *rows, double *khcc);
 double orc_ch4_jl_pow(double x, double y);
+/* the transport's diffusivity of species s (0 CH4, 1 O2) at one level; gas != 0
+ * at or above the water table (ch4_oracle.c, :1818-1840) */
+double orc_ch4_diffusivity(const clmsoil_ch4_tran_params *p, int s, int gas, double t, double ws,
+                           double vol_min, double cellorg, double bsw, double liq, double ice);
 /* Test hook: a non-zero seed moves every pow() of the Zeng-Decker step one ulp
  * (direction from a hash of the operands and the seed); 0 restores exact pow(). */
 void orc_set_zd_pow_jitter(uint64_t seed);

@@ -271,6 +271,20 @@ def ch4_jl_pow(x: float, y: float) -> float:
     return float(L.orc_ch4_jl_pow(float(x), float(y)))
 
 
+def ch4_diffusivity(species: int, gas: bool, t: float, watsat: float, vol_min: float,
+                    cellorg: float = 0.0, bsw: float = 5.0, liq: float = 0.0, ice: float = 0.0,
+                    **params) -> float:
+    """The methane transport's diffusivity (m2/s) of `species` (0 CH4, 1 O2) at one
+    level: gaseous (`gas`, at or above the water table) or aqueous."""
+    L = lib()
+    L.orc_ch4_diffusivity.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 7
+    L.orc_ch4_diffusivity.restype = C.c_double
+    params.setdefault("nlevsoi", 1)
+    return float(L.orc_ch4_diffusivity(byref(ch4_tran_params(**params)), int(species), int(bool(gas)),
+                                       float(t), float(watsat), float(vol_min), float(cellorg),
+                                       float(bsw), float(liq), float(ice)))
+
+
 def jl_pow5(x: float) -> float:
     L = lib()
     L.orc_jl_pow5.argtypes = [C.c_double]

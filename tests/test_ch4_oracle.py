@@ -13,7 +13,12 @@ the mathematics fix:
 * Henry's-law solubilities at 25 C are the tabulated ones (CH4 0.0342, O2
   0.0318 dimensionless) and rise as the water cools;
 * the O2/CH4 competition rules on hand-computed single-level cases;
-* Julia's Float64 ^ with integer-valued exponents (x^2.0 is x*x).
+* Julia's Float64 ^ with integer-valued exponents (x^2.0 is x*x);
+* the gas and aqueous diffusivities (orc_ch4_diffusivity, the transport's own
+  routine) on states where the reference's closed form reduces to a number fixed
+  by hand: the organic term at f_a = 1 and 1/8, the mineral term at b = 3 and 1.5,
+  the organic weighting, the temperature polynomials of both species, the scale
+  factors, the frozen liquid share and the 1e-12 floor.
 """
 from fractions import Fraction
 
@@ -191,3 +196,76 @@ def test_bad_jwt_rejected():
     d["jwt"][7] = d["nlevsoi"] + 1
     with pytest.raises(RuntimeError):
         run(d)
+
+
+# ---- diffusivities (methane.jl:1818-1840; D_CON_G, D_CON_W from varcon.jl) ----
+# The gas diffusivity at or above the water table is
+#   (g0 + g1 tsc) 1e-4 [om f_a^(10/3) / ws^2 + (1 - om) eps^2 f_a^(3/b)] sfgas,
+# f_a = 1 - vol/ws, eps = ws - vol, om = min(om_frac_sf cellorg / organic_max, 1);
+# below it, ws^satpow (w0 + w1 tsc + w2 tsc^2) 1e-9 sfliq, times the liquid share
+# (liq/1000 + sn) / (liq/1000 + ice/917 + sn) when frozen; never below sn = 1e-12.
+# Each case below picks a state where that form reduces to a number fixed by
+# hand, so a dropped term, a swapped constant or a wrong exponent moves it.
+G_CH4, G_O2 = (0.1875, 0.0013), (0.1759, 0.00117)          # varcon.jl D_CON_G
+W_CH4, W_O2 = (0.9798, 0.02986, 0.0004381), (1.172, 0.03443, 0.0005048)   # D_CON_W
+TFRZ = 273.15
+
+
+def _rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+def test_gas_diffusivity_organic_term():
+    # om = 1 (cellorg above organic_max): f_a = 1 gives g0 1e-4 / ws^2
+    d = oracle.ch4_diffusivity(0, True, TFRZ, 0.5, 0.0, cellorg=200.0)
+    assert _rel(d, 0.1875e-4 / 0.25) < 1e-15
+    # f_a = 1/8 (vol = 7/8 ws): f_a^(10/3) = 2^-10 (3 or 3/10 as the exponent give 2^-9, 2^-0.9)
+    d = oracle.ch4_diffusivity(0, True, TFRZ, 0.5, 0.4375, cellorg=200.0)
+    assert _rel(d, 0.1875e-4 * 2.0 ** -10 / 0.25) < 1e-14
+    # organic_max = 0 means all organic (om = 1) whatever cellorg
+    assert oracle.ch4_diffusivity(0, True, TFRZ, 0.5, 0.4375, cellorg=0.0, organic_max=0.0) == d
+
+
+def test_gas_diffusivity_mineral_term():
+    # om = 0 (cellorg 0): eps^2 f_a^(3/b); b = 3 gives f_a, b = 1.5 gives f_a^2
+    # (b/3 as the exponent would give f_a^0.5)
+    d3 = oracle.ch4_diffusivity(0, True, TFRZ, 0.5, 0.25, cellorg=0.0, bsw=3.0)
+    assert _rel(d3, 0.1875e-4 * 0.0625 * 0.5) < 1e-15
+    d15 = oracle.ch4_diffusivity(0, True, TFRZ, 0.5, 0.25, cellorg=0.0, bsw=1.5)
+    assert _rel(d15, 0.1875e-4 * 0.0625 * 0.25) < 1e-14
+    # a half-organic level is the mean of the two terms (om = 65 / 130)
+    d_org = oracle.ch4_diffusivity(0, True, TFRZ, 0.5, 0.25, cellorg=200.0, bsw=3.0)
+    d_half = oracle.ch4_diffusivity(0, True, TFRZ, 0.5, 0.25, cellorg=65.0, bsw=3.0)
+    assert _rel(d_half, 0.5 * (d_org + d3)) < 1e-15
+
+
+def test_gas_diffusivity_temperature_species_scale():
+    base = dict(cellorg=200.0)
+    for s, (g0, g1) in ((0, G_CH4), (1, G_O2)):
+        d0 = oracle.ch4_diffusivity(s, True, TFRZ, 0.5, 0.0, **base)
+        d10 = oracle.ch4_diffusivity(s, True, TFRZ + 10.0, 0.5, 0.0, **base)
+        assert _rel(d0, g0 * 1e-4 / 0.25) < 1e-15
+        assert _rel(d10 / d0, (g0 + 10.0 * g1) / g0) < 1e-12
+        # the scale factor multiplies (exactly, for a power of two)
+        assert oracle.ch4_diffusivity(s, True, TFRZ + 10.0, 0.5, 0.0, scale_factor_gasdiff=2.0,
+                                      **base) == 2.0 * d10
+    # a saturated level above the water table (f_a = 0) sits at the floor
+    assert oracle.ch4_diffusivity(0, True, TFRZ, 0.5, 0.5, cellorg=200.0) == 1e-12
+
+
+def test_aqueous_diffusivity():
+    for s, (w0, w1, w2) in ((0, W_CH4), (1, W_O2)):
+        # thawed: ws^2 (w0 + w1 tsc + w2 tsc^2) 1e-9 at tsc = 10 (satpow 2 is ws*ws)
+        d = oracle.ch4_diffusivity(s, False, TFRZ + 10.0, 0.5, 0.5)
+        assert _rel(d, 0.25 * (w0 + 10.0 * w1 + 100.0 * w2) * 1e-9) < 1e-12
+        # frozen without ice: the liquid share is exactly 1
+        d = oracle.ch4_diffusivity(s, False, TFRZ - 10.0, 0.5, 0.5, liq=1.0, ice=0.0)
+        assert _rel(d, 0.25 * (w0 - 10.0 * w1 + 100.0 * w2) * 1e-9) < 1e-12
+        # frozen, liquid/1000 = ice/917: the share is 1/2 (to sn)
+        d_half = oracle.ch4_diffusivity(s, False, TFRZ - 10.0, 0.5, 0.5, liq=1.0, ice=0.917)
+        assert _rel(d_half, 0.5 * d) < 1e-8
+    # all ice: at the floor
+    assert oracle.ch4_diffusivity(0, False, TFRZ - 10.0, 0.5, 0.5, liq=0.0, ice=10.0) == 1e-12
+    # a non-integer satpow goes through pow, a scale factor multiplies
+    d = oracle.ch4_diffusivity(0, False, TFRZ + 10.0, 0.5, 0.5, satpow=1.7, scale_factor_liqdiff=2.0)
+    assert _rel(d, 0.5 ** 1.7 * (0.9798 + 0.2986 + 0.04381) * 1e-9 * 2.0) < 1e-12
