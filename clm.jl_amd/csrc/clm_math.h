@@ -150,8 +150,9 @@ CLM_MATH_FN clm_dd clm_log_ext_normal(double x) { return clm_log_ext_bits(clm_bi
  * code: out-of-range and NaN arguments are clamped for the evaluation and the
  * result is selected at the end. */
 CLM_MATH_FN double clm_exp_dd(double hi, double lo) {
-    /* above 709.8 the result overflows, below -746 it underflows to zero */
-    const double hc = (hi > 709.8) ? 709.8 : ((hi < -746.0) ? -746.0 : (hi == hi ? hi : 0.0));
+    /* above 709.8 the result overflows, below -746 it underflows to zero; a NaN
+     * (whose result is selected at the end) is evaluated at -746 */
+    const double hc = fmin(fmax(hi, -746.0), 709.8);
     const double lc = (hc == hi) ? lo : 0.0;
     const double kd = rint(hc * CLM_INVLN2N);
     const int ki = (int)kd;   /* |kd| < 2^18 */
