@@ -97,11 +97,18 @@ CLM_MATH_FN double clm_fast_two_sum(double a, double b, double *e) {
  * x = 1, relative 2^-100 in the subinterval around 1): clm_log_ext below.
  * Subnormal x is scaled by 2^52 first (kadj = -52). Straight-line code: no
  * branches. */
+/* The exponent extraction below reads tmp's high word alone. */
+typedef char clm_log_off_low_word_is_zero[(CLM_LOG_OFF & 0xffffffffULL) == 0 ? 1 : -1];
+
 /* Core: ix = the bits of a positive normal double, kadj added to its exponent. */
 CLM_MATH_FN clm_dd clm_log_ext_bits(uint64_t ix, int kadj) {
     const uint64_t tmp = ix - CLM_LOG_OFF;
     const int i = (int)((tmp >> 44) & 255u);
-    const int k = (int)((int64_t)tmp >> 52) + kadj;
+    /* the exponent from tmp's high word in 32-bit arithmetic (CLM_LOG_OFF's low
+     * word is zero): (int64_t)tmp >> 52, which would convert to double as an
+     * int64 */
+    const int32_t thi = (int32_t)(uint32_t)(tmp >> 32);
+    const int k = (thi >> 20) + kadj;
     const uint64_t iz = ix - (tmp & 0xfff0000000000000ULL);
     const double z = clm_from_bits(iz);
     const double invc = CLM_LOG_TAB(i, 0);
