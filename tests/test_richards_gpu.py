@@ -283,6 +283,32 @@ def test_lds_cache_variants_give_the_same_bits(gpu, fields):
 
 
 @pytest.mark.gpu
+def test_residency_gives_the_same_bits(gpu):
+    """The resident Richards grid at 1 and 2 blocks per CU and at a request above
+    what fits (clamped to the resident count) against the default: same bits on
+    more ragged production-day columns than one resident grid holds at a time."""
+    import torch
+    from clmsoil import tuning
+    cfg = api.init_soilwater_movement(soilwater_movement_method=api.MOISTURE_FORM,
+                                      lower_boundary_condition=api.BC_WATERTABLE)
+    keys = ("h2osoi_liq", "smp_l", "hk_l", "qcharge", "nsubsteps")
+    out = {}
+    try:
+        for res in (0, 1, 2, 8):
+            api.set_richards_residency(res)
+            dev = tuning.production_state(140000, seed=17)
+            tuning.richards_step(dev, cfg)
+            torch.cuda.synchronize()
+            out[res] = {k: dev[k].cpu().numpy() for k in keys}
+    finally:
+        api.set_richards_residency(0)
+    for res in (1, 2, 8):
+        for k in keys:
+            assert np.array_equal(out[0][k], out[res][k], equal_nan=True), (res, k)
+    assert api.device_errors() == 0
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("aquifer", [False, True])
 def test_small_layout_vs_oracle(gpu, aquifer):
     """The reference tests' smaller layout (5 snow, 8 soil, 10 ground levels) runs the
