@@ -19,7 +19,12 @@ with the plain controller.
 
 Train (on a GPU, from the repository root):
     PYTHONPATH=clm.jl_amd python -m clmsoil.learned --nc 131072 --steps 40 --epochs 8
-which writes the weights that DtPredictor.load() reads by default.
+which writes the weights that DtPredictor.load() reads by default. With
+--workload production_day the data are logged on production-day columns
+(heat + Richards, BC_WATERTABLE) and the weights go to
+dt_predictor_production_day.npz in the working directory; no such weights
+ship, because on held-out production-day columns they did not cut substep
+iterations.
 """
 from __future__ import annotations
 
@@ -278,6 +283,42 @@ def log_plain_controller(nc: int, steps: int, seed: int, cfg: SoilWaterMovementC
     return X, Y
 
 
+def production_day_state(nc: int, seed: int, device="cuda"):
+    """Columns of the production day (0-12 snow layers, bedrock index 5-20,
+    water table 0.5-10 m, heat and Richards inputs) as device tensors."""
+    import torch
+    from .tuning import production_state
+    dev = production_state(nc, seed)
+    dev["nrejected"] = torch.zeros(nc, dtype=torch.float64, device=device)
+    return dev
+
+
+def log_production_day(nc: int, steps: int, seed: int, cfg: SoilWaterMovementConfig,
+                       device="cuda", progress=None):
+    """Features and labels as log_plain_controller, logged on production-day
+    columns: each step runs the heat step, logs the Richards state it leaves
+    (features, best candidate first substep) and advances it with the plain
+    controller (BC_WATERTABLE in `cfg`)."""
+    import torch
+    from .tuning import heat_step
+    dev = production_day_state(nc, seed, device)
+    dtime = float(dev["dtime"])
+    X = torch.empty((NIN, nc * steps), dtype=torch.float32, device=device)
+    Y = torch.empty(nc * steps, dtype=torch.float32, device=device)
+    work = dict(dev)
+    for s in range(steps):
+        heat_step(dev, dtime)
+        st, mi = mf_views(dev)
+        richards_features(st, mi, None, dev["nbedrock"], dev["nlevsno"], dev["nlevsoi"],
+                          dev["nlevgrnd"], dtime, out=X[:, s * nc:(s + 1) * nc])
+        best_dt, best_n = best_candidate_dtsub0(dev, cfg, work)
+        Y[s * nc:(s + 1) * nc] = torch.log(best_dt / dtime).float()
+        run_step(dev, cfg)   # the plain controller advances the state
+        if progress:
+            progress(s, float(best_n.mean()), float(dev["nsubsteps"].mean()))
+    return X, Y
+
+
 def fit_normalisation(X, Y):
     """norm[50] = mean[24], scale[24], y_mean, y_scale (unit scale for constants)."""
     import torch
@@ -419,29 +460,36 @@ def main(argv=None) -> int:
     import json
     import time
     import torch
-    from .api import init_soilwater_movement, MOISTURE_FORM, BC_ZERO_FLUX
+    from .api import init_soilwater_movement, MOISTURE_FORM, BC_WATERTABLE, BC_ZERO_FLUX
     ap = argparse.ArgumentParser(description="Train the first-substep predictor on the GPU.")
+    ap.add_argument("--workload", choices=("config0", "production_day"), default="config0",
+                    help="columns the training data are logged on: config-0 columns "
+                         "(Richards only, BC_ZERO_FLUX) or production-day columns (heat + "
+                         "Richards, BC_WATERTABLE, ragged snow and bedrock)")
     ap.add_argument("--nc", type=int, default=131072)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--epochs", type=int, default=6)
-    ap.add_argument("--out", default=str(DEFAULT_WEIGHTS))
+    ap.add_argument("--out", default=None,
+                    help="weights file (default: the workload's committed weights)")
     args = ap.parse_args(argv)
+    prod = args.workload == "production_day"
+    out = args.out or ("dt_predictor_production_day.npz" if prod else str(DEFAULT_WEIGHTS))
     cfg = init_soilwater_movement(soilwater_movement_method=MOISTURE_FORM,
-                                  lower_boundary_condition=BC_ZERO_FLUX)
+                                  lower_boundary_condition=BC_WATERTABLE if prod else BC_ZERO_FLUX)
     t0 = time.time()
-    X, Y = log_plain_controller(args.nc, args.steps, args.seed, cfg,
-                                progress=lambda s, b, p: print(
-                                    f"# log step {s}: best-candidate iters {b:.2f}, plain {p:.2f}",
-                                    flush=True))
+    logger = log_production_day if prod else log_plain_controller
+    X, Y = logger(args.nc, args.steps, args.seed, cfg,
+                  progress=lambda s, b, p: print(
+                      f"# log step {s}: best-candidate iters {b:.2f}, plain {p:.2f}", flush=True))
     torch.cuda.synchronize()
     t1 = time.time()
     pred = train(X, Y, epochs=args.epochs,
                  progress=lambda e, l: print(f"# epoch {e}: loss {l:.4f}", flush=True))
     torch.cuda.synchronize()
-    pred.save(args.out)
-    print(json.dumps({"samples": int(X.shape[1]), "log_s": t1 - t0, "train_s": time.time() - t1,
-                      "out": args.out}), flush=True)
+    pred.save(out)
+    print(json.dumps({"workload": args.workload, "samples": int(X.shape[1]), "log_s": t1 - t0,
+                      "train_s": time.time() - t1, "out": out}), flush=True)
     return 0
 
 
