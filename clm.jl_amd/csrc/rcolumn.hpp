@@ -150,20 +150,21 @@ struct RColumn {
             const int j = i * G + k + 1;
             double lq = 0.0, dzv = 1.0, ws = 1.0, hks = 0.0, icf = 0.0, icf1 = 0.0, zj = 0.0,
                    zj1 = 0.0, bs = 1.0, su = 0.0, rt = 0.0;
-            // every level of the soil grid is read, above bedrock or not: the loads
-            // then need only the column index and are all in flight at once
-            if (j <= nlevsoi) {
+            // only the levels above bedrock are read (n is known here: the bedrock
+            // test above has already waited for it), and layer j+1's ice fraction
+            // and depth only where layer j has an interior interface below it
+            if (j <= n) {
                 const int64_t o = soil(j);
                 const int64_t os = c + (int64_t)(joff + j - 1) * ld;
                 lq = p_liq[os];
-                dzv = __builtin_nontemporal_load(&p_dz[os]);
-                ws = __builtin_nontemporal_load(&p_ws[o]);
-                hks = __builtin_nontemporal_load(&p_hks[o]);
-                icf = __builtin_nontemporal_load(&p_icf[o]);
-                zj = __builtin_nontemporal_load(&p_z[os]);
-                if (j < nlevsoi) {
-                    icf1 = __builtin_nontemporal_load(&p_icf[o + ld]);
-                    zj1 = __builtin_nontemporal_load(&p_z[os + ld]);
+                dzv = p_dz[os];
+                ws = p_ws[o];
+                hks = p_hks[o];
+                icf = p_icf[o];
+                zj = p_z[os];
+                if (j < n) {
+                    icf1 = p_icf[o + ld];
+                    zj1 = p_z[os + ld];
                 }
                 if constexpr (LC) {
                     bs = p_bsw[o];
